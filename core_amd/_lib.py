@@ -1,0 +1,104 @@
+"""ctypes binding of the gfx950 solver library (include/irls2d.h).
+
+torch is imported before the library is opened so that the library's libamdhip64 dependency
+resolves to the HIP runtime torch already loaded (same soname): device pointers and stream
+handles then come from one runtime.  There is no fallback: if the library is missing or
+fails to load, every solve raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede the library load, see above)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libirls2d.so")
+
+OK, EINVAL, EUNSUPPORTED, EWORKSPACE, EHIP = 0, 1, 2, 3, 4
+FLAG_GRAPH = 1
+
+EXPORTS = (
+    "irls2d_version",
+    "irls2d_last_error",
+    "irls2d_supported_length",
+    "irls2d_arena_layout",
+    "irls2d_workspace_size",
+    "irls2d_solve",
+    "irls2d_release",
+)
+
+_lib = None
+
+
+class SolverError(RuntimeError):
+    pass
+
+
+def load():
+    """Open libirls2d.so (once) and declare its signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SolverError(f"{LIB_PATH} is missing: build it with `python -m core_amd.build` "
+                          "(this package has no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    L.irls2d_version.restype = ctypes.c_int
+    L.irls2d_last_error.restype = ctypes.c_char_p
+    L.irls2d_supported_length.argtypes = [ctypes.c_int64]
+    L.irls2d_supported_length.restype = ctypes.c_int
+    L.irls2d_arena_layout.argtypes = [ctypes.c_int, i64p, i64p, i64p]
+    L.irls2d_arena_layout.restype = ctypes.c_int64
+    L.irls2d_workspace_size.argtypes = [ctypes.c_int, i64p, i64p, ctypes.POINTER(ctypes.c_size_t)]
+    L.irls2d_workspace_size.restype = ctypes.c_int
+    L.irls2d_solve.argtypes = [
+        ctypes.c_int, i64p, i64p,
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+        ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+        ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint,
+    ]
+    L.irls2d_solve.restype = ctypes.c_int
+    L.irls2d_release.argtypes = [ctypes.c_void_p]
+    L.irls2d_release.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str):
+    if rc != OK:
+        msg = load().irls2d_last_error().decode(errors="replace")
+        if rc == EINVAL:
+            raise ValueError(f"{what}: {msg}")
+        raise SolverError(f"{what} failed (code {rc}): {msg}")
+
+
+def shapes_arrays(shapes):
+    n = len(shapes)
+    H = (ctypes.c_int64 * n)(*[int(h) for h, _ in shapes])
+    W = (ctypes.c_int64 * n)(*[int(w) for _, w in shapes])
+    return n, H, W
+
+
+def arena_layout(shapes):
+    """Element offsets of each image in an arena and the arena length."""
+    L = load()
+    n, H, W = shapes_arrays(shapes)
+    off = (ctypes.c_int64 * n)()
+    total = L.irls2d_arena_layout(n, H, W, off)
+    if total < 0:
+        raise ValueError(f"bad shapes {shapes}")
+    return [int(o) for o in off], int(total)
+
+
+def workspace_size(shapes) -> int:
+    L = load()
+    n, H, W = shapes_arrays(shapes)
+    b = ctypes.c_size_t(0)
+    check(L.irls2d_workspace_size(n, H, W, ctypes.byref(b)), "irls2d_workspace_size")
+    return int(b.value)
+
+
+def supported_length(n: int) -> bool:
+    return bool(load().irls2d_supported_length(int(n)))

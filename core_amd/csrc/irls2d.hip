@@ -1,0 +1,596 @@
+// Host side of the MI355X IRLS unwrap solver: transform plans, workspace layout, the launch
+// sequence of one (batched) solve, hipGraph capture/replay, and the C ABI of include/irls2d.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "irls2d.h"
+#include "kernels.hpp"
+
+namespace irls2d {
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(IRLS2D_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+// ---------------------------------------------------------------------------------------------
+// Plans and tables (double precision on the host, stored as float2 / double)
+// ---------------------------------------------------------------------------------------------
+
+using cd = std::complex<double>;
+
+static bool smooth235(int64_t n) {
+  if (n < 1) return false;
+  for (int p : {2, 3, 5})
+    while (n % p == 0) n /= p;
+  return n == 1;
+}
+
+static std::vector<int> radices(int L) {
+  std::vector<int> r;
+  while (L % 8 == 0) { r.push_back(8); L /= 8; }
+  while (L % 4 == 0) { r.push_back(4); L /= 4; }
+  while (L % 2 == 0) { r.push_back(2); L /= 2; }
+  while (L % 3 == 0) { r.push_back(3); L /= 3; }
+  while (L % 5 == 0) { r.push_back(5); L /= 5; }
+  return r;  // L == 1 for 5-smooth lengths
+}
+
+// Recursive mixed-radix DFT in double (host only; builds the Bluestein kernel spectrum).
+static void host_fft(std::vector<cd> &a) {
+  const size_t L = a.size();
+  if (L <= 1) return;
+  size_t p = 2;
+  while (L % p) ++p;
+  const size_t m = L / p;
+  std::vector<std::vector<cd>> sub(p, std::vector<cd>(m));
+  for (size_t r = 0; r < p; ++r)
+    for (size_t k = 0; k < m; ++k) sub[r][k] = a[k * p + r];
+  for (auto &s : sub) host_fft(s);
+  for (size_t k = 0; k < L; ++k) {
+    cd acc = 0;
+    for (size_t r = 0; r < p; ++r) {
+      const double ang = -2.0 * M_PI * (double)((r * k) % L) / (double)L;
+      acc += sub[r][k % m] * cd(std::cos(ang), std::sin(ang));
+    }
+    a[k] = acc;
+  }
+}
+
+struct HostDct {
+  DctPlan plan;                 // offsets relative to `tab` below
+  std::vector<float2> tab;
+};
+
+static std::mutex g_plan_mu;
+static std::map<int, std::shared_ptr<HostDct>> g_plans;
+
+static int64_t bluestein_len(int64_t N) {
+  int64_t M = 2 * N - 1;
+  while (!smooth235(M)) ++M;
+  return M;
+}
+
+int supported(int64_t N) {
+  if (N < 1) return 0;
+  if (smooth235(N)) return N <= MAX_FFT_LEN;
+  return bluestein_len(N) <= MAX_FFT_LEN;
+}
+
+static std::shared_ptr<HostDct> get_plan(int N) {
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  auto it = g_plans.find(N);
+  if (it != g_plans.end()) return it->second;
+  auto h = std::make_shared<HostDct>();
+  DctPlan &P = h->plan;
+  std::memset(&P, 0, sizeof(P));
+  P.N = N;
+  P.blue = smooth235(N) ? 0 : 1;
+  P.M = P.blue ? (int)bluestein_len(N) : N;
+  P.invN = (float)(1.0 / (double)N);
+  auto push = [&](const std::vector<cd> &v) {
+    int off = (int)h->tab.size();
+    for (const cd &z : v) h->tab.push_back(make_float2((float)z.real(), (float)z.imag()));
+    return off;
+  };
+  const int M = P.M;
+  std::vector<cd> tw(M);
+  for (int m = 0; m < M; ++m) {
+    const double ang = -2.0 * M_PI * (double)m / (double)M;
+    tw[m] = cd(std::cos(ang), std::sin(ang));
+  }
+  P.fft.L = M;
+  P.fft.tw_off = push(tw);
+  auto rad = radices(M);
+  P.fft.npass = (int)rad.size();
+  for (size_t s = 0; s < rad.size(); ++s) P.fft.radix[s] = rad[s];
+  std::vector<cd> mk(N);
+  for (int k = 0; k < N; ++k) {
+    const double ang = -M_PI * (double)k / (2.0 * (double)N);
+    mk[k] = cd(std::cos(ang), std::sin(ang));
+  }
+  P.mk_off = push(mk);
+  if (P.blue) {
+    std::vector<cd> ch(N);
+    for (int64_t n = 0; n < N; ++n) {
+      const int64_t q = (n * n) % (2 * (int64_t)N);
+      const double ang = -M_PI * (double)q / (double)N;
+      ch[n] = cd(std::cos(ang), std::sin(ang));
+    }
+    P.chirp_off = push(ch);
+    std::vector<cd> b(M, cd(0, 0));
+    for (int n = 0; n < N; ++n) b[n] = std::conj(ch[n]);
+    for (int n = 1; n < N; ++n) b[M - n] = std::conj(ch[n]);
+    host_fft(b);
+    for (auto &z : b) z /= (double)M;
+    P.bhat_off = push(b);
+  }
+  g_plans[N] = h;
+  return h;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Batch layout
+// ---------------------------------------------------------------------------------------------
+
+constexpr int NFIELD = 16;  // tx ty wx wy u_prev u_best wxi wyi b r z p0 p1 Ap T Tt
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+struct Layout {
+  int n = 0;
+  std::vector<int64_t> H, W, off;
+  int64_t total = 0;
+  // metadata (host images of the device region)
+  std::vector<ImgDesc> desc;
+  std::vector<int> geo[4];
+  std::vector<DctPlan> plans;
+  std::vector<float2> tab;
+  std::vector<double> dtab;
+  int max_blocks = 0;
+  int row_threads = 64, col_threads = 64;
+  size_t row_lds = 0, col_lds = 0;
+  // byte offsets inside the workspace
+  size_t o_fields = 0, o_valid = 0, o_meta = 0, o_desc = 0, o_state = 0, o_geo[4] = {0, 0, 0, 0},
+         o_plans = 0, o_tab = 0, o_dtab = 0, o_partials = 0, bytes = 0, meta_bytes = 0;
+  uint64_t sig = 0;
+};
+
+static uint64_t shape_sig(int n, const int64_t *H, const int64_t *W) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) { h ^= v; h *= 1099511628211ull; };
+  mix((uint64_t)n);
+  for (int i = 0; i < n; ++i) { mix((uint64_t)H[i]); mix((uint64_t)W[i]); }
+  return h;
+}
+
+static int64_t arena_layout(int n, const int64_t *H, const int64_t *W, int64_t *off) {
+  int64_t tot = 0;
+  for (int i = 0; i < n; ++i) {
+    if (H[i] < 1 || W[i] < 1) return -1;
+    if (off) off[i] = tot;
+    tot += (H[i] * W[i] + 63) / 64 * 64;
+  }
+  return tot;
+}
+
+static int threads_for(int L) {
+  int t = (L + EPT - 1) / EPT;
+  t = (t + 63) / 64 * 64;
+  return std::max(64, std::min(MAX_FFT_THREADS, t));
+}
+
+static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
+  if (n < 1) return fail(IRLS2D_EINVAL, "n must be >= 1");
+  if (!H || !W) return fail(IRLS2D_EINVAL, "null shape arrays");
+  Lo = Layout();
+  Lo.n = n;
+  Lo.H.assign(H, H + n);
+  Lo.W.assign(W, W + n);
+  Lo.off.resize(n);
+  Lo.total = arena_layout(n, H, W, Lo.off.data());
+  if (Lo.total < 0) return fail(IRLS2D_EINVAL, "every image needs H >= 1 and W >= 1");
+  for (int i = 0; i < n; ++i) {
+    if (H[i] > (int64_t)1 << 30 || W[i] > (int64_t)1 << 30) return fail(IRLS2D_EINVAL, "dimension too large");
+    if (!supported(H[i]) || !supported(W[i]))
+      return fail(IRLS2D_EUNSUPPORTED, "grid " + std::to_string(H[i]) + "x" + std::to_string(W[i]) +
+                                           " exceeds the LDS-resident DCT lengths");
+  }
+  Lo.sig = shape_sig(n, H, W);
+  std::map<int, int> plan_idx, tab_base, cos_off;
+  auto add_plan = [&](int N) -> int {
+    auto it = plan_idx.find(N);
+    if (it != plan_idx.end()) return it->second;
+    auto h = get_plan(N);
+    DctPlan P = h->plan;
+    const int base = (int)Lo.tab.size();
+    Lo.tab.insert(Lo.tab.end(), h->tab.begin(), h->tab.end());
+    P.fft.tw_off += base;
+    P.mk_off += base;
+    if (P.blue) { P.chirp_off += base; P.bhat_off += base; }
+    Lo.plans.push_back(P);
+    const int idx = (int)Lo.plans.size() - 1;
+    plan_idx[N] = idx;
+    return idx;
+  };
+  auto add_cos = [&](int N) -> int {
+    auto it = cos_off.find(N);
+    if (it != cos_off.end()) return it->second;
+    const int o = (int)Lo.dtab.size();
+    for (int i = 0; i < N; ++i) Lo.dtab.push_back(std::cos(M_PI * (double)i / (double)N));
+    cos_off[N] = o;
+    return o;
+  };
+  for (int g = 0; g < 4; ++g) Lo.geo[g].assign(1, 0);
+  int maxM_row = 1, maxM_col = 1;
+  for (int i = 0; i < n; ++i) {
+    const int h = (int)H[i], w = (int)W[i];
+    ImgDesc d;
+    d.off = Lo.off[i];
+    d.H = h;
+    d.W = w;
+    d.rowplan = add_plan(w);
+    d.colplan = add_plan(h);
+    d.cosH_off = add_cos(h);
+    d.cosW_off = add_cos(w);
+    Lo.desc.push_back(d);
+    maxM_row = std::max(maxM_row, Lo.plans[d.rowplan].M);
+    maxM_col = std::max(maxM_col, Lo.plans[d.colplan].M);
+    const int64_t tiles = ((int64_t)(h + TILE - 1) / TILE) * ((w + TILE - 1) / TILE);
+    const int64_t ttiles = ((int64_t)(h + TT - 1) / TT) * ((w + TT - 1) / TT);
+    const int64_t geo_add[4] = {tiles, (h + 1) / 2, (w + 1) / 2, ttiles};
+    for (int g = 0; g < 4; ++g) {
+      const int64_t v = (int64_t)Lo.geo[g].back() + geo_add[g];
+      if (v > INT32_MAX) return fail(IRLS2D_EUNSUPPORTED, "too many blocks in one launch");
+      Lo.geo[g].push_back((int)v);
+    }
+  }
+  for (int g = 0; g < 4; ++g) Lo.max_blocks = std::max(Lo.max_blocks, Lo.geo[g].back());
+  Lo.row_threads = threads_for(maxM_row);
+  Lo.col_threads = threads_for(maxM_col);
+  Lo.row_lds = (size_t)maxM_row * sizeof(float2);
+  Lo.col_lds = (size_t)maxM_col * sizeof(float2);
+  // workspace map
+  size_t o = 0;
+  Lo.o_fields = o;
+  o = align_up(o + (size_t)NFIELD * Lo.total * sizeof(float), 256);
+  Lo.o_valid = o;
+  o = align_up(o + (size_t)Lo.total, 256);
+  Lo.o_state = o;
+  o = align_up(o + n * sizeof(ImgState), 256);
+  Lo.o_partials = o;
+  o = align_up(o + (size_t)Lo.max_blocks * MAX_NV * sizeof(double), 256);
+  Lo.o_meta = o;  // everything below is uploaded from the host in one copy
+  Lo.o_desc = o;
+  o = align_up(o + n * sizeof(ImgDesc), 256);
+  for (int g = 0; g < 4; ++g) {
+    Lo.o_geo[g] = o;
+    o = align_up(o + (n + 1) * sizeof(int), 256);
+  }
+  Lo.o_plans = o;
+  o = align_up(o + Lo.plans.size() * sizeof(DctPlan), 256);
+  Lo.o_tab = o;
+  o = align_up(o + Lo.tab.size() * sizeof(float2), 256);
+  Lo.o_dtab = o;
+  o = align_up(o + Lo.dtab.size() * sizeof(double), 256);
+  Lo.bytes = o;
+  Lo.meta_bytes = o - Lo.o_meta;
+  return IRLS2D_OK;
+}
+
+static std::vector<unsigned char> pack_meta(const Layout &Lo) {
+  std::vector<unsigned char> buf(Lo.meta_bytes, 0);
+  auto put = [&](size_t off, const void *src, size_t nbytes) {
+    if (nbytes) std::memcpy(buf.data() + (off - Lo.o_meta), src, nbytes);
+  };
+  put(Lo.o_desc, Lo.desc.data(), Lo.desc.size() * sizeof(ImgDesc));
+  for (int g = 0; g < 4; ++g) put(Lo.o_geo[g], Lo.geo[g].data(), Lo.geo[g].size() * sizeof(int));
+  put(Lo.o_plans, Lo.plans.data(), Lo.plans.size() * sizeof(DctPlan));
+  put(Lo.o_tab, Lo.tab.data(), Lo.tab.size() * sizeof(float2));
+  put(Lo.o_dtab, Lo.dtab.data(), Lo.dtab.size() * sizeof(double));
+  return buf;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Launch sequence
+// ---------------------------------------------------------------------------------------------
+
+struct Params {
+  int max_iter, cg_max_iter;
+  double tol, cg_tol, epsilon;
+};
+
+static Ctx make_ctx(const Layout &Lo, unsigned char *ws, float *u, const Params &p) {
+  Ctx c;
+  std::memset(&c, 0, sizeof(c));
+  c.img = (const ImgDesc *)(ws + Lo.o_desc);
+  c.st = (ImgState *)(ws + Lo.o_state);
+  c.plans = (const DctPlan *)(ws + Lo.o_plans);
+  c.tab = (const float2 *)(ws + Lo.o_tab);
+  c.dtab = (const double *)(ws + Lo.o_dtab);
+  c.partials = (double *)(ws + Lo.o_partials);
+  c.geo.tile = (const int *)(ws + Lo.o_geo[0]);
+  c.geo.rowpair = (const int *)(ws + Lo.o_geo[1]);
+  c.geo.colpair = (const int *)(ws + Lo.o_geo[2]);
+  c.geo.trans = (const int *)(ws + Lo.o_geo[3]);
+  float *F = (float *)(ws + Lo.o_fields);
+  float *fld[NFIELD];
+  for (int k = 0; k < NFIELD; ++k) fld[k] = F + (size_t)k * Lo.total;
+  c.f.tx = fld[0]; c.f.ty = fld[1]; c.f.wx = fld[2]; c.f.wy = fld[3];
+  c.f.u_prev = fld[4]; c.f.u_best = fld[5]; c.f.wxi = fld[6]; c.f.wyi = fld[7];
+  c.f.b = fld[8]; c.f.r = fld[9]; c.f.z = fld[10]; c.f.p0 = fld[11]; c.f.p1 = fld[12];
+  c.f.Ap = fld[13]; c.f.T = fld[14]; c.f.Tt = fld[15];
+  c.f.u = u;
+  c.f.valid = ws + Lo.o_valid;
+  c.n = Lo.n;
+  c.eps_sq = (float)(p.epsilon * p.epsilon);
+  c.tol = p.tol;
+  c.cg_tol = p.cg_tol;
+  return c;
+}
+
+static void precond(const Layout &Lo, const Ctx &c, float *src, float *dst, int dot, int guard, hipStream_t s) {
+  const int grp = Lo.geo[1].back(), gcp = Lo.geo[2].back(), gtr = Lo.geo[3].back();
+  hipLaunchKernelGGL(k_row_dct2<false>, dim3(grp), dim3(Lo.row_threads), Lo.row_lds, s, c, src, c.f.T,
+                     (const float *)nullptr, guard);
+  hipLaunchKernelGGL(k_transpose, dim3(gtr), dim3(256), 0, s, c, c.f.T, c.f.Tt, 0, guard);
+  hipLaunchKernelGGL(k_col, dim3(gcp), dim3(Lo.col_threads), Lo.col_lds, s, c, c.f.Tt, guard);
+  hipLaunchKernelGGL(k_transpose, dim3(gtr), dim3(256), 0, s, c, c.f.Tt, c.f.T, 1, guard);
+  hipLaunchKernelGGL(k_row_dct3, dim3(grp), dim3(Lo.row_threads), Lo.row_lds, s, c, c.f.T, dst, dot, guard);
+}
+
+static void enqueue(const Layout &Lo, const Ctx &c, const float *phase, const float *weight, int32_t *info,
+                    const Params &p, hipStream_t s) {
+  const int gt = Lo.geo[0].back(), grp = Lo.geo[1].back(), gtr = Lo.geo[3].back(), gcp = Lo.geo[2].back();
+  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, c);
+  hipLaunchKernelGGL(k_setup, dim3(gt), dim3(TILE_THREADS), 0, s, c, phase, weight);
+  hipLaunchKernelGGL(k_rhs_static, dim3(gt), dim3(TILE_THREADS), 0, s, c, c.f.r);
+  precond(Lo, c, c.f.r, c.f.u, DOT_NONE, GUARD_NONE, s);
+  hipLaunchKernelGGL(k_center_reduce, dim3(gt), dim3(TILE_THREADS), 0, s, c);
+  hipLaunchKernelGGL(k_center_apply, dim3(gt), dim3(TILE_THREADS), 0, s, c);
+  for (int it = 0; it < p.max_iter; ++it) {
+    hipLaunchKernelGGL(k_irls_mean, dim3(gt), dim3(TILE_THREADS), 0, s, c);
+    hipLaunchKernelGGL(k_irls_update, dim3(gt), dim3(TILE_THREADS), 0, s, c, it);
+    hipLaunchKernelGGL(k_irls_commit, dim3(gt), dim3(TILE_THREADS), 0, s, c, it);
+    hipLaunchKernelGGL(k_cg_init, dim3(gt), dim3(TILE_THREADS), 0, s, c);
+    precond(Lo, c, c.f.r, c.f.z, DOT_RZ_INIT, GUARD_CG, s);
+    for (int i = 0; i < p.cg_max_iter; ++i) {
+      float *pold = (i & 1) ? c.f.p1 : c.f.p0;
+      float *pnew = (i & 1) ? c.f.p0 : c.f.p1;
+      hipLaunchKernelGGL(k_cg_pap, dim3(gt), dim3(TILE_THREADS), 0, s, c, (const float *)pold, pnew);
+      hipLaunchKernelGGL(k_row_dct2<true>, dim3(grp), dim3(Lo.row_threads), Lo.row_lds, s, c, c.f.r, c.f.T,
+                         (const float *)pnew, (int)GUARD_CG);
+      hipLaunchKernelGGL(k_transpose, dim3(gtr), dim3(256), 0, s, c, c.f.T, c.f.Tt, 0, (int)GUARD_CG);
+      hipLaunchKernelGGL(k_col, dim3(gcp), dim3(Lo.col_threads), Lo.col_lds, s, c, c.f.Tt, (int)GUARD_CG);
+      hipLaunchKernelGGL(k_transpose, dim3(gtr), dim3(256), 0, s, c, c.f.Tt, c.f.T, 1, (int)GUARD_CG);
+      hipLaunchKernelGGL(k_row_dct3, dim3(grp), dim3(Lo.row_threads), Lo.row_lds, s, c, c.f.T, c.f.z,
+                         (int)DOT_RZ, (int)GUARD_CG);
+    }
+    hipLaunchKernelGGL(k_irls_check, dim3(gt), dim3(TILE_THREADS), 0, s, c, it);
+    hipLaunchKernelGGL(k_restore, dim3(gt), dim3(TILE_THREADS), 0, s, c, it);
+  }
+  if (info) hipLaunchKernelGGL(k_info, dim3(1), dim3(64), 0, s, c, info);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Caches: layouts per shape signature, uploaded metadata per workspace, graphs per call signature
+// ---------------------------------------------------------------------------------------------
+
+static std::mutex g_mu;
+static std::map<uint64_t, std::shared_ptr<Layout>> g_layouts;
+static std::map<void *, uint64_t> g_uploaded;  // workspace -> signature of the metadata it holds
+
+struct GraphKey {
+  void *ws;
+  const float *phase, *weight;
+  float *out;
+  int32_t *info;
+  uint64_t sig;
+  int max_iter, cg_max_iter;
+  double tol, cg_tol, eps;
+  bool operator<(const GraphKey &o) const {
+    return std::tie(ws, phase, weight, out, info, sig, max_iter, cg_max_iter, tol, cg_tol, eps) <
+           std::tie(o.ws, o.phase, o.weight, o.out, o.info, o.sig, o.max_iter, o.cg_max_iter, o.tol, o.cg_tol,
+                    o.eps);
+  }
+};
+static std::map<GraphKey, hipGraphExec_t> g_graphs;
+static std::map<int, hipStream_t> g_capture_stream;  // per device
+
+static int get_layout(int n, const int64_t *H, const int64_t *W, std::shared_ptr<Layout> &out) {
+  const uint64_t sig = shape_sig(n, H, W);
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_layouts.find(sig);
+    if (it != g_layouts.end() && it->second->n == n &&
+        std::equal(H, H + n, it->second->H.begin()) && std::equal(W, W + n, it->second->W.begin())) {
+      out = it->second;
+      return IRLS2D_OK;
+    }
+  }
+  auto Lo = std::make_shared<Layout>();
+  int rc = build_layout(n, H, W, *Lo);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_layouts.size() > 64) g_layouts.clear();
+  g_layouts[sig] = Lo;
+  out = Lo;
+  return IRLS2D_OK;
+}
+
+static bool g_attr_done[64];
+
+static int set_attrs() {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev >= 0 && dev < 64 && g_attr_done[dev]) return IRLS2D_OK;
+  const int lds = MAX_FFT_LEN * (int)sizeof(float2);  // largest transform buffer (128 KiB)
+  HIP_TRY(hipFuncSetAttribute((const void *)k_row_dct2<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  HIP_TRY(hipFuncSetAttribute((const void *)k_row_dct2<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  HIP_TRY(hipFuncSetAttribute((const void *)k_col, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  HIP_TRY(hipFuncSetAttribute((const void *)k_row_dct3, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  if (dev >= 0 && dev < 64) g_attr_done[dev] = true;
+  return IRLS2D_OK;
+}
+
+int solve(int n, const int64_t *H, const int64_t *W, const float *phase, const float *weight, float *out,
+          int max_iter, double tol, int cg_max_iter, double cg_tol, double epsilon, void *workspace,
+          size_t ws_bytes, int32_t *info, void *stream, unsigned flags) {
+  g_err.clear();
+  if (!phase || !out || !workspace) return fail(IRLS2D_EINVAL, "null phase, output or workspace pointer");
+  if (max_iter < 0 || cg_max_iter < 0) return fail(IRLS2D_EINVAL, "iteration counts must be >= 0");
+  if (!(epsilon >= 0) || !std::isfinite(epsilon)) return fail(IRLS2D_EINVAL, "epsilon must be finite and >= 0");
+  if (std::isnan(tol) || std::isnan(cg_tol)) return fail(IRLS2D_EINVAL, "tol and cg_tol must not be NaN");
+  std::shared_ptr<Layout> Lo;
+  int rc = get_layout(n, H, W, Lo);
+  if (rc) return rc;
+  if (ws_bytes < Lo->bytes)
+    return fail(IRLS2D_EWORKSPACE, "workspace has " + std::to_string(ws_bytes) + " bytes, needs " +
+                                       std::to_string(Lo->bytes));
+  rc = set_attrs();
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  unsigned char *ws = (unsigned char *)workspace;
+  bool need_upload;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_uploaded.find(workspace);
+    need_upload = (it == g_uploaded.end() || it->second != Lo->sig);
+  }
+  if (need_upload) {
+    std::vector<unsigned char> meta = pack_meta(*Lo);
+    HIP_TRY(hipMemcpyAsync(ws + Lo->o_meta, meta.data(), meta.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_uploaded[workspace] = Lo->sig;
+  }
+  Params p{max_iter, cg_max_iter, tol, cg_tol, epsilon};
+  Ctx c = make_ctx(*Lo, ws, out, p);
+  if (!(flags & IRLS2D_FLAG_GRAPH)) {
+    enqueue(*Lo, c, phase, weight, info, p, s);
+    HIP_TRY(hipGetLastError());
+    return IRLS2D_OK;
+  }
+  // Graph path: capture once on a private stream, replay with event ordering against `s`.
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  hipStream_t cs;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_capture_stream.find(dev);
+    if (it == g_capture_stream.end()) {
+      hipStream_t ns;
+      HIP_TRY(hipStreamCreateWithFlags(&ns, hipStreamNonBlocking));
+      g_capture_stream[dev] = ns;
+      cs = ns;
+    } else {
+      cs = it->second;
+    }
+  }
+  GraphKey key{workspace, phase, weight, out, info, Lo->sig, max_iter, cg_max_iter, tol, cg_tol, epsilon};
+  hipGraphExec_t exec = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_graphs.find(key);
+    if (it != g_graphs.end()) exec = it->second;
+  }
+  if (!exec) {
+    hipGraph_t graph;
+    HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    enqueue(*Lo, c, phase, weight, info, p, cs);
+    hipError_t le = hipGetLastError();
+    HIP_TRY(hipStreamEndCapture(cs, &graph));
+    if (le != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(IRLS2D_EHIP, std::string("launch during capture: ") + hipGetErrorString(le));
+    }
+    HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    HIP_TRY(hipGraphDestroy(graph));
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_graphs[key] = exec;
+  }
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreateWithFlags(&e0, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(e0, s));
+  HIP_TRY(hipStreamWaitEvent(cs, e0, 0));
+  HIP_TRY(hipGraphLaunch(exec, cs));
+  HIP_TRY(hipEventRecord(e1, cs));
+  HIP_TRY(hipStreamWaitEvent(s, e1, 0));
+  HIP_TRY(hipEventDestroy(e0));
+  HIP_TRY(hipEventDestroy(e1));
+  return IRLS2D_OK;
+}
+
+int release(void *workspace) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto it = g_graphs.begin(); it != g_graphs.end();) {
+    if (!workspace || it->first.ws == workspace) {
+      (void)hipGraphExecDestroy(it->second);
+      it = g_graphs.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  if (workspace) g_uploaded.erase(workspace); else g_uploaded.clear();
+  return IRLS2D_OK;
+}
+
+}  // namespace irls2d
+
+// ---------------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------------
+
+extern "C" {
+
+int irls2d_version(void) { return 100; }
+
+const char *irls2d_last_error(void) { return irls2d::g_err.c_str(); }
+
+int irls2d_supported_length(int64_t n) { return irls2d::supported(n); }
+
+int64_t irls2d_arena_layout(int n, const int64_t *H, const int64_t *W, int64_t *offsets) {
+  if (n < 1 || !H || !W) return -1;
+  return irls2d::arena_layout(n, H, W, offsets);
+}
+
+int irls2d_workspace_size(int n, const int64_t *H, const int64_t *W, size_t *bytes) {
+  if (!bytes) return irls2d::fail(IRLS2D_EINVAL, "null bytes pointer");
+  std::shared_ptr<irls2d::Layout> Lo;
+  int rc = irls2d::get_layout(n, H, W, Lo);
+  if (rc) return rc;
+  *bytes = Lo->bytes;
+  return IRLS2D_OK;
+}
+
+int irls2d_solve(int n, const int64_t *H, const int64_t *W, const float *phase, const float *weight,
+                 float *unwrapped, int max_iter, double tol, int cg_max_iter, double cg_tol, double epsilon,
+                 void *workspace, size_t workspace_bytes, int32_t *info, void *stream, unsigned flags) {
+  return irls2d::solve(n, H, W, phase, weight, unwrapped, max_iter, tol, cg_max_iter, cg_tol, epsilon, workspace,
+                       workspace_bytes, info, stream, flags);
+}
+
+int irls2d_release(void *workspace) { return irls2d::release(workspace); }
+
+}  // extern "C"

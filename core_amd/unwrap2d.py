@@ -1,0 +1,216 @@
+"""Drop-in replacement of insardev's 2-D IRLS unwrap call on MI355X.
+
+`irls_unwrap_2d` keeps the signature, defaults, return types, warnings and errors of
+insardev/insardev/utils_unwrap2d.py:1082-1766; the solve runs in the gfx950 library
+(include/irls2d.h) through ctypes, with device memory and streams from torch.
+`irls_unwrap_2d_batch` unwraps many interferograms of uneven shapes in one library call
+(one launch per solver step for the whole stack), which is what the reference's dask graph
+does one pair at a time under a process-wide GPU lock (Stack_unwrap2d.py:17-29, :32-71).
+`solve_device` is the device-level entry (tensors already in HBM) used by the benchmark.
+"""
+from __future__ import annotations
+
+import threading
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import postprocess as pp
+
+__all__ = ["irls_unwrap_2d", "irls_unwrap_2d_batch", "solve_device", "resolve_device", "Workspace"]
+
+# The reference serialises GPU solves per process with one lock (Stack_unwrap2d.py:20, :63);
+# the same lock guards the shared workspaces here.
+_gpu_lock = threading.RLock()
+
+
+def resolve_device(device) -> torch.device:
+    """Map the reference's device argument (utils_unwrap2d.py:1173-1186, utils_torch.py:12-62).
+
+    'auto' and 'cuda[:k]' select the HIP device (PyTorch-ROCm names it 'cuda').  This build has
+    only the gfx950 path: 'cpu', 'mps' and 'tpu' raise ValueError instead of falling back.
+    """
+    if isinstance(device, torch.device):
+        dev = device
+    elif isinstance(device, str):
+        if device == 'auto':
+            if not torch.cuda.is_available():
+                raise RuntimeError("irls_unwrap_2d: device='auto' found no HIP GPU; this build has no CPU path")
+            dev = torch.device('cuda', torch.cuda.current_device())
+        elif device in ('cpu', 'mps', 'tpu'):
+            raise ValueError(f"irls_unwrap_2d: device={device!r} is not supported by the MI355X build "
+                             "(use 'cuda' or 'auto')")
+        else:
+            dev = torch.device(device)
+    else:
+        raise ValueError(f"irls_unwrap_2d: bad device {device!r}")
+    if dev.type != 'cuda':
+        raise ValueError(f"irls_unwrap_2d: device {dev} is not a HIP device")
+    if dev.index is None:
+        dev = torch.device('cuda', torch.cuda.current_device())
+    return dev
+
+
+class Workspace:
+    """Device workspace for one set of shapes (reused across calls with the same shapes)."""
+
+    _cache: dict = {}
+
+    def __init__(self, shapes, device):
+        self.shapes = [(int(h), int(w)) for h, w in shapes]
+        self.device = device
+        self.offsets, self.total = _lib.arena_layout(self.shapes)
+        self.nbytes = _lib.workspace_size(self.shapes)
+        self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=device)
+        self.info = torch.zeros(4 * len(self.shapes), dtype=torch.int32, device=device)
+
+    @classmethod
+    def get(cls, shapes, device):
+        key = (device.index, tuple((int(h), int(w)) for h, w in shapes))
+        ws = cls._cache.get(key)
+        if ws is None:
+            if len(cls._cache) >= 4:
+                for k in list(cls._cache)[:len(cls._cache) - 3]:
+                    old = cls._cache.pop(k)
+                    old.release()
+            ws = cls(shapes, device)
+            cls._cache[key] = ws
+        return ws
+
+    def release(self):
+        _lib.check(_lib.load().irls2d_release(self.buf.data_ptr()), "irls2d_release")
+
+    @classmethod
+    def clear(cls):
+        for ws in cls._cache.values():
+            ws.release()
+        cls._cache.clear()
+
+
+def solve_device(phase: torch.Tensor, weight, shapes, max_iter=50, tol=1e-3, cg_max_iter=20, cg_tol=1e-4,
+                 epsilon=1e-2, out: torch.Tensor | None = None, graph: bool = False, workspace=None):
+    """Run the solver on device arenas.
+
+    phase, weight: float32 device tensors holding the images at `Workspace.offsets`
+    (weight may be None).  Returns (u arena tensor, info int32[n, 4] device tensor).
+    """
+    if phase.dtype != torch.float32 or phase.device.type != 'cuda' or not phase.is_contiguous():
+        raise ValueError("phase must be a contiguous float32 HIP tensor")
+    if weight is not None and (weight.dtype != torch.float32 or weight.device != phase.device
+                               or not weight.is_contiguous() or weight.numel() != phase.numel()):
+        raise ValueError("weight must match phase (contiguous float32 on the same device)")
+    L = _lib.load()
+    ws = workspace or Workspace.get(shapes, phase.device)
+    if phase.numel() < ws.total:
+        raise ValueError(f"phase arena has {phase.numel()} elements, layout needs {ws.total}")
+    if out is None:
+        out = torch.empty(ws.total, dtype=torch.float32, device=phase.device)
+    elif out.numel() < ws.total or out.dtype != torch.float32 or not out.is_contiguous():
+        raise ValueError("out arena too small or not contiguous float32")
+    n, H, W = _lib.shapes_arrays(ws.shapes)
+    stream = torch.cuda.current_stream(phase.device).cuda_stream
+    rc = L.irls2d_solve(n, H, W, phase.data_ptr(), weight.data_ptr() if weight is not None else None,
+                        out.data_ptr(), int(max_iter), float(tol), int(cg_max_iter), float(cg_tol),
+                        float(epsilon), ws.buf.data_ptr(), ws.nbytes, ws.info.data_ptr(), stream,
+                        _lib.FLAG_GRAPH if graph else 0)
+    _lib.check(rc, "irls2d_solve")
+    return out, ws.info.view(n, 4)
+
+
+def _pack(arrays, shapes, offsets, total, device):
+    host = torch.empty(total, dtype=torch.float32, pin_memory=True)
+    hv = host.numpy()
+    for a, (h, w), o in zip(arrays, shapes, offsets):
+        hv[o:o + h * w].reshape(h, w)[...] = a
+    return host.to(device, non_blocking=True)
+
+
+def irls_unwrap_2d_batch(phases, weights=None, device='auto', max_iter=50, tol=1e-3, cg_max_iter=20,
+                         cg_tol=1e-4, epsilon=1e-2, conncomp_size=30, debug=False, graph=False):
+    """Unwrap a list of 2-D interferograms in one device solve.
+
+    Each image gets exactly the per-image semantics of `irls_unwrap_2d` (crop, solve, epilogue);
+    returns a list of (unwrapped float32, conncomp uint16) tuples in input order.
+    """
+    phases = list(phases)
+    if weights is None:
+        weights = [None] * len(phases)
+    weights = list(weights)
+    if len(weights) != len(phases):
+        raise ValueError("weights must match phases")
+    dev = resolve_device(device)
+    results = [None] * len(phases)
+    jobs = []
+    for k, (ph, wt) in enumerate(zip(phases, weights)):
+        ph = np.asarray(ph)
+        if ph.ndim != 2:
+            raise ValueError(f"irls_unwrap_2d expects 2D phase array, got shape {ph.shape}")
+        nan_mask, box = pp.crop_box(ph)
+        if box is None:
+            results[k] = (np.full_like(ph, np.nan), np.zeros(ph.shape, np.uint16))
+            continue
+        r0, r1, c0, c1 = box
+        jobs.append((k, ph, None if wt is None else np.asarray(wt), nan_mask[r0:r1, c0:c1], box))
+    if not jobs:
+        return results
+    t0 = time.time()
+    shapes = [(j[4][1] - j[4][0], j[4][3] - j[4][2]) for j in jobs]
+    has_w = [j[2] is not None for j in jobs]
+    with _gpu_lock, torch.cuda.device(dev):
+        ws = Workspace.get(shapes, dev)
+        ph_t = _pack([j[1][j[4][0]:j[4][1], j[4][2]:j[4][3]] for j in jobs], shapes, ws.offsets, ws.total, dev)
+        w_t = None
+        if any(has_w):
+            wl = []
+            for j, hw in zip(jobs, has_w):
+                r0, r1, c0, c1 = j[4]
+                if hw:
+                    wl.append(j[2][r0:r1, c0:c1])
+                else:  # uniform weight 1 - nan (utils_unwrap2d.py:1239-1241) via weight 1 on valid pixels
+                    wl.append(np.ones((r1 - r0, c1 - c0), np.float32))
+            w_t = _pack(wl, shapes, ws.offsets, ws.total, dev)
+        t1 = time.time()
+        u_t, info = solve_device(ph_t, w_t, shapes, max_iter, tol, cg_max_iter, cg_tol, epsilon,
+                                 graph=graph, workspace=ws)
+        u_host = torch.empty(ws.total, dtype=torch.float32, pin_memory=True)
+        u_host.copy_(u_t, non_blocking=True)
+        info_h = info.cpu().numpy().copy()
+        torch.cuda.synchronize(dev)
+        t2 = time.time()
+    uh = u_host.numpy()
+    for (k, ph, wt, nm, box), (h, w), o, inf in zip(jobs, shapes, ws.offsets, info_h):
+        u = uh[o:o + h * w].reshape(h, w).copy()
+        r0, r1, c0, c1 = box
+        ep = pp.epilogue(u, ph[r0:r1, c0:c1], nm, conncomp_size)
+        if ep is None:
+            results[k] = (np.full(ph.shape, np.nan, np.float32), np.zeros(ph.shape, np.uint16))
+            continue
+        results[k] = pp.uncrop(ep[0], ep[1], box, ph.shape)
+        if debug:
+            print(f'  image {k}: {h}x{w} irls_iters={inf[1]} cg_iters={inf[2]} status={inf[0]} valid={inf[3]}')
+    if debug:
+        t3 = time.time()
+        print(f'TIMING irls_unwrap_2d_batch ({len(jobs)} images) on {dev}: upload {(t1 - t0) * 1e3:.1f} ms, '
+              f'solve+download {(t2 - t1) * 1e3:.1f} ms, epilogue {(t3 - t2) * 1e3:.1f} ms')
+    return results
+
+
+def irls_unwrap_2d(phase, weight=None, device='auto', max_iter=50, tol=1e-3,
+                   cg_max_iter=20, cg_tol=1e-4, epsilon=1e-2, conncomp_size=30, debug=False):
+    """Unwrap 2D phase with the DCT-preconditioned IRLS L1 solver on an MI355X.
+
+    Same contract as insardev.utils_unwrap2d.irls_unwrap_2d (utils_unwrap2d.py:1082-1766):
+    returns (unwrapped float32[H, W], conncomp uint16[H, W]); ValueError for non-2D input;
+    all-NaN input gives (NaN, zeros); a non-finite solve warns (RuntimeWarning) and returns
+    (NaN, zeros).
+    """
+    if debug:
+        print(f"DEBUG irls_unwrap_2d: input phase.shape={phase.shape}, "
+              f"weight.shape={weight.shape if weight is not None else None}")
+    if phase.ndim != 2:
+        raise ValueError(f"irls_unwrap_2d expects 2D phase array, got shape {phase.shape}")
+    return irls_unwrap_2d_batch([phase], None if weight is None else [weight], device=device, max_iter=max_iter,
+                                tol=tol, cg_max_iter=cg_max_iter, cg_tol=cg_tol, epsilon=epsilon,
+                                conncomp_size=conncomp_size, debug=debug)[0]

@@ -1,0 +1,99 @@
+/*
+ * irls2d.h -- C ABI of the MI355X (gfx950) DCT-preconditioned IRLS L1 phase-unwrapping solver.
+ *
+ * Reference interface replaced: the device section of
+ *   insardev/insardev/utils_unwrap2d.py:1247-1632  (irls_unwrap_2d: H2D of the filled,
+ *   bbox-cropped phase and weights, targets, edge weights, DCT eigenvalues, L2 DCT
+ *   initialisation, IRLS loop with PCG, u_best tracking)
+ * and, for n > 1 images, the sequential per-pair loop that
+ *   insardev/insardev/Stack_unwrap2d.py:32-71,926-945 (blockwise task + process GPU lock)
+ * runs over an SBAS stack.  The reference has no FFI for this path (it is Python calling
+ * torch); the Python boundary core_amd.irls_unwrap_2d binds these symbols with ctypes.
+ *
+ * Conventions
+ *   - All array arguments are DEVICE pointers (HIP global memory) unless noted.
+ *   - Images are packed in "arenas": image i occupies elements
+ *     [offsets[i], offsets[i] + H[i]*W[i]) in row-major order; offsets come from
+ *     irls2d_arena_layout (each image starts on a 64-element boundary).
+ *   - phase: float32 radians, NaN marks invalid pixels (the crop is done by the caller).
+ *   - weight: float32 quality (NaN allowed) or NULL for uniform weights.
+ *   - unwrapped: float32 output arena (the raw solver output u, before the host epilogue).
+ *   - Every call is stream-ordered on `stream` (a hipStream_t, NULL = default stream) and
+ *     does not synchronise the host except when workspace tables are (re)uploaded.
+ *   - Return value: IRLS2D_OK or an error code; irls2d_last_error() gives the message.
+ */
+#ifndef IRLS2D_H
+#define IRLS2D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IRLS2D_OK 0
+#define IRLS2D_EINVAL 1       /* bad argument (shape, pointer, iteration count) */
+#define IRLS2D_EUNSUPPORTED 2 /* grid dimension beyond what the LDS transforms handle */
+#define IRLS2D_EWORKSPACE 3   /* workspace too small */
+#define IRLS2D_EHIP 4         /* HIP runtime error */
+
+/* flags for irls2d_solve */
+#define IRLS2D_FLAG_GRAPH 1u /* capture the whole launch sequence once in a hipGraph and replay it */
+
+/* info[i*4 + k] written by irls2d_solve (device int32 array of 4*n) */
+#define IRLS2D_INFO_STATUS 0   /* bit0: IRLS hit a non-finite u and restored u_best; bit1: stopped by tol */
+#define IRLS2D_INFO_IRLS_ITERS 1
+#define IRLS2D_INFO_CG_ITERS 2 /* PCG iterations summed over the IRLS loop */
+#define IRLS2D_INFO_VALID 3    /* number of valid (non-NaN) pixels */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int irls2d_version(void);
+
+/* Message of the last error on the calling thread ("" if none). */
+const char *irls2d_last_error(void);
+
+/* Longest grid side the solver handles for a DCT length N (1 = supported). */
+int irls2d_supported_length(int64_t n);
+
+/*
+ * Arena layout.  Host arrays H[n], W[n] in, host offsets[n] out.  Returns the total number of
+ * elements of one arena, or -1 on bad arguments.  No reference counterpart (the reference solves
+ * one numpy array per call); it defines the packing of many uneven images for one launch.
+ */
+int64_t irls2d_arena_layout(int n, const int64_t *H, const int64_t *W, int64_t *offsets);
+
+/*
+ * Workspace bytes needed by irls2d_solve for this set of shapes (host arrays).  The workspace
+ * holds the solver's working fields (targets, edge and IRLS weights, PCG vectors, DCT buffers),
+ * the per-image FFT/DCT plans and twiddle tables, and reduction scratch.
+ */
+int irls2d_workspace_size(int n, const int64_t *H, const int64_t *W, size_t *bytes);
+
+/*
+ * Solve n images.  Replaces utils_unwrap2d.py:1247-1632 (per image) with identical control flow:
+ * L2 DCT start, max_iter IRLS iterations of w/sqrt(r^2+eps^2) clamped to [1e-6, 1e6], each
+ * followed by up to cg_max_iter DCT-preconditioned CG iterations (absolute residual test
+ * cg_tol), relative-change test tol, u_best tracking.  tol = 0 and cg_tol = 0 give fixed
+ * iteration counts.  H, W: host arrays; phase, weight, unwrapped: device arenas laid out by
+ * irls2d_arena_layout; info: device int32[4*n] (may be NULL); workspace: device memory of at
+ * least irls2d_workspace_size bytes, owned by the library for the duration of the call.
+ * Calls that share a workspace must not overlap.
+ */
+int irls2d_solve(int n, const int64_t *H, const int64_t *W,
+                 const float *phase, const float *weight, float *unwrapped,
+                 int max_iter, double tol, int cg_max_iter, double cg_tol, double epsilon,
+                 void *workspace, size_t workspace_bytes, int32_t *info,
+                 void *stream, unsigned flags);
+
+/*
+ * Release cached hipGraphs and plans that refer to `workspace` (NULL = all).  Call before
+ * freeing a workspace that was used with IRLS2D_FLAG_GRAPH.
+ */
+int irls2d_release(void *workspace);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IRLS2D_H */
