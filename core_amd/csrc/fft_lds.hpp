@@ -176,22 +176,29 @@ __device__ __forceinline__ float2 dct_Z(const float2 *buf, const DctPlan *P, con
 
 __device__ __forceinline__ int makhoul_pos(int m, int N) { return (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1); }
 
+// Element m of the two lines into its Makhoul position in LDS (chirp-weighted for Bluestein).
+__device__ __forceinline__ void dct2_pair_put(float2 *buf, const DctPlan *P, const float2 *__restrict__ tab,
+                                              int m, float a, float b) {
+  const int n = makhoul_pos(m, P->N);
+  float2 z = make_float2(a, b);
+  if (P->blue) z = cmul(z, tab[P->chirp_off + n]);
+  buf[n] = z;
+}
+
+// After every element went through dct2_pair_put: zero the Bluestein padding and transform.
+__device__ __forceinline__ void dct2_pair_run(float2 *buf, const DctPlan *P, const float2 *__restrict__ tab) {
+  if (P->blue)
+    for (int n = P->N + threadIdx.x; n < P->M; n += blockDim.x) buf[n] = make_float2(0.f, 0.f);
+  __syncthreads();
+  dct_fft(buf, P, tab);
+}
+
 // DCT-II of two real lines a, b (b may be null = zeros) of length N into LDS spectrum form.
 __device__ __forceinline__ void dct2_pair_load(float2 *buf, const DctPlan *P, const float2 *__restrict__ tab,
                                                const float *a, const float *b) {
-  const int N = P->N, M = P->M, T = blockDim.x;
-  const float2 *chirp = tab + P->chirp_off;
-  const bool blue = P->blue;
-  for (int m = threadIdx.x; m < N; m += T) {
-    float2 z = make_float2(a[m], b ? b[m] : 0.f);
-    const int n = makhoul_pos(m, N);
-    if (blue) z = cmul(z, chirp[n]);
-    buf[n] = z;
-  }
-  if (blue)
-    for (int n = N + threadIdx.x; n < M; n += T) buf[n] = make_float2(0.f, 0.f);
-  __syncthreads();
-  dct_fft(buf, P, tab);
+  const int N = P->N, T = blockDim.x;
+  for (int m = threadIdx.x; m < N; m += T) dct2_pair_put(buf, P, tab, m, a[m], b ? b[m] : 0.f);
+  dct2_pair_run(buf, P, tab);
 }
 
 // After dct2_pair_load: DCT-II coefficient k of both lines.

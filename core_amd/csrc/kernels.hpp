@@ -346,24 +346,32 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) void k_row_dct2(Ctx c, float *in_,
   const int W = d.W;
   const long long oa = d.off + (long long)ra * W;
   double acc[2] = {0.0, 0.0};  // r.r, non-finite x
+  const DctPlan *P = c.plans + d.rowplan;
   if (FUSE_UPDATE) {
-    float *r = in_;
+    // x += a p, r -= a Ap on both rows; the new r goes straight into the LDS transform buffer
     float *x = c.f.u;
     const float *Ap = c.f.Ap;
     const float alpha = (float)st->alpha;
-    const int nrow = hasb ? 2 : 1;
-    for (int m = threadIdx.x; m < nrow * W; m += blockDim.x) {
-      const long long k = oa + m;
-      const float xn = x[k] + alpha * p[k];
-      const float rn = r[k] - alpha * Ap[k];
-      x[k] = xn;
-      r[k] = rn;
-      acc[0] += (double)rn * rn;
-      acc[1] += isfinite(xn) ? 0.0 : 1.0;
+    for (int m = threadIdx.x; m < W; m += blockDim.x) {
+      float rv[2] = {0.f, 0.f};
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (h == 1 && !hasb) break;
+        const long long k = oa + (long long)h * W + m;
+        const float xn = x[k] + alpha * p[k];
+        const float rn = in_[k] - alpha * Ap[k];
+        x[k] = xn;
+        in_[k] = rn;
+        rv[h] = rn;
+        acc[0] += (double)rn * rn;
+        acc[1] += isfinite(xn) ? 0.0 : 1.0;
+      }
+      dct2_pair_put(lds, P, c.tab, m, rv[0], rv[1]);
     }
+    dct2_pair_run(lds, P, c.tab);
+  } else {
+    dct2_pair_load(lds, P, c.tab, in_ + oa, hasb ? in_ + oa + W : nullptr);
   }
-  const DctPlan *P = c.plans + d.rowplan;
-  dct2_pair_load(lds, P, c.tab, in_ + oa, hasb ? in_ + oa + W : nullptr);
   for (int k = threadIdx.x; k < W; k += blockDim.x) {
     float Xa, Xb;
     dct2_pair_coef(lds, P, c.tab, k, Xa, Xb);
