@@ -49,7 +49,7 @@ def load():
     L.irls2d_last_error.restype = ctypes.c_char_p
     L.irls2d_supported_length.argtypes = [ctypes.c_int64]
     L.irls2d_supported_length.restype = ctypes.c_int
-    L.irls2d_arena_layout.argtypes = [ctypes.c_int, i64p, i64p, i64p]
+    L.irls2d_arena_layout.argtypes = [ctypes.c_int, i64p, i64p, i64p, i64p]
     L.irls2d_arena_layout.restype = ctypes.c_int64
     L.irls2d_workspace_size.argtypes = [ctypes.c_int, i64p, i64p, ctypes.POINTER(ctypes.c_size_t)]
     L.irls2d_workspace_size.restype = ctypes.c_int
@@ -82,14 +82,15 @@ def shapes_arrays(shapes):
 
 
 def arena_layout(shapes):
-    """Element offsets of each image in an arena and the arena length."""
+    """(offsets, pitches, total): image i is H rows of pitches[i] elements at offsets[i]."""
     L = load()
     n, H, W = shapes_arrays(shapes)
     off = (ctypes.c_int64 * n)()
-    total = L.irls2d_arena_layout(n, H, W, off)
+    pitch = (ctypes.c_int64 * n)()
+    total = L.irls2d_arena_layout(n, H, W, off, pitch)
     if total < 0:
         raise ValueError(f"bad shapes {shapes}")
-    return [int(o) for o in off], int(total)
+    return [int(o) for o in off], [int(p) for p in pitch], int(total)
 
 
 def workspace_size(shapes) -> int:

@@ -6,25 +6,48 @@
 
 namespace irls2d {
 
-// Elementwise/stencil tile: 64 columns x 64 rows, 256 threads (4 wave64s); lane = column,
-// wave w handles rows w, w+4, ... so that each wave reads 256-byte row segments.
-constexpr int TILE = 64;
+// Elementwise/stencil tile: 1024 columns x 16 rows, 256 threads (4 wave64s).  Wave w owns the
+// 256-column strip j0 + 256w and walks down its 16 rows.  Scalar kernels give lane l the columns
+// strip + l + 64u (u < 4); the float4 stencils give it the four columns strip + 4l .. 4l + 3.
+constexpr int TILE_W = 1024;
+constexpr int TILE_H = 16;
+constexpr int STRIP = 256;
 constexpr int TILE_THREADS = 256;
+constexpr int NUM_XCD = 8;  // workgroups are dispatched round-robin over the 8 XCDs of an MI355X
 // Transpose tile
 constexpr int TT = 64;
-// LDS FFT: at most EPT complex values per thread per pass, <= 1024 threads per workgroup.
-constexpr int EPT = 16;
-constexpr int MAX_FFT_THREADS = 1024;
-constexpr int MAX_FFT_LEN = EPT * MAX_FFT_THREADS;  // 16384
+// LDS FFT: at most E complex values per thread per pass (E = 16 or 24, a template parameter of
+// the transform kernels), <= 512 threads per workgroup (256 VGPRs per lane, no spills at E = 24).
+constexpr int MAX_EPT = 24;
+constexpr int MAX_FFT_THREADS = 512;
+constexpr int MAX_FFT_LEN = MAX_EPT * MAX_FFT_THREADS;  // 12288
 constexpr int MAX_PASS = 16;
 constexpr int MAX_NV = 3;  // values per block partial
+// Split twiddle table in LDS: 64 "lo" entries then up to MAX_FFT_LEN/64 "hi" entries.
+constexpr int TW_LO = 64;
+constexpr int TW_LDS = TW_LO + MAX_FFT_LEN / 64;
+// Padded LDS length of a sequence of L complex values (one pad slot per 16).
+constexpr int lds_seq_len(int L) { return L + (L >> 4) + 1; }
+
+// One Stockham pass, precomputed on the host so the device does no integer division.
+struct PassDesc {
+  int R;        // radix
+  int Ns;       // product of the earlier radices
+  int nb;       // L / R butterflies
+  int twmul;    // L / (Ns R): twiddle index step
+  int rstride;  // padded LDS stride between a butterfly's reads when nb % 16 == 0, else 0
+  int wstride;  // padded LDS stride between its writes when Ns % 16 == 0 (or Ns == 1, R | 16), else 0
+  int nsmask;   // Ns - 1 when Ns is a power of two, else -1
+  float inv_ns; // 1 / Ns for the float quotient of the mixed-radix passes
+};
 
 struct FftPlan {
   int L;       // complex length
   int npass;   // number of Stockham passes
-  int tw_off;  // float2 offset of exp(-2 pi i m / L), m in [0, L)
+  int tw_off;  // float2 offset of the split twiddle table: exp(-2 pi i b / L), b < 64, then
+               // exp(-2 pi i 64 a / L), a < ceil(L / 64)
   int pad;
-  int radix[MAX_PASS];
+  PassDesc pass[MAX_PASS];
 };
 
 struct DctPlan {
@@ -40,15 +63,19 @@ struct DctPlan {
 };
 
 struct ImgDesc {
-  long long off;  // element offset of the image in every field arena
+  long long off;   // element offset of the image in every field arena (multiple of 64)
+  long long offT;  // element offset of the image in the transposed arena (multiple of 64)
   int H, W;
-  int rowplan;    // DctPlan index for length W
-  int colplan;    // DctPlan index for length H
-  int cosH_off;   // double offset of cos(pi i / H), i < H
-  int cosW_off;   // double offset of cos(pi j / W), j < W
+  int P;           // row pitch in elements: W rounded up to a multiple of 64
+  int PT;          // row pitch of the transposed image: H rounded up to a multiple of 64
+  int rowplan;     // DctPlan index for length W
+  int colplan;     // DctPlan index for length H
+  int cosH_off;    // double offset of cos(pi i / H), i < H
+  int cosW_off;    // double offset of cos(pi j / W), j < W
 };
 
-// Per-image solver state.  Written only by "last block" finalizers, read by later kernels.
+// Per-image solver state.  Written by k_state_init and k_finalize (one block per image), read by
+// the other kernels to decide whether to run.
 struct ImgState {
   double rz;        // r.z of the current PCG iteration
   double alpha;
@@ -64,8 +91,7 @@ struct ImgState {
   int improved_it;  // IRLS iteration whose u became u_best
   int restore_it;   // IRLS iteration that restored u_best after a non-finite u
   int status;
-  unsigned counter; // arrival counter of the last-block reductions
-  int pad;
+  int pad0, pad1;
 };
 
 struct Fields {
@@ -96,6 +122,14 @@ struct Ctx {
 };
 
 #ifdef __HIPCC__
+
+// Logical block index such that each XCD (which receives hardware blocks b, b + 8, b + 16, ...)
+// works on one contiguous range of logical blocks: neighbouring tiles then share that XCD's L2.
+__device__ __forceinline__ int xcd_block(int b, int G) {
+  const int q = G / NUM_XCD, r = G % NUM_XCD;
+  const int x = b % NUM_XCD, i = b / NUM_XCD;
+  return x * q + min(x, r) + i;
+}
 
 __device__ __forceinline__ int find_img(const int *start, int n, int blk) {
   int lo = 0, hi = n;
@@ -133,38 +167,20 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
   __syncthreads();
 }
 
-// Two-stage deterministic reduction over the nblk blocks of one image.  Each block stores its
-// partial; the block that arrives last sums all partials of the image in a fixed order and
-// returns true with the totals in `tot` (all its threads).  Other blocks return false.
+// First stage of a deterministic two-stage reduction: the block total of each value goes to
+// partials[blockIdx.x * MAX_NV + q]; k_finalize sums an image's partials in block order.
 template <int NV>
-__device__ __forceinline__ bool reduce_last(double (&v)[NV], double *partials, unsigned *counter,
-                                            int first_blk, int nblk, double (&tot)[NV]) {
-  __shared__ int is_last;
+__device__ __forceinline__ void block_partial(double (&v)[NV], double *partials, int slot) {
   block_sum<NV>(v);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) partials[(long long)blockIdx.x * NV + q] = v[q];
-    __threadfence();
-    unsigned prev = atomicAdd(counter, 1u);
-    is_last = (prev == (unsigned)(nblk - 1));
+    for (int q = 0; q < NV; ++q) partials[(long long)slot * MAX_NV + q] = v[q];
   }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  double acc[NV];
-#pragma unroll
-  for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q)
-      acc[q] += __hip_atomic_load(&partials[(long long)(first_blk + i) * NV + q], __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
-  }
-  block_sum<NV>(acc);
-#pragma unroll
-  for (int q = 0; q < NV; ++q) tot[q] = acc[q];
-  if (threadIdx.x == 0) atomicExch(counter, 0u);
-  return true;
+}
+
+template <int NV>
+__device__ __forceinline__ void block_partial(double (&v)[NV], double *partials) {
+  block_partial<NV>(v, partials, blockIdx.x);
 }
 
 __device__ __forceinline__ float clampf_nan(float v, float lo, float hi) {

@@ -4,9 +4,13 @@
 // parts of one complex sequence (the even/odd Makhoul reordering, J. Makhoul, "A fast cosine
 // transform in one and two dimensions", IEEE TASSP 28(1), 1980), transformed by one complex FFT
 // of length N, and separated with the Hermitian symmetry of real spectra.  The complex FFT is a
-// Stockham auto-sort FFT (radices 8, 4, 2, 3, 5) working in place in LDS: every pass reads all of
-// its butterfly inputs into registers, synchronises, then writes.  Lengths whose prime factors
+// Stockham auto-sort FFT (radices 16, 8, 4, 2, 3, 5) working in place in LDS: every pass reads all
+// of its butterfly inputs into registers, synchronises, then writes.  Lengths whose prime factors
 // exceed 5 use Bluestein's chirp-z algorithm with a 5-smooth length M >= 2N - 1.
+//
+// LDS layout: element i of the sequence lives at pidx(i) = i + i/16 (one pad slot every 16
+// elements), which spreads the stride-R writes of the early Stockham passes over all banks.
+// Twiddles exp(-2 pi i m / L) come from a split table in LDS: lo[m mod 64] * hi[m div 64].
 //
 // DCT conventions (the torch-dct package's norm=None pair the reference calls at
 // insardev/insardev/utils_unwrap2d.py:1334): DCT-II X_k = 2 sum_n x_n cos(pi k (2n+1) / 2N) and
@@ -25,6 +29,8 @@ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
 __device__ __forceinline__ float2 mul_negi(float2 a) { return make_float2(a.y, -a.x); }  // -i * a
+
+__device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
 
 template <int R> struct Dft;
 
@@ -81,17 +87,25 @@ template <> struct Dft<5> {
   }
 };
 
+// multiply by exp(-2 pi i k / 8) for k = 1, 3
+__device__ __forceinline__ float2 w8_1(float2 a) {
+  const float h = 0.70710678118654752440f;
+  return make_float2((a.x + a.y) * h, (a.y - a.x) * h);
+}
+__device__ __forceinline__ float2 w8_3(float2 a) {
+  const float h = 0.70710678118654752440f;
+  return make_float2((a.y - a.x) * h, -(a.x + a.y) * h);
+}
+
 template <> struct Dft<8> {
   static __device__ __forceinline__ void run(float2 *v) {
-    const float h = 0.70710678118654752440f;
     float2 e[4] = {v[0], v[2], v[4], v[6]};
     float2 o[4] = {v[1], v[3], v[5], v[7]};
     Dft<4>::run(e);
     Dft<4>::run(o);
-    // o_k *= exp(-2 pi i k / 8)
-    o[1] = make_float2((o[1].x + o[1].y) * h, (o[1].y - o[1].x) * h);
+    o[1] = w8_1(o[1]);
     o[2] = mul_negi(o[2]);
-    o[3] = make_float2((o[3].y - o[3].x) * h, -(o[3].x + o[3].y) * h);
+    o[3] = w8_3(o[3]);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       v[k] = cadd(e[k], o[k]);
@@ -100,26 +114,94 @@ template <> struct Dft<8> {
   }
 };
 
+// 16-point DFT as 4 x 4: inner DFT4 over m of x[q + 4m], twiddle exp(-2 pi i q k1 / 16), outer DFT4
+// over q; X[k1 + 4 k2].
+template <> struct Dft<16> {
+  static __device__ __forceinline__ void run(float2 *v) {
+    const float c1 = 0.92387953251128675613f;  // cos(pi/8)
+    const float s1 = 0.38268343236508977173f;  // sin(pi/8)
+    float2 b[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      float2 a[4] = {v[q], v[q + 4], v[q + 8], v[q + 12]};
+      Dft<4>::run(a);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) b[q][k1] = a[k1];
+    }
+    // twiddles exp(-2 pi i q k1 / 16), q k1 in {1, 2, 3, 4, 6, 9}
+    b[1][1] = cmul(b[1][1], make_float2(c1, -s1));
+    b[1][2] = w8_1(b[1][2]);
+    b[1][3] = cmul(b[1][3], make_float2(s1, -c1));
+    b[2][1] = w8_1(b[2][1]);
+    b[2][2] = mul_negi(b[2][2]);
+    b[2][3] = w8_3(b[2][3]);
+    b[3][1] = cmul(b[3][1], make_float2(s1, -c1));
+    b[3][2] = w8_3(b[3][2]);
+    b[3][3] = cmul(b[3][3], make_float2(-c1, s1));
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      float2 a[4] = {b[0][k1], b[1][k1], b[2][k1], b[3][k1]};
+      Dft<4>::run(a);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = a[k2];
+    }
+  }
+};
+
+// Twiddle exp(-2 pi i m / L) from the split LDS table (lo: 64 entries, hi: ceil(L/64) entries).
+__device__ __forceinline__ float2 twiddle(const float2 *tws, int m) {
+  return cmul(tws[TW_LO + (m >> 6)], tws[m & 63]);
+}
+
+__device__ __forceinline__ float2 csqr(float2 a) {
+  return make_float2(a.x * a.x - a.y * a.y, 2.f * a.x * a.y);
+}
+
 // One Stockham pass of radix R over a length-L sequence in LDS, Ns = product of earlier radices.
 // Butterfly j reads x[j + t L/R] (t < R), twiddles by exp(-2 pi i t k / (Ns R)) with k = j mod Ns,
-// and writes y[(j - k) R + k + t Ns].
-template <int R>
-__device__ __forceinline__ void stockham_pass(float2 *buf, int L, int Ns, const float2 *__restrict__ tw) {
-  constexpr int MB = (EPT + R - 1) / R;
+// and writes y[(j - k) R + k + t Ns].  Index arithmetic uses 24-bit multiplies and, when the
+// host marked the strides as aligned to the 16-element padding, running LDS offsets.
+template <int R, int E>
+__device__ __forceinline__ void stockham_pass(float2 *buf, const float2 *tws, const PassDesc &pd) {
+  constexpr int MB = (E + R - 1) / R;
   const int T = blockDim.x, tid = threadIdx.x;
-  const int nb = L / R;
-  const int twmul = L / (Ns * R);
+  const int nb = pd.nb, Ns = pd.Ns;
   float2 v[MB][R];
 #pragma unroll
   for (int q = 0; q < MB; ++q) {
     const int j = tid + q * T;
     if (j < nb) {
-      const int k = j % Ns;
+      if (pd.rstride) {
+        int a = pidx(j);
 #pragma unroll
-      for (int t = 0; t < R; ++t) v[q][t] = buf[j + t * nb];
+        for (int t = 0; t < R; ++t) {
+          v[q][t] = buf[a];
+          a += pd.rstride;
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < R; ++t) v[q][t] = buf[pidx(j + __umul24(t, nb))];
+      }
       if (Ns > 1) {
+        int k;
+        if (pd.nsmask >= 0) {
+          k = j & pd.nsmask;
+        } else {
+          int qt = (int)((float)j * pd.inv_ns);
+          k = j - __umul24(qt, Ns);
+          if (k < 0) k += Ns;
+          if (k >= Ns) k -= Ns;
+        }
+        const int m1 = __umul24(k, pd.twmul);
+        float2 w[R];
+        w[1] = twiddle(tws, m1);
 #pragma unroll
-        for (int t = 1; t < R; ++t) v[q][t] = cmul(v[q][t], tw[(t * k) * twmul]);
+        for (int t = 2; t < R; ++t) {
+          // restart from the table every 8 powers, otherwise a short product tree
+          w[t] = (t % 8 == 0) ? twiddle(tws, __umul24(t, m1)) : ((t % 2 == 0) ? csqr(w[t / 2]) : cmul(w[t - 1], w[1]));
+        }
+#pragma unroll
+        for (int t = 1; t < R; ++t) v[q][t] = cmul(v[q][t], w[t]);
       }
       Dft<R>::run(v[q]);
     }
@@ -129,132 +211,176 @@ __device__ __forceinline__ void stockham_pass(float2 *buf, int L, int Ns, const 
   for (int q = 0; q < MB; ++q) {
     const int j = tid + q * T;
     if (j < nb) {
-      const int k = j % Ns;
-      const int base = (j - k) * R + k;
+      int k;
+      if (Ns == 1) {
+        k = 0;
+      } else if (pd.nsmask >= 0) {
+        k = j & pd.nsmask;
+      } else {
+        int qt = (int)((float)j * pd.inv_ns);
+        k = j - __umul24(qt, Ns);
+        if (k < 0) k += Ns;
+        if (k >= Ns) k -= Ns;
+      }
+      const int base = __umul24(j - k, R) + k;
+      if (pd.wstride) {
+        int a = pidx(base);
 #pragma unroll
-      for (int t = 0; t < R; ++t) buf[base + t * Ns] = v[q][t];
+        for (int t = 0; t < R; ++t) {
+          buf[a] = v[q][t];
+          a += pd.wstride;
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < R; ++t) buf[pidx(base + __umul24(t, Ns))] = v[q][t];
+      }
     }
   }
   __syncthreads();
 }
 
-// Forward complex FFT of length P->L in LDS (in place).  Requires L <= EPT * blockDim.x.
-__device__ __forceinline__ void fft_lds(float2 *buf, const FftPlan *P, const float2 *__restrict__ tab) {
-  const int L = P->L, np = P->npass;
-  const float2 *tw = tab + P->tw_off;
-  int Ns = 1;
+// Forward complex FFT of length P->L in LDS (in place).  Requires L <= E * blockDim.x.
+template <int E>
+__device__ __forceinline__ void fft_lds(float2 *buf, const float2 *tws, const FftPlan *P) {
+  const int np = P->npass;
   for (int s = 0; s < np; ++s) {
-    const int R = P->radix[s];
-    switch (R) {
-      case 8: stockham_pass<8>(buf, L, Ns, tw); break;
-      case 4: stockham_pass<4>(buf, L, Ns, tw); break;
-      case 2: stockham_pass<2>(buf, L, Ns, tw); break;
-      case 3: stockham_pass<3>(buf, L, Ns, tw); break;
-      default: stockham_pass<5>(buf, L, Ns, tw); break;
+    const PassDesc pd = P->pass[s];
+    switch (pd.R) {
+      case 16: stockham_pass<16, E>(buf, tws, pd); break;
+      case 8: stockham_pass<8, E>(buf, tws, pd); break;
+      case 4: stockham_pass<4, E>(buf, tws, pd); break;
+      case 2: stockham_pass<2, E>(buf, tws, pd); break;
+      case 3: stockham_pass<3, E>(buf, tws, pd); break;
+      default: stockham_pass<5, E>(buf, tws, pd); break;
     }
-    Ns *= R;
   }
+}
+
+// Workgroup context of one DCT length: LDS buffer, LDS twiddles, plan, global tables.
+struct DctCtx {
+  float2 *buf;
+  float2 *tws;
+  const DctPlan *P;
+  const float2 *tab;
+};
+
+// Points into the dynamic LDS block (twiddles first, then the padded sequence) and loads the
+// twiddles.  The first __syncthreads of the caller's transform orders the loads before use.
+__device__ __forceinline__ DctCtx dct_begin(float2 *lds, const DctPlan *P, const float2 *__restrict__ tab) {
+  DctCtx d;
+  d.tws = lds;
+  d.buf = lds + TW_LDS;
+  d.P = P;
+  d.tab = tab;
+  const int n = TW_LO + ((P->fft.L + 63) >> 6);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) d.tws[i] = tab[P->fft.tw_off + i];
+  return d;
 }
 
 // Length-N forward DFT of the sequence in buf[0, N).  For Bluestein the caller has already
 // multiplied element n by chirp[n] and zeroed buf[N, M); the spectrum is then read via dct_Z.
-__device__ __forceinline__ void dct_fft(float2 *buf, const DctPlan *P, const float2 *__restrict__ tab) {
-  fft_lds(buf, &P->fft, tab);
-  if (P->blue) {
-    const float2 *bh = tab + P->bhat_off;
-    const int M = P->M;
-    for (int k = threadIdx.x; k < M; k += blockDim.x) buf[k] = cconj(cmul(buf[k], bh[k]));
+template <int E>
+__device__ __forceinline__ void dct_fft(const DctCtx &d) {
+  fft_lds<E>(d.buf, d.tws, &d.P->fft);
+  if (d.P->blue) {
+    const float2 *bh = d.tab + d.P->bhat_off;
+    const int M = d.P->M;
+    for (int k = threadIdx.x; k < M; k += blockDim.x) {
+      const int pk = pidx(k);
+      d.buf[pk] = cconj(cmul(d.buf[pk], bh[k]));
+    }
     __syncthreads();
-    fft_lds(buf, &P->fft, tab);
+    fft_lds<E>(d.buf, d.tws, &d.P->fft);
   }
 }
 
-__device__ __forceinline__ float2 dct_Z(const float2 *buf, const DctPlan *P, const float2 *__restrict__ chirp, int k) {
-  float2 v = buf[k];
-  return P->blue ? cmul(chirp[k], cconj(v)) : v;
+__device__ __forceinline__ float2 dct_Z(const DctCtx &d, int k) {
+  float2 v = d.buf[pidx(k)];
+  return d.P->blue ? cmul(d.tab[d.P->chirp_off + k], cconj(v)) : v;
 }
 
 __device__ __forceinline__ int makhoul_pos(int m, int N) { return (m & 1) ? (N - 1 - (m >> 1)) : (m >> 1); }
 
 // Element m of the two lines into its Makhoul position in LDS (chirp-weighted for Bluestein).
-__device__ __forceinline__ void dct2_pair_put(float2 *buf, const DctPlan *P, const float2 *__restrict__ tab,
-                                              int m, float a, float b) {
-  const int n = makhoul_pos(m, P->N);
+__device__ __forceinline__ void dct2_pair_put(const DctCtx &d, int m, float a, float b) {
+  const int n = makhoul_pos(m, d.P->N);
   float2 z = make_float2(a, b);
-  if (P->blue) z = cmul(z, tab[P->chirp_off + n]);
-  buf[n] = z;
+  if (d.P->blue) z = cmul(z, d.tab[d.P->chirp_off + n]);
+  d.buf[pidx(n)] = z;
 }
 
 // After every element went through dct2_pair_put: zero the Bluestein padding and transform.
-__device__ __forceinline__ void dct2_pair_run(float2 *buf, const DctPlan *P, const float2 *__restrict__ tab) {
-  if (P->blue)
-    for (int n = P->N + threadIdx.x; n < P->M; n += blockDim.x) buf[n] = make_float2(0.f, 0.f);
+template <int E>
+__device__ __forceinline__ void dct2_pair_run(const DctCtx &d) {
+  if (d.P->blue)
+    for (int n = d.P->N + threadIdx.x; n < d.P->M; n += blockDim.x) d.buf[pidx(n)] = make_float2(0.f, 0.f);
   __syncthreads();
-  dct_fft(buf, P, tab);
+  dct_fft<E>(d);
 }
 
 // DCT-II of two real lines a, b (b may be null = zeros) of length N into LDS spectrum form.
-__device__ __forceinline__ void dct2_pair_load(float2 *buf, const DctPlan *P, const float2 *__restrict__ tab,
-                                               const float *a, const float *b) {
-  const int N = P->N, T = blockDim.x;
-  for (int m = threadIdx.x; m < N; m += T) dct2_pair_put(buf, P, tab, m, a[m], b ? b[m] : 0.f);
-  dct2_pair_run(buf, P, tab);
+template <int E>
+__device__ __forceinline__ void dct2_pair_load(const DctCtx &d, const float *a, const float *b) {
+  const int N = d.P->N, T = blockDim.x;
+#pragma unroll 4
+  for (int m = threadIdx.x; m < N; m += T) dct2_pair_put(d, m, a[m], b ? b[m] : 0.f);
+  dct2_pair_run<E>(d);
 }
 
-// After dct2_pair_load: DCT-II coefficient k of both lines.
-__device__ __forceinline__ void dct2_pair_coef(const float2 *buf, const DctPlan *P, const float2 *__restrict__ tab,
-                                               int k, float &Xa, float &Xb) {
-  const int N = P->N;
-  const float2 *chirp = tab + P->chirp_off;
-  const float2 Zk = dct_Z(buf, P, chirp, k);
-  const float2 Zn = dct_Z(buf, P, chirp, k ? N - k : 0);
-  const float2 w = tab[P->mk_off + k];
-  Xa = w.x * (Zk.x + Zn.x) - w.y * (Zk.y - Zn.y);
-  Xb = w.x * (Zk.y + Zn.y) + w.y * (Zk.x - Zn.x);
+// After dct2_pair_run: DCT-II coefficients of both lines at k and at nk = (N - k) mod N, from the
+// two spectrum values Z_k, Z_nk (Hermitian separation + Makhoul post-twiddle).
+__device__ __forceinline__ void dct2_coef_pair(const DctCtx &d, int k, int nk, float &Xak, float &Xbk,
+                                               float &Xan, float &Xbn) {
+  const float2 Zk = dct_Z(d, k);
+  const float2 Zn = dct_Z(d, nk);
+  const float2 wk = d.tab[d.P->mk_off + k];
+  const float2 wn = d.tab[d.P->mk_off + nk];
+  Xak = wk.x * (Zk.x + Zn.x) - wk.y * (Zk.y - Zn.y);
+  Xbk = wk.x * (Zk.y + Zn.y) + wk.y * (Zk.x - Zn.x);
+  Xan = wn.x * (Zn.x + Zk.x) - wn.y * (Zn.y - Zk.y);
+  Xbn = wn.x * (Zn.y + Zk.y) + wn.y * (Zn.x - Zk.x);
 }
 
-// Inverse of the DCT-II for two lines whose coefficients sit in buf[k] = (Xa_k, Xb_k), k < N.
-// After the call, dct3_pair_value(m) gives sample m of both lines.
-__device__ __forceinline__ void dct3_pair(float2 *buf, const DctPlan *P, const float2 *__restrict__ tab) {
-  const int N = P->N, M = P->M, T = blockDim.x, tid = threadIdx.x;
-  const float2 *chirp = tab + P->chirp_off;
-  const float2 *mk = tab + P->mk_off;
-  const bool blue = P->blue;
-  float2 reg[EPT];
-#pragma unroll
-  for (int q = 0; q < EPT; ++q) {
-    const int k = tid + q * T;
-    if (k < N) {
-      const float2 Xk = buf[k];
-      const float2 Xn = k ? buf[N - k] : make_float2(0.f, 0.f);
-      const float ya = 0.5f * Xk.x, yb = 0.5f * Xk.y, yan = 0.5f * Xn.x, ybn = 0.5f * Xn.y;
-      const float2 w = mk[k];  // (cos t, -sin t), t = pi k / 2N
-      const float cx = w.x, cy = -w.y;
-      const float vax = cx * ya + cy * yan, vay = cy * ya - cx * yan;
-      const float vbx = cx * yb + cy * ybn, vby = cy * yb - cx * ybn;
-      float2 cz = make_float2(vax - vby, -(vay + vbx));  // conj(Va + i Vb)
-      if (blue) cz = cmul(cz, chirp[k]);
-      reg[q] = cz;
-    }
+// DCT-III input step at index k: from X_k and X_{N-k} (zero for k = 0) of both lines, the value
+// conj(V_a + i V_b) (chirp-weighted for Bluestein) whose forward DFT gives N * conj(output).
+__device__ __forceinline__ float2 dct3_pre(const DctCtx &d, int k, float Xak, float Xbk, float Xan, float Xbn) {
+  const float ya = 0.5f * Xak, yb = 0.5f * Xbk, yan = 0.5f * Xan, ybn = 0.5f * Xbn;
+  const float2 w = d.tab[d.P->mk_off + k];  // (cos t, -sin t), t = pi k / 2N
+  const float cx = w.x, cy = -w.y;
+  const float vax = cx * ya + cy * yan, vay = cy * ya - cx * yan;
+  const float vbx = cx * yb + cy * ybn, vby = cy * yb - cx * ybn;
+  float2 cz = make_float2(vax - vby, -(vay + vbx));
+  if (d.P->blue) cz = cmul(cz, d.tab[d.P->chirp_off + k]);
+  return cz;
+}
+
+// Stores the DCT-III input values of the pair (k, nk) given both lines' coefficients there.
+// Each pair of LDS slots is owned by one thread, so this can run in place over the spectrum.
+__device__ __forceinline__ void dct3_put_pair(const DctCtx &d, int k, int nk, float Xak, float Xbk, float Xan,
+                                              float Xbn) {
+  if (k == 0) {
+    d.buf[pidx(0)] = dct3_pre(d, 0, Xak, Xbk, 0.f, 0.f);
+  } else {
+    d.buf[pidx(k)] = dct3_pre(d, k, Xak, Xbk, Xan, Xbn);
+    if (nk != k) d.buf[pidx(nk)] = dct3_pre(d, nk, Xan, Xbn, Xak, Xbk);
   }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < EPT; ++q) {
-    const int k = tid + q * T;
-    if (k < N) buf[k] = reg[q];
-  }
-  if (blue)
-    for (int n = N + tid; n < M; n += T) buf[n] = make_float2(0.f, 0.f);
-  __syncthreads();
-  dct_fft(buf, P, tab);
 }
 
-__device__ __forceinline__ void dct3_pair_value(const float2 *buf, const DctPlan *P, const float2 *__restrict__ tab,
-                                                int m, float &a, float &b) {
-  const int n = makhoul_pos(m, P->N);
-  const float2 w = dct_Z(buf, P, tab + P->chirp_off, n);
-  a = w.x * P->invN;
-  b = -w.y * P->invN;
+// After every pair went through dct3_put_pair: zero the Bluestein padding and transform;
+// dct3_pair_value(m) then gives sample m of both lines.
+template <int E>
+__device__ __forceinline__ void dct3_run(const DctCtx &d) {
+  if (d.P->blue)
+    for (int n = d.P->N + threadIdx.x; n < d.P->M; n += blockDim.x) d.buf[pidx(n)] = make_float2(0.f, 0.f);
+  __syncthreads();
+  dct_fft<E>(d);
+}
+
+__device__ __forceinline__ void dct3_pair_value(const DctCtx &d, int m, float &a, float &b) {
+  const int n = makhoul_pos(m, d.P->N);
+  const float2 w = dct_Z(d, n);
+  a = w.x * d.P->invN;
+  b = -w.y * d.P->invN;
 }
 
 }  // namespace irls2d

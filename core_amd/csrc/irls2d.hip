@@ -6,6 +6,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -48,6 +49,7 @@ static bool smooth235(int64_t n) {
 
 static std::vector<int> radices(int L) {
   std::vector<int> r;
+  while (L % 16 == 0) { r.push_back(16); L /= 16; }
   while (L % 8 == 0) { r.push_back(8); L /= 8; }
   while (L % 4 == 0) { r.push_back(4); L /= 4; }
   while (L % 2 == 0) { r.push_back(2); L /= 2; }
@@ -85,10 +87,22 @@ struct HostDct {
 static std::mutex g_plan_mu;
 static std::map<int, std::shared_ptr<HostDct>> g_plans;
 
+// Bluestein length: the 5-smooth M >= 2N - 1 (within 30%) with the lowest M * (passes + 1).
 static int64_t bluestein_len(int64_t N) {
-  int64_t M = 2 * N - 1;
-  while (!smooth235(M)) ++M;
-  return M;
+  const int64_t lo = 2 * N - 1;
+  int64_t best = -1;
+  double best_cost = 0;
+  const int64_t hi = std::max<int64_t>(lo, std::min<int64_t>(lo + lo * 3 / 10 + 8, MAX_FFT_LEN));
+  for (int64_t M = lo; M <= hi; ++M) {
+    if (!smooth235(M)) continue;
+    const double cost = (double)M * (double)(radices((int)std::min<int64_t>(M, 1 << 30)).size() + 1);
+    if (best < 0 || cost < best_cost) { best = M; best_cost = cost; }
+  }
+  if (best < 0) {  // beyond the LDS limit: first 5-smooth length (reported as unsupported)
+    best = lo;
+    while (!smooth235(best)) ++best;
+  }
+  return best;
 }
 
 int supported(int64_t N) {
@@ -114,16 +128,34 @@ static std::shared_ptr<HostDct> get_plan(int N) {
     return off;
   };
   const int M = P.M;
-  std::vector<cd> tw(M);
-  for (int m = 0; m < M; ++m) {
-    const double ang = -2.0 * M_PI * (double)m / (double)M;
-    tw[m] = cd(std::cos(ang), std::sin(ang));
+  // split twiddle table: exp(-2 pi i b / M) for b < 64, then exp(-2 pi i 64 a / M), a < ceil(M/64)
+  std::vector<cd> tw(TW_LO + (M + 63) / 64);
+  for (int b = 0; b < TW_LO; ++b) {
+    const double ang = -2.0 * M_PI * (double)b / (double)M;
+    tw[b] = cd(std::cos(ang), std::sin(ang));
+  }
+  for (int a = 0; a < (M + 63) / 64; ++a) {
+    const double ang = -2.0 * M_PI * (double)(64 * (int64_t)a) / (double)M;
+    tw[TW_LO + a] = cd(std::cos(ang), std::sin(ang));
   }
   P.fft.L = M;
   P.fft.tw_off = push(tw);
   auto rad = radices(M);
   P.fft.npass = (int)rad.size();
-  for (size_t s = 0; s < rad.size(); ++s) P.fft.radix[s] = rad[s];
+  int Ns = 1;
+  for (size_t s = 0; s < rad.size(); ++s) {
+    PassDesc &pd = P.fft.pass[s];
+    const int R = rad[s];
+    pd.R = R;
+    pd.Ns = Ns;
+    pd.nb = M / R;
+    pd.twmul = M / (Ns * R);
+    pd.rstride = (pd.nb % 16 == 0) ? pd.nb + pd.nb / 16 : 0;
+    pd.wstride = (Ns % 16 == 0) ? Ns + Ns / 16 : ((Ns == 1 && 16 % R == 0) ? 1 : 0);
+    pd.nsmask = ((Ns & (Ns - 1)) == 0) ? Ns - 1 : -1;
+    pd.inv_ns = (float)(1.0 / (double)Ns);
+    Ns *= R;
+  }
   std::vector<cd> mk(N);
   for (int k = 0; k < N; ++k) {
     const double ang = -M_PI * (double)k / (2.0 * (double)N);
@@ -153,14 +185,18 @@ static std::shared_ptr<HostDct> get_plan(int N) {
 // Batch layout
 // ---------------------------------------------------------------------------------------------
 
-constexpr int NFIELD = 16;  // tx ty wx wy u_prev u_best wxi wyi b r z p0 p1 Ap T Tt
+constexpr int NFIELD = 15;  // tx ty wx wy u_prev u_best wxi wyi b r z p0 p1 Ap T (Tt has its own arena)
+// Field stride padding: 149 x 256 B (an odd number of 256 B units, not a multiple of 4 KiB), so
+// that the same pixel of different fields does not land on the same HBM channel and bank.
+constexpr int64_t FIELD_PAD = 149 * 64;
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 struct Layout {
   int n = 0;
-  std::vector<int64_t> H, W, off;
-  int64_t total = 0;
+  std::vector<int64_t> H, W, off, pitch, offT;
+  int64_t total = 0, totalT = 0;
+  int64_t fstride = 0;  // elements between consecutive workspace fields
   // metadata (host images of the device region)
   std::vector<ImgDesc> desc;
   std::vector<int> geo[4];
@@ -168,10 +204,10 @@ struct Layout {
   std::vector<float2> tab;
   std::vector<double> dtab;
   int max_blocks = 0;
-  int row_threads = 64, col_threads = 64;
+  int row_threads = 64, col_threads = 64, row_E = 16, col_E = 16;
   size_t row_lds = 0, col_lds = 0;
   // byte offsets inside the workspace
-  size_t o_fields = 0, o_valid = 0, o_meta = 0, o_desc = 0, o_state = 0, o_geo[4] = {0, 0, 0, 0},
+  size_t o_fields = 0, o_Tt = 0, o_valid = 0, o_meta = 0, o_desc = 0, o_state = 0, o_geo[4] = {0, 0, 0, 0},
          o_plans = 0, o_tab = 0, o_dtab = 0, o_partials = 0, bytes = 0, meta_bytes = 0;
   uint64_t sig = 0;
 };
@@ -184,20 +220,26 @@ static uint64_t shape_sig(int n, const int64_t *H, const int64_t *W) {
   return h;
 }
 
-static int64_t arena_layout(int n, const int64_t *H, const int64_t *W, int64_t *off) {
+static int64_t pitch_of(int64_t w) { return (w + 63) / 64 * 64; }
+
+// Image i occupies rows of pitch_of(W[i]) elements starting at off[i] (a multiple of 64).
+static int64_t arena_layout(int n, const int64_t *H, const int64_t *W, int64_t *off, int64_t *pitch) {
   int64_t tot = 0;
   for (int i = 0; i < n; ++i) {
     if (H[i] < 1 || W[i] < 1) return -1;
     if (off) off[i] = tot;
-    tot += (H[i] * W[i] + 63) / 64 * 64;
+    if (pitch) pitch[i] = pitch_of(W[i]);
+    tot += H[i] * pitch_of(W[i]);
   }
   return tot;
 }
 
-static int threads_for(int L) {
-  int t = (L + EPT - 1) / EPT;
+// Elements per thread (16 or 32) and threads per workgroup for transforms of length <= L.
+static void fft_shape(int L, int &E, int &T) {
+  E = (L <= 16 * MAX_FFT_THREADS) ? 16 : 24;
+  int t = (L + E - 1) / E;
   t = (t + 63) / 64 * 64;
-  return std::max(64, std::min(MAX_FFT_THREADS, t));
+  T = std::max(64, std::min(MAX_FFT_THREADS, t));
 }
 
 static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
@@ -208,8 +250,11 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
   Lo.H.assign(H, H + n);
   Lo.W.assign(W, W + n);
   Lo.off.resize(n);
-  Lo.total = arena_layout(n, H, W, Lo.off.data());
+  Lo.pitch.resize(n);
+  Lo.offT.resize(n);
+  Lo.total = arena_layout(n, H, W, Lo.off.data(), Lo.pitch.data());
   if (Lo.total < 0) return fail(IRLS2D_EINVAL, "every image needs H >= 1 and W >= 1");
+  Lo.totalT = arena_layout(n, W, H, Lo.offT.data(), nullptr);  // transposed images
   for (int i = 0; i < n; ++i) {
     if (H[i] > (int64_t)1 << 30 || W[i] > (int64_t)1 << 30) return fail(IRLS2D_EINVAL, "dimension too large");
     if (!supported(H[i]) || !supported(W[i]))
@@ -247,8 +292,11 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     const int h = (int)H[i], w = (int)W[i];
     ImgDesc d;
     d.off = Lo.off[i];
+    d.offT = Lo.offT[i];
     d.H = h;
     d.W = w;
+    d.P = (int)Lo.pitch[i];
+    d.PT = (int)pitch_of(h);
     d.rowplan = add_plan(w);
     d.colplan = add_plan(h);
     d.cosH_off = add_cos(h);
@@ -256,7 +304,7 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     Lo.desc.push_back(d);
     maxM_row = std::max(maxM_row, Lo.plans[d.rowplan].M);
     maxM_col = std::max(maxM_col, Lo.plans[d.colplan].M);
-    const int64_t tiles = ((int64_t)(h + TILE - 1) / TILE) * ((w + TILE - 1) / TILE);
+    const int64_t tiles = ((int64_t)(h + TILE_H - 1) / TILE_H) * ((w + TILE_W - 1) / TILE_W);
     const int64_t ttiles = ((int64_t)(h + TT - 1) / TT) * ((w + TT - 1) / TT);
     const int64_t geo_add[4] = {tiles, (h + 1) / 2, (w + 1) / 2, ttiles};
     for (int g = 0; g < 4; ++g) {
@@ -266,14 +314,17 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     }
   }
   for (int g = 0; g < 4; ++g) Lo.max_blocks = std::max(Lo.max_blocks, Lo.geo[g].back());
-  Lo.row_threads = threads_for(maxM_row);
-  Lo.col_threads = threads_for(maxM_col);
-  Lo.row_lds = (size_t)maxM_row * sizeof(float2);
-  Lo.col_lds = (size_t)maxM_col * sizeof(float2);
+  fft_shape(maxM_row, Lo.row_E, Lo.row_threads);
+  fft_shape(maxM_col, Lo.col_E, Lo.col_threads);
+  Lo.row_lds = (size_t)(TW_LDS + lds_seq_len(maxM_row)) * sizeof(float2);
+  Lo.col_lds = (size_t)(TW_LDS + lds_seq_len(maxM_col)) * sizeof(float2);
   // workspace map
   size_t o = 0;
+  Lo.fstride = Lo.total + FIELD_PAD;
   Lo.o_fields = o;
-  o = align_up(o + (size_t)NFIELD * Lo.total * sizeof(float), 256);
+  o = align_up(o + (size_t)NFIELD * Lo.fstride * sizeof(float), 256);
+  Lo.o_Tt = o;
+  o = align_up(o + (size_t)(Lo.totalT + FIELD_PAD) * sizeof(float), 256);
   Lo.o_valid = o;
   o = align_up(o + (size_t)Lo.total, 256);
   Lo.o_state = o;
@@ -320,6 +371,7 @@ struct Params {
   double tol, cg_tol, epsilon;
 };
 
+
 static Ctx make_ctx(const Layout &Lo, unsigned char *ws, float *u, const Params &p) {
   Ctx c;
   std::memset(&c, 0, sizeof(c));
@@ -335,11 +387,12 @@ static Ctx make_ctx(const Layout &Lo, unsigned char *ws, float *u, const Params 
   c.geo.trans = (const int *)(ws + Lo.o_geo[3]);
   float *F = (float *)(ws + Lo.o_fields);
   float *fld[NFIELD];
-  for (int k = 0; k < NFIELD; ++k) fld[k] = F + (size_t)k * Lo.total;
+  for (int k = 0; k < NFIELD; ++k) fld[k] = F + (size_t)k * Lo.fstride;
   c.f.tx = fld[0]; c.f.ty = fld[1]; c.f.wx = fld[2]; c.f.wy = fld[3];
   c.f.u_prev = fld[4]; c.f.u_best = fld[5]; c.f.wxi = fld[6]; c.f.wyi = fld[7];
   c.f.b = fld[8]; c.f.r = fld[9]; c.f.z = fld[10]; c.f.p0 = fld[11]; c.f.p1 = fld[12];
-  c.f.Ap = fld[13]; c.f.T = fld[14]; c.f.Tt = fld[15];
+  c.f.Ap = fld[13]; c.f.T = fld[14];
+  c.f.Tt = (float *)(ws + Lo.o_Tt);
   c.f.u = u;
   c.f.valid = ws + Lo.o_valid;
   c.n = Lo.n;
@@ -349,45 +402,89 @@ static Ctx make_ctx(const Layout &Lo, unsigned char *ws, float *u, const Params 
   return c;
 }
 
+static void fin(const Layout &Lo, const Ctx &c, int kind, int it, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize, dim3(Lo.n), dim3(256), 0, s, c, kind, it);
+}
+
+// Transform launches, dispatched on the elements-per-thread template parameter.
+static void row_dct2(const Layout &Lo, const Ctx &c, bool fuse, float *src, float *dst, const float *p, int guard,
+                     hipStream_t s) {
+  const dim3 g(Lo.geo[1].back()), b(Lo.row_threads);
+  if (fuse) {
+    if (Lo.row_E == 16) hipLaunchKernelGGL((k_row_dct2<true, 16>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
+    else hipLaunchKernelGGL((k_row_dct2<true, 24>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
+  } else {
+    if (Lo.row_E == 16) hipLaunchKernelGGL((k_row_dct2<false, 16>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
+    else hipLaunchKernelGGL((k_row_dct2<false, 24>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
+  }
+}
+
+static void row_dct3(const Layout &Lo, const Ctx &c, const float *src, float *dst, int dot, int guard, hipStream_t s) {
+  const dim3 g(Lo.geo[1].back()), b(Lo.row_threads);
+  if (Lo.row_E == 16) hipLaunchKernelGGL((k_row_dct3<16>), g, b, Lo.row_lds, s, c, src, dst, dot, guard);
+  else hipLaunchKernelGGL((k_row_dct3<24>), g, b, Lo.row_lds, s, c, src, dst, dot, guard);
+}
+
+static void col_pass(const Layout &Lo, const Ctx &c, int guard, hipStream_t s) {
+  const dim3 g(Lo.geo[2].back()), b(Lo.col_threads);
+  if (Lo.col_E == 16) hipLaunchKernelGGL((k_col<16>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
+  else hipLaunchKernelGGL((k_col<24>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
+}
+
+static void transpose(const Layout &Lo, const Ctx &c, int dir, int guard, hipStream_t s) {
+  const float *src = dir == 0 ? c.f.T : c.f.Tt;
+  float *dst = dir == 0 ? c.f.Tt : c.f.T;
+  hipLaunchKernelGGL(k_transpose, dim3(Lo.geo[3].back()), dim3(256), 0, s, c, src, dst, dir, guard);
+}
+
+// z = M^{-1} src (utils_unwrap2d.py:1390-1403); `dot` adds r.z (DOT_RZ_INIT / DOT_RZ).
 static void precond(const Layout &Lo, const Ctx &c, float *src, float *dst, int dot, int guard, hipStream_t s) {
-  const int grp = Lo.geo[1].back(), gcp = Lo.geo[2].back(), gtr = Lo.geo[3].back();
-  hipLaunchKernelGGL(k_row_dct2<false>, dim3(grp), dim3(Lo.row_threads), Lo.row_lds, s, c, src, c.f.T,
-                     (const float *)nullptr, guard);
-  hipLaunchKernelGGL(k_transpose, dim3(gtr), dim3(256), 0, s, c, c.f.T, c.f.Tt, 0, guard);
-  hipLaunchKernelGGL(k_col, dim3(gcp), dim3(Lo.col_threads), Lo.col_lds, s, c, c.f.Tt, guard);
-  hipLaunchKernelGGL(k_transpose, dim3(gtr), dim3(256), 0, s, c, c.f.Tt, c.f.T, 1, guard);
-  hipLaunchKernelGGL(k_row_dct3, dim3(grp), dim3(Lo.row_threads), Lo.row_lds, s, c, c.f.T, dst, dot, guard);
+  row_dct2(Lo, c, false, src, c.f.T, nullptr, guard, s);
+  transpose(Lo, c, 0, guard, s);
+  col_pass(Lo, c, guard, s);
+  transpose(Lo, c, 1, guard, s);
+  row_dct3(Lo, c, c.f.T, dst, dot, guard, s);
+  if (dot == DOT_RZ_INIT) fin(Lo, c, FIN_RZ_INIT, 0, s);
+  if (dot == DOT_RZ) fin(Lo, c, FIN_RZ, 0, s);
 }
 
 static void enqueue(const Layout &Lo, const Ctx &c, const float *phase, const float *weight, int32_t *info,
                     const Params &p, hipStream_t s) {
-  const int gt = Lo.geo[0].back(), grp = Lo.geo[1].back(), gtr = Lo.geo[3].back(), gcp = Lo.geo[2].back();
+  const int gt = Lo.geo[0].back();
+  const dim3 tb(TILE_THREADS);
   hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, c);
-  hipLaunchKernelGGL(k_setup, dim3(gt), dim3(TILE_THREADS), 0, s, c, phase, weight);
-  hipLaunchKernelGGL(k_rhs_static, dim3(gt), dim3(TILE_THREADS), 0, s, c, c.f.r);
+  hipLaunchKernelGGL(k_setup, dim3(gt), tb, 0, s, c, phase, weight);
+  fin(Lo, c, FIN_VCOUNT, 0, s);
+  hipLaunchKernelGGL(k_rhs_static, dim3(gt), tb, 0, s, c, c.f.r);
   precond(Lo, c, c.f.r, c.f.u, DOT_NONE, GUARD_NONE, s);
-  hipLaunchKernelGGL(k_center_reduce, dim3(gt), dim3(TILE_THREADS), 0, s, c);
-  hipLaunchKernelGGL(k_center_apply, dim3(gt), dim3(TILE_THREADS), 0, s, c);
+  hipLaunchKernelGGL(k_center_reduce, dim3(gt), tb, 0, s, c);
+  fin(Lo, c, FIN_CENTER, 0, s);
+  hipLaunchKernelGGL(k_center_apply, dim3(gt), tb, 0, s, c);
   for (int it = 0; it < p.max_iter; ++it) {
-    hipLaunchKernelGGL(k_irls_mean, dim3(gt), dim3(TILE_THREADS), 0, s, c);
-    hipLaunchKernelGGL(k_irls_update, dim3(gt), dim3(TILE_THREADS), 0, s, c, it);
-    hipLaunchKernelGGL(k_irls_commit, dim3(gt), dim3(TILE_THREADS), 0, s, c, it);
-    hipLaunchKernelGGL(k_cg_init, dim3(gt), dim3(TILE_THREADS), 0, s, c);
+    hipLaunchKernelGGL(k_irls_mean, dim3(gt), tb, 0, s, c);
+    fin(Lo, c, FIN_IRLS_MEAN, it, s);
+    hipLaunchKernelGGL(k_irls_update, dim3(gt), tb, 0, s, c);
+    fin(Lo, c, FIN_IRLS_UPDATE, it, s);
+    hipLaunchKernelGGL(k_irls_commit, dim3(gt), tb, 0, s, c, it);
+    hipLaunchKernelGGL(k_cg_init, dim3(gt), tb, 0, s, c);
+    fin(Lo, c, FIN_CG_INIT, it, s);
     precond(Lo, c, c.f.r, c.f.z, DOT_RZ_INIT, GUARD_CG, s);
     for (int i = 0; i < p.cg_max_iter; ++i) {
       float *pold = (i & 1) ? c.f.p1 : c.f.p0;
       float *pnew = (i & 1) ? c.f.p0 : c.f.p1;
-      hipLaunchKernelGGL(k_cg_pap, dim3(gt), dim3(TILE_THREADS), 0, s, c, (const float *)pold, pnew);
-      hipLaunchKernelGGL(k_row_dct2<true>, dim3(grp), dim3(Lo.row_threads), Lo.row_lds, s, c, c.f.r, c.f.T,
-                         (const float *)pnew, (int)GUARD_CG);
-      hipLaunchKernelGGL(k_transpose, dim3(gtr), dim3(256), 0, s, c, c.f.T, c.f.Tt, 0, (int)GUARD_CG);
-      hipLaunchKernelGGL(k_col, dim3(gcp), dim3(Lo.col_threads), Lo.col_lds, s, c, c.f.Tt, (int)GUARD_CG);
-      hipLaunchKernelGGL(k_transpose, dim3(gtr), dim3(256), 0, s, c, c.f.Tt, c.f.T, 1, (int)GUARD_CG);
-      hipLaunchKernelGGL(k_row_dct3, dim3(grp), dim3(Lo.row_threads), Lo.row_lds, s, c, c.f.T, c.f.z,
-                         (int)DOT_RZ, (int)GUARD_CG);
+      hipLaunchKernelGGL(k_cg_pap, dim3(gt), tb, 0, s, c, (const float *)pold, pnew);
+      fin(Lo, c, FIN_PAP, it, s);
+      row_dct2(Lo, c, true, c.f.r, c.f.T, pnew, GUARD_CG, s);
+      fin(Lo, c, FIN_UPDATE, it, s);
+      transpose(Lo, c, 0, GUARD_CG, s);
+      col_pass(Lo, c, GUARD_CG, s);
+      transpose(Lo, c, 1, GUARD_CG, s);
+      row_dct3(Lo, c, c.f.T, c.f.z, DOT_RZ, GUARD_CG, s);
+      fin(Lo, c, FIN_RZ, it, s);
     }
-    hipLaunchKernelGGL(k_irls_check, dim3(gt), dim3(TILE_THREADS), 0, s, c, it);
-    hipLaunchKernelGGL(k_restore, dim3(gt), dim3(TILE_THREADS), 0, s, c, it);
+    hipLaunchKernelGGL(k_irls_check, dim3(gt), tb, 0, s, c);
+    fin(Lo, c, FIN_IRLS_CHECK, it, s);
+    hipLaunchKernelGGL(k_restore, dim3(gt), tb, 0, s, c, it);
   }
   if (info) hipLaunchKernelGGL(k_info, dim3(1), dim3(64), 0, s, c, info);
 }
@@ -444,11 +541,12 @@ static int set_attrs() {
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
   if (dev >= 0 && dev < 64 && g_attr_done[dev]) return IRLS2D_OK;
-  const int lds = MAX_FFT_LEN * (int)sizeof(float2);  // largest transform buffer (128 KiB)
-  HIP_TRY(hipFuncSetAttribute((const void *)k_row_dct2<false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  HIP_TRY(hipFuncSetAttribute((const void *)k_row_dct2<true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  HIP_TRY(hipFuncSetAttribute((const void *)k_col, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  HIP_TRY(hipFuncSetAttribute((const void *)k_row_dct3, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  const int lds = (TW_LDS + lds_seq_len(MAX_FFT_LEN)) * (int)sizeof(float2);  // largest transform
+  const void *fns[] = {(const void *)k_row_dct2<false, 16>, (const void *)k_row_dct2<false, 24>,
+                       (const void *)k_row_dct2<true, 16>,  (const void *)k_row_dct2<true, 24>,
+                       (const void *)k_col<16>,             (const void *)k_col<24>,
+                       (const void *)k_row_dct3<16>,        (const void *)k_row_dct3<24>};
+  for (const void *f : fns) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   if (dev >= 0 && dev < 64) g_attr_done[dev] = true;
   return IRLS2D_OK;
 }
@@ -570,9 +668,9 @@ const char *irls2d_last_error(void) { return irls2d::g_err.c_str(); }
 
 int irls2d_supported_length(int64_t n) { return irls2d::supported(n); }
 
-int64_t irls2d_arena_layout(int n, const int64_t *H, const int64_t *W, int64_t *offsets) {
+int64_t irls2d_arena_layout(int n, const int64_t *H, const int64_t *W, int64_t *offsets, int64_t *pitches) {
   if (n < 1 || !H || !W) return -1;
-  return irls2d::arena_layout(n, H, W, offsets);
+  return irls2d::arena_layout(n, H, W, offsets, pitches);
 }
 
 int irls2d_workspace_size(int n, const int64_t *H, const int64_t *W, size_t *bytes) {

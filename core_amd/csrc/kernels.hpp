@@ -1,8 +1,9 @@
 // Solver kernels.  Every kernel covers all images of a batch in one launch: blockIdx.x is mapped
-// to (image, local block) through the per-geometry block-start tables in Ctx::geo.  Kernels that
-// end a reduction finish it in the last-arriving block of each image ("finalizer"), which is the
-// only writer of that image's ImgState fields; later kernels read the state to decide whether to
-// run, so the whole fixed-count iteration can be enqueued (or captured in a hipGraph) without any
+// to (image, local block) through the per-geometry block-start tables in Ctx::geo.  Reductions
+// are two-stage and deterministic: each block writes its partial sums, then k_finalize (one block
+// per image) adds them in block order and applies the reference's scalar logic (alpha, beta, the
+// PCG and IRLS stopping tests).  The per-image ImgState flags written there decide whether later
+// kernels run, so a fixed-count iteration can be enqueued (or captured in a hipGraph) without
 // host synchronisation while keeping the reference's early exits.
 #pragma once
 
@@ -12,25 +13,36 @@ namespace irls2d {
 
 enum Guard { GUARD_NONE = 0, GUARD_CG = 1, GUARD_IRLS = 2 };
 enum DotMode { DOT_NONE = 0, DOT_RZ_INIT = 1, DOT_RZ = 2 };
+enum FinKind {
+  FIN_VCOUNT = 0,   // k_setup          -> valid pixel count
+  FIN_CENTER,       // k_center_reduce  -> mean of the DCT start
+  FIN_IRLS_MEAN,    // k_irls_mean      -> masked mean
+  FIN_IRLS_UPDATE,  // k_irls_update    -> residual, u_best tracking
+  FIN_CG_INIT,      // k_cg_init        -> non-finite r0 test
+  FIN_PAP,          // k_cg_pap         -> alpha or pAp break
+  FIN_UPDATE,       // k_row_dct2<true> -> x finite / |r| tests
+  FIN_RZ_INIT,      // k_row_dct3 DOT_RZ_INIT -> rz
+  FIN_RZ,           // k_row_dct3 DOT_RZ      -> beta or rz break
+  FIN_IRLS_CHECK,   // k_irls_check     -> restore / relative change
+};
 
 __device__ __forceinline__ bool guard_off(const ImgState *st, int guard) {
   return (guard == GUARD_CG && !st->cg_active) || (guard == GUARD_IRLS && !st->irls_active);
 }
 
 struct TilePos {
-  int img, i0, j0, local, nblk;
+  int img, i0, j0, blk;  // blk: logical block index (partials slot)
 };
 
 __device__ __forceinline__ TilePos tile_pos(const Ctx &c) {
   TilePos t;
-  t.img = find_img(c.geo.tile, c.n, blockIdx.x);
-  const int s = c.geo.tile[t.img];
-  t.nblk = c.geo.tile[t.img + 1] - s;
-  t.local = blockIdx.x - s;
+  t.blk = xcd_block(blockIdx.x, gridDim.x);
+  t.img = find_img(c.geo.tile, c.n, t.blk);
+  const int local = t.blk - c.geo.tile[t.img];
   const int W = c.img[t.img].W;
-  const int ntx = (W + TILE - 1) / TILE;
-  t.i0 = (t.local / ntx) * TILE;
-  t.j0 = (t.local % ntx) * TILE;
+  const int ntx = (W + TILE_W - 1) / TILE_W;
+  t.i0 = (local / ntx) * TILE_H;
+  t.j0 = (local % ntx) * TILE_W;
   return t;
 }
 
@@ -38,26 +50,43 @@ __device__ __forceinline__ TilePos tile_pos(const Ctx &c) {
 template <class F>
 __device__ __forceinline__ void for_tile(const ImgDesc &d, const TilePos &t, F &&f) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int j = t.j0 + lane;
-  if (j >= d.W) return;
-  const int iend = min(t.i0 + TILE, d.H);
-  for (int i = t.i0 + wv; i < iend; i += TILE_THREADS / 64) {
-    const long long idx = d.off + (long long)i * d.W + j;
-    f(i, j, idx);
+  const int jw = t.j0 + STRIP * wv;
+  if (jw >= d.W) return;
+  const int iend = min(t.i0 + TILE_H, d.H);
+  for (int i = t.i0; i < iend; ++i) {
+    const long long row = d.off + (long long)i * d.P;
+#pragma unroll
+    for (int u = 0; u < STRIP / 64; ++u) {
+      const int j = jw + lane + 64 * u;
+      if (j < d.W) f(i, j, row + j);
+    }
   }
 }
 
 // (A x)_ij with A = D^T W D (forward differences, Neumann borders), accumulated in the order of
 // utils_unwrap2d.py:1378-1387: 0 - dx_{j-1} + dx_j - dy_{i-1} + dy_i, then negated.
-template <class LX>
-__device__ __forceinline__ float apply_A(const float *__restrict__ wxi, const float *__restrict__ wyi,
-                                         long long idx, int i, int j, int H, int W, float xc, LX &&X) {
-  float s = 0.f;
-  if (j > 0) s = -((xc - X(idx - 1)) * wxi[idx - 1]);
-  if (j < W - 1) s += (X(idx + 1) - xc) * wxi[idx];
-  if (i > 0) s -= (xc - X(idx - W)) * wyi[idx - W];
-  if (i < H - 1) s += (X(idx + W) - xc) * wyi[idx];
-  return -s;
+// Same operator on four consecutive pixels of a row held in registers: x[k] for columns
+// c0 + k, xl / xr the pixels left of c0 and right of c0 + 3, xu / xd the rows above and below,
+// wxl the x-edge weight left of c0.  Columns >= W give 0.
+__device__ __forceinline__ float4 apply_A4(int i, int c0, int H, int W, float4 x, float xl, float xr, float4 xu,
+                                           float4 xd, float4 wx, float wxl, float4 wyu, float4 wy) {
+  const float xs[6] = {xl, x.x, x.y, x.z, x.w, xr};
+  const float wxs[5] = {wxl, wx.x, wx.y, wx.z, wx.w};
+  const float us[4] = {xu.x, xu.y, xu.z, xu.w}, ds[4] = {xd.x, xd.y, xd.z, xd.w};
+  const float wus[4] = {wyu.x, wyu.y, wyu.z, wyu.w}, wds[4] = {wy.x, wy.y, wy.z, wy.w};
+  float a[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int j = c0 + k;
+    const float xc = xs[k + 1];
+    float s = 0.f;
+    if (j > 0) s = -((xc - xs[k]) * wxs[k]);
+    if (j < W - 1) s += (xs[k + 2] - xc) * wxs[k + 1];
+    if (i > 0) s -= (xc - us[k]) * wus[k];
+    if (i < H - 1) s += (ds[k] - xc) * wds[k];
+    a[k] = (j < W) ? -s : 0.f;
+  }
+  return make_float4(a[0], a[1], a[2], a[3]);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -73,7 +102,7 @@ __global__ void k_state_init(Ctx c) {
     s.irls_active = 1; s.cg_active = 0;
     s.n_cg = 0; s.n_cg_total = 0; s.n_irls = 0;
     s.improved_it = -2; s.restore_it = -2; s.status = 0;
-    s.counter = 0u; s.pad = 0;
+    s.pad0 = 0; s.pad1 = 0;
     c.st[i] = s;
   }
 }
@@ -112,7 +141,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_setup(Ctx c, const float *__re
     }
     if (i < d.H - 1) {
       float phd, wd; bool vd;
-      load_pw(phase, weight, idx + d.W, phd, wd, vd);
+      load_pw(phase, weight, idx + d.P, phd, wd, vd);
       ty = wrap_atan2(phd - phi);
       wy = ((w + wd) / 2.f) * ((v && vd) ? 1.f : 0.f);
     }
@@ -120,10 +149,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_setup(Ctx c, const float *__re
     c.f.valid[idx] = v ? 1 : 0;
     acc[0] += v ? 1.0 : 0.0;
   });
-  double tot[1];
-  if (reduce_last<1>(acc, c.partials, &c.st[t.img].counter, c.geo.tile[t.img], t.nblk, tot)) {
-    if (threadIdx.x == 0) c.st[t.img].vcount = tot[0];
-  }
+  block_partial<1>(acc, c.partials, t.blk);
 }
 
 // rho = D^T W t with the static edge weights (utils_unwrap2d.py:1476-1483).
@@ -135,13 +161,13 @@ __global__ __launch_bounds__(TILE_THREADS) void k_rhs_static(Ctx c, float *__res
     float s = 0.f;
     if (j > 0) s += wx[idx - 1] * tx[idx - 1];
     if (j < d.W - 1) s -= wx[idx] * tx[idx];
-    if (i > 0) s += wy[idx - d.W] * ty[idx - d.W];
+    if (i > 0) s += wy[idx - d.P] * ty[idx - d.P];
     if (i < d.H - 1) s -= wy[idx] * ty[idx];
     out[idx] = s;
   });
 }
 
-// Masked mean of nan_to_num(u) after the DCT start (utils_unwrap2d.py:1498-1513).
+// Masked sum of nan_to_num(u) after the DCT start (utils_unwrap2d.py:1498-1513).
 __global__ __launch_bounds__(TILE_THREADS) void k_center_reduce(Ctx c) {
   const TilePos t = tile_pos(c);
   const ImgDesc d = c.img[t.img];
@@ -151,13 +177,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_center_reduce(Ctx c) {
     if (!isfinite(u)) u = 0.f;
     if (c.f.valid[idx]) acc[0] += (double)u;
   });
-  double tot[1];
-  if (reduce_last<1>(acc, c.partials, &c.st[t.img].counter, c.geo.tile[t.img], t.nblk, tot)) {
-    if (threadIdx.x == 0) {
-      ImgState *st = &c.st[t.img];
-      st->mean = st->vcount > 0 ? tot[0] / st->vcount : 0.0;
-    }
-  }
+  block_partial<1>(acc, c.partials, t.blk);
 }
 
 __global__ __launch_bounds__(TILE_THREADS) void k_center_apply(Ctx c) {
@@ -183,42 +203,37 @@ __global__ __launch_bounds__(TILE_THREADS) void k_center_apply(Ctx c) {
 __global__ __launch_bounds__(TILE_THREADS) void k_irls_mean(Ctx c) {
   const TilePos t = tile_pos(c);
   const ImgDesc d = c.img[t.img];
-  ImgState *st = &c.st[t.img];
-  if (!st->irls_active) return;
+  if (!c.st[t.img].irls_active) return;
   double acc[1] = {0.0};
   for_tile(d, t, [&](int i, int j, long long idx) {
     const float u = c.f.u[idx];
     c.f.u_prev[idx] = u;
     if (c.f.valid[idx]) acc[0] += (double)u;
   });
-  double tot[1];
-  if (reduce_last<1>(acc, c.partials, &st->counter, c.geo.tile[t.img], t.nblk, tot)) {
-    if (threadIdx.x == 0) st->mean = st->vcount > 0 ? tot[0] / st->vcount : 0.0;
-  }
+  block_partial<1>(acc, c.partials, t.blk);
 }
 
 // Centre u, residuals r = D u - t, residual norm, IRLS weights and right-hand side b.
-__global__ __launch_bounds__(TILE_THREADS) void k_irls_update(Ctx c, int it) {
+// Neighbouring tiles read this tile's u, so the centred u goes to z (free between PCG solves)
+// and k_irls_commit moves it into u.
+__global__ __launch_bounds__(TILE_THREADS) void k_irls_update(Ctx c) {
   const TilePos t = tile_pos(c);
   const ImgDesc d = c.img[t.img];
-  ImgState *st = &c.st[t.img];
+  const ImgState *st = &c.st[t.img];
   if (!st->irls_active) return;
   const float m = st->vcount > 0 ? (float)st->mean : 0.f;
   const float eps_sq = c.eps_sq;
   const float *u = c.f.u, *tx = c.f.tx, *ty = c.f.ty, *wx = c.f.wx, *wy = c.f.wy;
   double acc[2] = {0.0, 0.0};  // residual, non-finite count
-  // Neighbouring tiles read this tile's u, so the centred u goes to z (free between PCG solves)
-  // and k_irls_commit moves it into u.
   for_tile(d, t, [&](int i, int j, long long idx) {
     const float uc = u[idx] - m;
     float rx = 0.f, ry = 0.f;
-    float wxi_r = 0.f, wyi_d = 0.f;
     if (j < d.W - 1) rx = (u[idx + 1] - m) - uc;
     rx = rx - tx[idx];
-    if (i < d.H - 1) ry = (u[idx + d.W] - m) - uc;
+    if (i < d.H - 1) ry = (u[idx + d.P] - m) - uc;
     ry = ry - ty[idx];
-    wxi_r = clampf_nan(wx[idx] / sqrtf(rx * rx + eps_sq), 1e-6f, 1e6f);
-    wyi_d = clampf_nan(wy[idx] / sqrtf(ry * ry + eps_sq), 1e-6f, 1e6f);
+    const float wxi_r = clampf_nan(wx[idx] / sqrtf(rx * rx + eps_sq), 1e-6f, 1e6f);
+    const float wyi_d = clampf_nan(wy[idx] / sqrtf(ry * ry + eps_sq), 1e-6f, 1e6f);
     float bb = 0.f;
     if (j > 0) {
       const float rxl = (uc - (u[idx - 1] - m)) - tx[idx - 1];
@@ -227,28 +242,19 @@ __global__ __launch_bounds__(TILE_THREADS) void k_irls_update(Ctx c, int it) {
     }
     if (j < d.W - 1) bb -= wxi_r * tx[idx];
     if (i > 0) {
-      const float ryu = (uc - (u[idx - d.W] - m)) - ty[idx - d.W];
-      const float wu = clampf_nan(wy[idx - d.W] / sqrtf(ryu * ryu + eps_sq), 1e-6f, 1e6f);
-      bb += wu * ty[idx - d.W];
+      const float ryu = (uc - (u[idx - d.P] - m)) - ty[idx - d.P];
+      const float wu = clampf_nan(wy[idx - d.P] / sqrtf(ryu * ryu + eps_sq), 1e-6f, 1e6f);
+      bb += wu * ty[idx - d.P];
     }
     if (i < d.H - 1) bb -= wyi_d * ty[idx];
     c.f.wxi[idx] = wxi_r;
     c.f.wyi[idx] = wyi_d;
     c.f.b[idx] = bb;
+    c.f.z[idx] = uc;
     acc[0] += (double)rx * rx + (double)ry * ry;
     acc[1] += isfinite(uc) ? 0.0 : 1.0;
-    c.f.z[idx] = uc;
   });
-  double tot[2];
-  if (reduce_last<2>(acc, c.partials, &st->counter, c.geo.tile[t.img], t.nblk, tot)) {
-    if (threadIdx.x == 0) {
-      const double res = tot[0];
-      if (res < st->best_res && tot[1] == 0.0) {
-        st->best_res = res;
-        st->improved_it = it;
-      }
-    }
-  }
+  block_partial<2>(acc, c.partials, t.blk);
 }
 
 // u = centred u (from z); u_best = u when this iteration improved the residual.
@@ -265,63 +271,125 @@ __global__ __launch_bounds__(TILE_THREADS) void k_irls_commit(Ctx c, int it) {
   });
 }
 
+// float4 access to 16-byte aligned rows (pitch and strip offsets are multiples of 4).
+__device__ __forceinline__ float4 ld4(const float *__restrict__ a, long long g) {
+  return *reinterpret_cast<const float4 *>(a + g);
+}
+__device__ __forceinline__ void st4(float *__restrict__ a, long long g, float4 v) {
+  *reinterpret_cast<float4 *>(a + g) = v;
+}
+__device__ __forceinline__ float4 mask4(float4 v, int c0, int W) {
+  return make_float4(c0 < W ? v.x : 0.f, c0 + 1 < W ? v.y : 0.f, c0 + 2 < W ? v.z : 0.f, c0 + 3 < W ? v.w : 0.f);
+}
+
+// Row-marching driver of the float4 stencils: wave w walks down the strip j0 + 256 w of its
+// tile; lane l owns columns c0 = strip + 4l .. c0 + 3.  X(i) loads the operand at (i, c0..c0+3)
+// (zero outside the image), XE(i, j) one operand value (zero outside), E(i, ...) consumes row i.
+// Lanes with c0 >= W own no pixel: they touch no memory (c0 < W keeps c0 + 3 inside the row
+// pitch, a multiple of 64) and only feed zeros to the shuffles.
+// Rows i - 1, i, i + 1 of the operand and the y-weights of row i - 1 stay in registers; the
+// left/right neighbours come from the adjacent lanes, at the strip edges from one extra load.
+template <class LX, class LXE, class EMIT>
+__device__ __forceinline__ void march_rows(const Ctx &c, const ImgDesc &d, const TilePos &t, LX &&X, LXE &&XE,
+                                           EMIT &&E) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int jw = t.j0 + STRIP * wv;
+  if (jw >= d.W) return;
+  const int c0 = jw + 4 * lane;
+  const int H = d.H, W = d.W;
+  const int iend = min(t.i0 + TILE_H, H);
+  const float *__restrict__ wxi = c.f.wxi;
+  const float *__restrict__ wyi = c.f.wyi;
+  const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  const bool own = c0 < W;
+  float4 xu = X(t.i0 - 1), xc = X(t.i0);
+  float4 wyu = (own && t.i0 > 0) ? ld4(wyi, d.off + (long long)(t.i0 - 1) * d.P + c0) : z4;
+  for (int i = t.i0; i < iend; ++i) {
+    const long long g = d.off + (long long)i * d.P + c0;
+    const float4 xd = X(i + 1);
+    const float4 wx = own ? ld4(wxi, g) : z4, wy = own ? ld4(wyi, g) : z4;
+    float xl = __shfl_up(xc.w, 1, 64);
+    float wxl = __shfl_up(wx.w, 1, 64);
+    float xr = __shfl_down(xc.x, 1, 64);
+    if (lane == 0) {
+      xl = XE(i, jw - 1);
+      wxl = (jw > 0) ? wxi[d.off + (long long)i * d.P + jw - 1] : 0.f;
+    }
+    if (lane == 63) xr = XE(i, jw + STRIP);
+    const float4 ax = apply_A4(i, c0, H, W, xc, xl, xr, xu, xd, wx, wxl, wyu, wy);
+    if (own) E(i, g, xc, ax);
+    xu = xc;
+    xc = xd;
+    wyu = wy;
+  }
+}
+
 // PCG start: r = b - A x (x = u); a non-finite r returns x unchanged (utils_unwrap2d.py:1414-1419).
 __global__ __launch_bounds__(TILE_THREADS) void k_cg_init(Ctx c) {
   const TilePos t = tile_pos(c);
   const ImgDesc d = c.img[t.img];
-  ImgState *st = &c.st[t.img];
-  if (!st->irls_active) return;
-  const float *u = c.f.u;
   double acc[1] = {0.0};
-  for_tile(d, t, [&](int i, int j, long long idx) {
-    const float xc = u[idx];
-    const float ax = apply_A(c.f.wxi, c.f.wyi, idx, i, j, d.H, d.W, xc, [&](long long k) { return u[k]; });
-    const float r = c.f.b[idx] - ax;
-    c.f.r[idx] = r;
-    acc[0] += isfinite(r) ? 0.0 : 1.0;
-  });
-  double tot[1];
-  if (reduce_last<1>(acc, c.partials, &st->counter, c.geo.tile[t.img], t.nblk, tot)) {
-    if (threadIdx.x == 0) {
-      st->cg_active = tot[0] == 0.0 ? 1 : 0;
-      st->n_cg = 0;
-    }
+  if (c.st[t.img].irls_active) {
+    const float *__restrict__ u = c.f.u;
+    const float *__restrict__ b = c.f.b;
+    float *__restrict__ r = c.f.r;
+    const int c0 = t.j0 + STRIP * (threadIdx.x >> 6) + 4 * (threadIdx.x & 63);
+    auto X = [&](int i) -> float4 {
+      if (i < 0 || i >= d.H || c0 >= d.W) return make_float4(0.f, 0.f, 0.f, 0.f);
+      return mask4(ld4(u, d.off + (long long)i * d.P + c0), c0, d.W);
+    };
+    auto XE = [&](int i, int j) -> float {
+      return (j >= 0 && j < d.W) ? u[d.off + (long long)i * d.P + j] : 0.f;
+    };
+    march_rows(c, d, t, X, XE, [&](int i, long long g, float4 x, float4 ax) {
+      const float4 bb = ld4(b, g);
+      const float4 rr = make_float4(bb.x - ax.x, bb.y - ax.y, bb.z - ax.z, bb.w - ax.w);
+      st4(r, g, rr);
+      const float rv[4] = {rr.x, rr.y, rr.z, rr.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (c0 + k < d.W) acc[0] += isfinite(rv[k]) ? 0.0 : 1.0;
+    });
+  } else {
+    return;
   }
+  block_partial<1>(acc, c.partials, t.blk);
 }
 
-// p = z (first iteration) or z + beta p; Ap = A p; p.Ap  (utils_unwrap2d.py:1429-1440, :1473)
-__global__ __launch_bounds__(TILE_THREADS) void k_cg_pap(Ctx c, const float *__restrict__ p_old, float *__restrict__ p_new) {
+// p = z (first iteration) or z + beta p; Ap = A p; p.Ap  (utils_unwrap2d.py:1429-1440, :1473).
+// float4 loads along rows, vertical neighbours kept in registers while marching down the strip.
+__global__ __launch_bounds__(TILE_THREADS) void k_cg_pap(Ctx c, const float *__restrict__ p_old,
+                                                          float *__restrict__ p_new) {
   const TilePos t = tile_pos(c);
   const ImgDesc d = c.img[t.img];
-  ImgState *st = &c.st[t.img];
+  const ImgState *st = &c.st[t.img];
   if (!st->cg_active) return;
   const bool first = st->n_cg == 0;
   const float beta = (float)st->beta;
-  const float *z = c.f.z;
-  auto P = [&](long long k) -> float {
-    return first ? z[k] : __fadd_rn(__fmul_rn(p_old[k], beta), z[k]);
+  const float *__restrict__ z = c.f.z;
+  float *__restrict__ Ap = c.f.Ap;
+  const int c0 = t.j0 + STRIP * (threadIdx.x >> 6) + 4 * (threadIdx.x & 63);
+  auto pn = [&](float zv, float pv) { return first ? zv : __fadd_rn(__fmul_rn(pv, beta), zv); };
+  auto X = [&](int i) -> float4 {
+    if (i < 0 || i >= d.H || c0 >= d.W) return make_float4(0.f, 0.f, 0.f, 0.f);
+    const long long g = d.off + (long long)i * d.P + c0;
+    const float4 z4 = ld4(z, g);
+    if (first) return mask4(z4, c0, d.W);
+    const float4 p4 = ld4(p_old, g);
+    return mask4(make_float4(pn(z4.x, p4.x), pn(z4.y, p4.y), pn(z4.z, p4.z), pn(z4.w, p4.w)), c0, d.W);
+  };
+  auto XE = [&](int i, int j) -> float {
+    if (j < 0 || j >= d.W) return 0.f;
+    const long long g = d.off + (long long)i * d.P + j;
+    return first ? z[g] : pn(z[g], p_old[g]);
   };
   double acc[1] = {0.0};
-  for_tile(d, t, [&](int i, int j, long long idx) {
-    const float pc = P(idx);
-    const float ap = apply_A(c.f.wxi, c.f.wyi, idx, i, j, d.H, d.W, pc, P);
-    p_new[idx] = pc;
-    c.f.Ap[idx] = ap;
-    acc[0] += (double)pc * (double)ap;
+  march_rows(c, d, t, X, XE, [&](int i, long long g, float4 x, float4 ax) {
+    st4(p_new, g, x);
+    st4(Ap, g, ax);
+    acc[0] += (double)x.x * ax.x + (double)x.y * ax.y + (double)x.z * ax.z + (double)x.w * ax.w;
   });
-  double tot[1];
-  if (reduce_last<1>(acc, c.partials, &st->counter, c.geo.tile[t.img], t.nblk, tot)) {
-    if (threadIdx.x == 0) {
-      const double pAp = tot[0];
-      st->n_cg += 1;
-      st->n_cg_total += 1;
-      if (!(fabs(pAp) >= 1e-15) || !isfinite(pAp)) {
-        st->cg_active = 0;
-      } else {
-        st->alpha = clampd_nan(st->rz / pAp, -1e6, 1e6);
-      }
-    }
-  }
+  block_partial<1>(acc, c.partials, t.blk);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -331,69 +399,85 @@ __global__ __launch_bounds__(TILE_THREADS) void k_cg_pap(Ctx c, const float *__r
 // (utils_unwrap2d.py:1444-1458) to the two rows it transforms.
 // ---------------------------------------------------------------------------------------------
 
-template <bool FUSE_UPDATE>
+template <bool FUSE_UPDATE, int E>
 __global__ __launch_bounds__(MAX_FFT_THREADS) void k_row_dct2(Ctx c, float *in_, float *__restrict__ out,
                                                                const float *__restrict__ p, int guard) {
   extern __shared__ float2 lds[];
   const int img = find_img(c.geo.rowpair, c.n, blockIdx.x);
   const ImgDesc d = c.img[img];
-  ImgState *st = &c.st[img];
+  const ImgState *st = &c.st[img];
   if (guard_off(st, guard)) return;
-  const int q = blockIdx.x - c.geo.rowpair[img];
-  const int nblk = c.geo.rowpair[img + 1] - c.geo.rowpair[img];
-  const int ra = 2 * q;
+  const int ra = 2 * (blockIdx.x - c.geo.rowpair[img]);
   const bool hasb = ra + 1 < d.H;
   const int W = d.W;
-  const long long oa = d.off + (long long)ra * W;
+  const long long oa = d.off + (long long)ra * d.P;
+  const DctCtx D = dct_begin(lds, c.plans + d.rowplan, c.tab);
   double acc[2] = {0.0, 0.0};  // r.r, non-finite x
-  const DctPlan *P = c.plans + d.rowplan;
   if (FUSE_UPDATE) {
-    // x += a p, r -= a Ap on both rows; the new r goes straight into the LDS transform buffer
+    // x += a p, r -= a Ap on both rows; the new r goes straight into the LDS transform buffer.
+    // Loads of a chunk are all issued before its stores so that they overlap in flight.
     float *x = c.f.u;
     const float *Ap = c.f.Ap;
     const float alpha = (float)st->alpha;
-    for (int m = threadIdx.x; m < W; m += blockDim.x) {
-      float rv[2] = {0.f, 0.f};
+    const int T = blockDim.x;
+    constexpr int U = 4;
+    for (int m0 = threadIdx.x; m0 < W; m0 += U * T) {
+      float xv[U][2], pv[U][2], rv[U][2], av[U][2];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        if (h == 1 && !hasb) break;
-        const long long k = oa + (long long)h * W + m;
-        const float xn = x[k] + alpha * p[k];
-        const float rn = in_[k] - alpha * Ap[k];
-        x[k] = xn;
-        in_[k] = rn;
-        rv[h] = rn;
-        acc[0] += (double)rn * rn;
-        acc[1] += isfinite(xn) ? 0.0 : 1.0;
-      }
-      dct2_pair_put(lds, P, c.tab, m, rv[0], rv[1]);
-    }
-    dct2_pair_run(lds, P, c.tab);
-  } else {
-    dct2_pair_load(lds, P, c.tab, in_ + oa, hasb ? in_ + oa + W : nullptr);
-  }
-  for (int k = threadIdx.x; k < W; k += blockDim.x) {
-    float Xa, Xb;
-    dct2_pair_coef(lds, P, c.tab, k, Xa, Xb);
-    out[oa + k] = Xa;
-    if (hasb) out[oa + W + k] = Xb;
-  }
-  if (FUSE_UPDATE) {
-    double tot[2];
-    if (reduce_last<2>(acc, c.partials, &st->counter, c.geo.rowpair[img], nblk, tot)) {
-      if (threadIdx.x == 0) {
-        if (tot[1] != 0.0) {
-          st->cg_active = 0;
-        } else {
-          const double rn = sqrt(tot[0]);
-          if (rn < c.cg_tol || !isfinite(rn)) st->cg_active = 0;
+      for (int u = 0; u < U; ++u) {
+        const int m = m0 + u * T;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          xv[u][h] = pv[u][h] = rv[u][h] = av[u][h] = 0.f;
+          if (m < W && (h == 0 || hasb)) {
+            const long long k = oa + (long long)h * d.P + m;
+            xv[u][h] = x[k];
+            pv[u][h] = p[k];
+            rv[u][h] = in_[k];
+            av[u][h] = Ap[k];
+          }
         }
       }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int m = m0 + u * T;
+        if (m >= W) break;
+        float rn2[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const float xn = xv[u][h] + alpha * pv[u][h];
+          const float rn = rv[u][h] - alpha * av[u][h];
+          rn2[h] = rn;
+          if (h == 0 || hasb) {
+            const long long k = oa + (long long)h * d.P + m;
+            x[k] = xn;
+            in_[k] = rn;
+            acc[0] += (double)rn * rn;
+            acc[1] += isfinite(xn) ? 0.0 : 1.0;
+          }
+        }
+        dct2_pair_put(D, m, rn2[0], hasb ? rn2[1] : 0.f);
+      }
+    }
+    dct2_pair_run<E>(D);
+  } else {
+    dct2_pair_load<E>(D, in_ + oa, hasb ? in_ + oa + d.P : nullptr);
+  }
+  for (int k = threadIdx.x; k <= W / 2; k += blockDim.x) {
+    const int nk = k ? W - k : 0;
+    float Xak, Xbk, Xan, Xbn;
+    dct2_coef_pair(D, k, nk, Xak, Xbk, Xan, Xbn);
+    out[oa + k] = Xak;
+    if (hasb) out[oa + d.P + k] = Xbk;
+    if (nk != k) {
+      out[oa + nk] = Xan;
+      if (hasb) out[oa + d.P + nk] = Xbn;
     }
   }
+  if (FUSE_UPDATE) block_partial<2>(acc, c.partials);
 }
 
-// T (H x W) -> Tt (W x H) when dir == 0, Tt -> T when dir == 1.
+// T (H x W, pitch P) -> Tt (W x H, pitch PT) when dir == 0, Tt -> T when dir == 1.
 __global__ __launch_bounds__(256) void k_transpose(Ctx c, const float *__restrict__ src, float *__restrict__ dst,
                                                    int dir, int guard) {
   __shared__ float tile[TT][TT + 1];
@@ -405,141 +489,132 @@ __global__ __launch_bounds__(256) void k_transpose(Ctx c, const float *__restric
   const int ti = (local / ntx) * TT, tj = (local % ntx) * TT;
   const int x = threadIdx.x & 63, y0 = threadIdx.x >> 6;
   const int H = d.H, W = d.W;
-  const float *s = src + d.off;
-  float *o = dst + d.off;
   if (dir == 0) {
+    const float *s = src + d.off;
+    float *o = dst + d.offT;
     for (int y = y0; y < TT; y += 4) {
       const int i = ti + y, j = tj + x;
-      if (i < H && j < W) tile[y][x] = s[(long long)i * W + j];
+      if (i < H && j < W) tile[y][x] = s[(long long)i * d.P + j];
     }
     __syncthreads();
     for (int y = y0; y < TT; y += 4) {
       const int j = tj + y, i = ti + x;
-      if (i < H && j < W) o[(long long)j * H + i] = tile[x][y];
+      if (i < H && j < W) o[(long long)j * d.PT + i] = tile[x][y];
     }
   } else {
+    const float *s = src + d.offT;
+    float *o = dst + d.off;
     for (int y = y0; y < TT; y += 4) {
       const int j = tj + y, i = ti + x;
-      if (i < H && j < W) tile[y][x] = s[(long long)j * H + i];
+      if (i < H && j < W) tile[y][x] = s[(long long)j * d.PT + i];
     }
     __syncthreads();
     for (int y = y0; y < TT; y += 4) {
       const int i = ti + y, j = tj + x;
-      if (i < H && j < W) o[(long long)i * W + j] = tile[x][y];
+      if (i < H && j < W) o[(long long)i * d.P + j] = tile[x][y];
     }
   }
 }
 
 // Column pass on the transposed field: DCT-II along H, divide by -lambda (float32 lambda from
 // float64 cosines, utils_unwrap2d.py:1284-1289, :1395-1398), zero the DC term, inverse DCT.
+template <int E>
 __global__ __launch_bounds__(MAX_FFT_THREADS) void k_col(Ctx c, float *__restrict__ Tt, int guard) {
   extern __shared__ float2 lds[];
   const int img = find_img(c.geo.colpair, c.n, blockIdx.x);
   const ImgDesc d = c.img[img];
   if (guard_off(&c.st[img], guard)) return;
-  const int q = blockIdx.x - c.geo.colpair[img];
-  const int ja = 2 * q;
+  const int ja = 2 * (blockIdx.x - c.geo.colpair[img]);
   const bool hasb = ja + 1 < d.W;
   const int H = d.H;
-  float *la = Tt + d.off + (long long)ja * H;
-  const DctPlan *P = c.plans + d.colplan;
-  dct2_pair_load(lds, P, c.tab, la, hasb ? la + H : nullptr);
+  float *la = Tt + d.offT + (long long)ja * d.PT;
+  const DctCtx D = dct_begin(lds, c.plans + d.colplan, c.tab);
+  dct2_pair_load<E>(D, la, hasb ? la + d.PT : nullptr);
   const double *cosH = c.dtab + d.cosH_off;
   const double cwa = 2.0 * c.dtab[d.cosW_off + ja];
   const double cwb = hasb ? 2.0 * c.dtab[d.cosW_off + ja + 1] : 0.0;
-  float2 reg[EPT];
   const int T = blockDim.x;
-#pragma unroll
-  for (int qq = 0; qq < EPT; ++qq) {
-    const int k = threadIdx.x + qq * T;
-    if (k < H) {
-      float Xa, Xb;
-      dct2_pair_coef(lds, P, c.tab, k, Xa, Xb);
-      const double ch = 2.0 * cosH[k];
-      float la_ = (float)(ch + cwa - 4.0);
-      float lb_ = (float)(ch + cwb - 4.0);
-      if (k == 0 && ja == 0) la_ = 1.0f;
-      float ya = -(Xa / la_);
-      float yb = hasb ? -(Xb / lb_) : 0.f;
-      if (k == 0 && ja == 0) ya = 0.f;
-      reg[qq] = make_float2(ya, yb);
-    }
+  // X / -lambda with lambda rounded to float from float64 cosines; the quotient uses the
+  // hardware reciprocal (within 2 ulp of the correctly rounded division).
+  auto scale = [&](int k, float &Xa, float &Xb) {
+    const double ch = 2.0 * cosH[k];
+    const float lam_a = (float)(ch + cwa - 4.0);
+    const float lam_b = (float)(ch + cwb - 4.0);
+    Xa = (k == 0 && ja == 0) ? 0.f : -(Xa * __builtin_amdgcn_rcpf(lam_a));
+    Xb = hasb ? -(Xb * __builtin_amdgcn_rcpf(lam_b)) : 0.f;
+  };
+  for (int k = threadIdx.x; k <= H / 2; k += T) {
+    const int nk = k ? H - k : 0;
+    float Xak, Xbk, Xan, Xbn;
+    dct2_coef_pair(D, k, nk, Xak, Xbk, Xan, Xbn);
+    scale(k, Xak, Xbk);
+    scale(nk, Xan, Xbn);
+    dct3_put_pair(D, k, nk, Xak, Xbk, Xan, Xbn);
   }
-  __syncthreads();
-#pragma unroll
-  for (int qq = 0; qq < EPT; ++qq) {
-    const int k = threadIdx.x + qq * T;
-    if (k < H) lds[k] = reg[qq];
-  }
-  __syncthreads();
-  dct3_pair(lds, P, c.tab);
+  dct3_run<E>(D);
   for (int m = threadIdx.x; m < H; m += T) {
     float a, b;
-    dct3_pair_value(lds, P, c.tab, m, a, b);
+    dct3_pair_value(D, m, a, b);
     la[m] = a;
-    if (hasb) la[H + m] = b;
+    if (hasb) la[d.PT + m] = b;
   }
 }
 
-// Row DCT-III: T -> out.  DOT_RZ_INIT: rz = r.z;  DOT_RZ: the PCG beta step
-// (utils_unwrap2d.py:1463-1474): beta = clamp(r.z / rz), rz = r.z, stop when |rz| < 1e-15.
+// Row DCT-III: T -> out, with the partial sums of r.out when `dot` is set.
+template <int E>
 __global__ __launch_bounds__(MAX_FFT_THREADS) void k_row_dct3(Ctx c, const float *__restrict__ T_,
                                                                float *__restrict__ out, int dot, int guard) {
   extern __shared__ float2 lds[];
   const int img = find_img(c.geo.rowpair, c.n, blockIdx.x);
   const ImgDesc d = c.img[img];
-  ImgState *st = &c.st[img];
-  if (guard_off(st, guard)) return;
-  const int q = blockIdx.x - c.geo.rowpair[img];
-  const int nblk = c.geo.rowpair[img + 1] - c.geo.rowpair[img];
-  const int ra = 2 * q;
+  if (guard_off(&c.st[img], guard)) return;
+  const int ra = 2 * (blockIdx.x - c.geo.rowpair[img]);
   const bool hasb = ra + 1 < d.H;
   const int W = d.W, T = blockDim.x;
-  const long long oa = d.off + (long long)ra * W;
-  const DctPlan *P = c.plans + d.rowplan;
-  for (int k = threadIdx.x; k < W; k += T)
-    lds[k] = make_float2(T_[oa + k], hasb ? T_[oa + W + k] : 0.f);
-  __syncthreads();
-  dct3_pair(lds, P, c.tab);
+  const long long oa = d.off + (long long)ra * d.P;
+  const DctCtx D = dct_begin(lds, c.plans + d.rowplan, c.tab);
+  for (int k = threadIdx.x; k <= W / 2; k += T) {
+    const int nk = k ? W - k : 0;
+    const float Xak = T_[oa + k], Xan = T_[oa + nk];
+    const float Xbk = hasb ? T_[oa + d.P + k] : 0.f, Xbn = hasb ? T_[oa + d.P + nk] : 0.f;
+    dct3_put_pair(D, k, nk, Xak, Xbk, Xan, Xbn);
+  }
+  dct3_run<E>(D);
   double acc[1] = {0.0};
   const float *r = c.f.r;
-  for (int m = threadIdx.x; m < W; m += T) {
-    float a, b;
-    dct3_pair_value(lds, P, c.tab, m, a, b);
-    out[oa + m] = a;
-    if (dot) acc[0] += (double)r[oa + m] * (double)a;
-    if (hasb) {
-      out[oa + W + m] = b;
-      if (dot) acc[0] += (double)r[oa + W + m] * (double)b;
-    }
-  }
-  if (dot) {
-    double tot[1];
-    if (reduce_last<1>(acc, c.partials, &st->counter, c.geo.rowpair[img], nblk, tot)) {
-      if (threadIdx.x == 0) {
-        const double rz_new = tot[0];
-        if (dot == DOT_RZ_INIT) {
-          st->rz = rz_new;
-        } else if (fabs(st->rz) < 1e-15) {
-          st->cg_active = 0;
-        } else {
-          st->beta = clampd_nan(rz_new / st->rz, -1e6, 1e6);
-          st->rz = rz_new;
-        }
+  constexpr int U = 4;
+  for (int m0 = threadIdx.x; m0 < W; m0 += U * T) {
+    float ra[U], rb[U];
+    if (dot) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int m = m0 + u * T;
+        ra[u] = (m < W) ? r[oa + m] : 0.f;
+        rb[u] = (m < W && hasb) ? r[oa + d.P + m] : 0.f;
       }
     }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int m = m0 + u * T;
+      if (m >= W) break;
+      float a, b;
+      dct3_pair_value(D, m, a, b);
+      out[oa + m] = a;
+      if (hasb) out[oa + d.P + m] = b;
+      if (dot) acc[0] += (double)ra[u] * (double)a + (double)rb[u] * (double)b;
+    }
   }
+  if (dot) block_partial<1>(acc, c.partials);
 }
 
 // ---------------------------------------------------------------------------------------------
 // End of an IRLS step (utils_unwrap2d.py:1599-1626)
 // ---------------------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(TILE_THREADS) void k_irls_check(Ctx c, int it) {
+__global__ __launch_bounds__(TILE_THREADS) void k_irls_check(Ctx c) {
   const TilePos t = tile_pos(c);
   const ImgDesc d = c.img[t.img];
-  ImgState *st = &c.st[t.img];
-  if (!st->irls_active) return;
+  if (!c.st[t.img].irls_active) return;
   double acc[3] = {0.0, 0.0, 0.0};  // non-finite u, |u_prev - u|^2, |u|^2
   for_tile(d, t, [&](int i, int j, long long idx) {
     const float u = c.f.u[idx];
@@ -548,24 +623,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_irls_check(Ctx c, int it) {
     acc[1] += (double)df * df;
     acc[2] += (double)u * u;
   });
-  double tot[3];
-  if (reduce_last<3>(acc, c.partials, &st->counter, c.geo.tile[t.img], t.nblk, tot)) {
-    if (threadIdx.x == 0) {
-      st->cg_active = 0;
-      st->n_irls = it + 1;
-      if (tot[0] != 0.0) {
-        st->restore_it = it;
-        st->irls_active = 0;
-        st->status |= 1;
-      } else {
-        const double rel = sqrt(tot[1]) / (sqrt(tot[2]) + 1e-10);
-        if (rel < c.tol) {
-          st->irls_active = 0;
-          st->status |= 2;
-        }
-      }
-    }
-  }
+  block_partial<3>(acc, c.partials, t.blk);
 }
 
 // u = u_best after a non-finite IRLS step.
@@ -574,6 +632,101 @@ __global__ __launch_bounds__(TILE_THREADS) void k_restore(Ctx c, int it) {
   const ImgDesc d = c.img[t.img];
   if (c.st[t.img].restore_it != it) return;
   for_tile(d, t, [&](int i, int j, long long idx) { c.f.u[idx] = c.f.u_best[idx]; });
+}
+
+// ---------------------------------------------------------------------------------------------
+// Second reduction stage + scalar logic, one block per image.
+// ---------------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_finalize(Ctx c, int kind, int it) {
+  const int img = blockIdx.x;
+  ImgState *st = &c.st[img];
+  int guard = GUARD_NONE;
+  if (kind == FIN_IRLS_MEAN || kind == FIN_IRLS_UPDATE || kind == FIN_CG_INIT || kind == FIN_IRLS_CHECK)
+    guard = GUARD_IRLS;
+  else if (kind == FIN_PAP || kind == FIN_UPDATE || kind == FIN_RZ_INIT || kind == FIN_RZ)
+    guard = GUARD_CG;
+  if (guard_off(st, guard)) return;
+  const bool rowgeo = (kind == FIN_UPDATE || kind == FIN_RZ_INIT || kind == FIN_RZ);
+  const int *geo = rowgeo ? c.geo.rowpair : c.geo.tile;
+  const int b0 = geo[img], b1 = geo[img + 1];
+  int nv = 1;
+  if (kind == FIN_IRLS_UPDATE || kind == FIN_UPDATE) nv = 2;
+  if (kind == FIN_IRLS_CHECK) nv = 3;
+  double acc[MAX_NV] = {0.0, 0.0, 0.0};
+  for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < MAX_NV; ++q)
+      if (q < nv) acc[q] += c.partials[(long long)b * MAX_NV + q];
+  }
+  block_sum<MAX_NV>(acc);
+  if (threadIdx.x != 0) return;
+  switch (kind) {
+    case FIN_VCOUNT:
+      st->vcount = acc[0];
+      break;
+    case FIN_CENTER:
+    case FIN_IRLS_MEAN:
+      st->mean = st->vcount > 0 ? acc[0] / st->vcount : 0.0;
+      break;
+    case FIN_IRLS_UPDATE:
+      // utils_unwrap2d.py:1555-1558: u_best when the residual improved and u is finite
+      if (acc[0] < st->best_res && acc[1] == 0.0) {
+        st->best_res = acc[0];
+        st->improved_it = it;
+      }
+      break;
+    case FIN_CG_INIT:
+      st->cg_active = acc[0] == 0.0 ? 1 : 0;
+      st->n_cg = 0;
+      break;
+    case FIN_PAP: {
+      const double pAp = acc[0];
+      st->n_cg += 1;
+      st->n_cg_total += 1;
+      if (!(fabs(pAp) >= 1e-15) || !isfinite(pAp))
+        st->cg_active = 0;
+      else
+        st->alpha = clampd_nan(st->rz / pAp, -1e6, 1e6);
+      break;
+    }
+    case FIN_UPDATE:
+      if (acc[1] != 0.0) {
+        st->cg_active = 0;
+      } else {
+        const double rn = sqrt(acc[0]);
+        if (rn < c.cg_tol || !isfinite(rn)) st->cg_active = 0;
+      }
+      break;
+    case FIN_RZ_INIT:
+      st->rz = acc[0];
+      break;
+    case FIN_RZ:
+      if (fabs(st->rz) < 1e-15) {
+        st->cg_active = 0;
+      } else {
+        st->beta = clampd_nan(acc[0] / st->rz, -1e6, 1e6);
+        st->rz = acc[0];
+      }
+      break;
+    case FIN_IRLS_CHECK:
+      st->cg_active = 0;
+      st->n_irls = it + 1;
+      if (acc[0] != 0.0) {
+        st->restore_it = it;
+        st->irls_active = 0;
+        st->status |= 1;
+      } else {
+        const double rel = sqrt(acc[1]) / (sqrt(acc[2]) + 1e-10);
+        if (rel < c.tol) {
+          st->irls_active = 0;
+          st->status |= 2;
+        }
+      }
+      break;
+    default:
+      break;
+  }
 }
 
 __global__ void k_info(Ctx c, int32_t *info) {

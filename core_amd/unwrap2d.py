@@ -61,7 +61,7 @@ class Workspace:
     def __init__(self, shapes, device):
         self.shapes = [(int(h), int(w)) for h, w in shapes]
         self.device = device
-        self.offsets, self.total = _lib.arena_layout(self.shapes)
+        self.offsets, self.pitches, self.total = _lib.arena_layout(self.shapes)
         self.nbytes = _lib.workspace_size(self.shapes)
         self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=device)
         self.info = torch.zeros(4 * len(self.shapes), dtype=torch.int32, device=device)
@@ -119,11 +119,19 @@ def solve_device(phase: torch.Tensor, weight, shapes, max_iter=50, tol=1e-3, cg_
     return out, ws.info.view(n, 4)
 
 
-def _pack(arrays, shapes, offsets, total, device):
-    host = torch.empty(total, dtype=torch.float32, pin_memory=True)
+def image_view(arena, ws, k):
+    """Rows of image k inside an arena (numpy array or torch tensor) laid out by `ws`."""
+    h, w = ws.shapes[k]
+    o, p = ws.offsets[k], ws.pitches[k]
+    return arena[o:o + h * p].reshape(h, p)[:, :w]
+
+
+def pack(arrays, ws, device):
+    """Copy 2-D float32 arrays into a pinned host arena and upload it."""
+    host = torch.zeros(ws.total, dtype=torch.float32, pin_memory=True)
     hv = host.numpy()
-    for a, (h, w), o in zip(arrays, shapes, offsets):
-        hv[o:o + h * w].reshape(h, w)[...] = a
+    for k, a in enumerate(arrays):
+        image_view(hv, ws, k)[...] = a
     return host.to(device, non_blocking=True)
 
 
@@ -160,7 +168,7 @@ def irls_unwrap_2d_batch(phases, weights=None, device='auto', max_iter=50, tol=1
     has_w = [j[2] is not None for j in jobs]
     with _gpu_lock, torch.cuda.device(dev):
         ws = Workspace.get(shapes, dev)
-        ph_t = _pack([j[1][j[4][0]:j[4][1], j[4][2]:j[4][3]] for j in jobs], shapes, ws.offsets, ws.total, dev)
+        ph_t = pack([j[1][j[4][0]:j[4][1], j[4][2]:j[4][3]] for j in jobs], ws, dev)
         w_t = None
         if any(has_w):
             wl = []
@@ -170,7 +178,7 @@ def irls_unwrap_2d_batch(phases, weights=None, device='auto', max_iter=50, tol=1
                     wl.append(j[2][r0:r1, c0:c1])
                 else:  # uniform weight 1 - nan (utils_unwrap2d.py:1239-1241) via weight 1 on valid pixels
                     wl.append(np.ones((r1 - r0, c1 - c0), np.float32))
-            w_t = _pack(wl, shapes, ws.offsets, ws.total, dev)
+            w_t = pack(wl, ws, dev)
         t1 = time.time()
         u_t, info = solve_device(ph_t, w_t, shapes, max_iter, tol, cg_max_iter, cg_tol, epsilon,
                                  graph=graph, workspace=ws)
@@ -180,8 +188,8 @@ def irls_unwrap_2d_batch(phases, weights=None, device='auto', max_iter=50, tol=1
         torch.cuda.synchronize(dev)
         t2 = time.time()
     uh = u_host.numpy()
-    for (k, ph, wt, nm, box), (h, w), o, inf in zip(jobs, shapes, ws.offsets, info_h):
-        u = uh[o:o + h * w].reshape(h, w).copy()
+    for idx, ((k, ph, wt, nm, box), (h, w), inf) in enumerate(zip(jobs, shapes, info_h)):
+        u = image_view(uh, ws, idx).copy()
         r0, r1, c0, c1 = box
         ep = pp.epilogue(u, ph[r0:r1, c0:c1], nm, conncomp_size)
         if ep is None:

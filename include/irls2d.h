@@ -12,9 +12,11 @@
  *
  * Conventions
  *   - All array arguments are DEVICE pointers (HIP global memory) unless noted.
- *   - Images are packed in "arenas": image i occupies elements
- *     [offsets[i], offsets[i] + H[i]*W[i]) in row-major order; offsets come from
- *     irls2d_arena_layout (each image starts on a 64-element boundary).
+ *   - Images are packed in "arenas": image i occupies H[i] rows of pitches[i] elements starting
+ *     at element offsets[i], row-major (pixel (y, x) at offsets[i] + y * pitches[i] + x); both
+ *     come from irls2d_arena_layout: pitches[i] is W[i] rounded up to a multiple of 64 and every
+ *     offset is a multiple of 64, so rows are 256-byte aligned.  Padding elements are ignored on
+ *     input and unspecified on output.
  *   - phase: float32 radians, NaN marks invalid pixels (the crop is done by the caller).
  *   - weight: float32 quality (NaN allowed) or NULL for uniform weights.
  *   - unwrapped: float32 output arena (the raw solver output u, before the host epilogue).
@@ -57,11 +59,12 @@ const char *irls2d_last_error(void);
 int irls2d_supported_length(int64_t n);
 
 /*
- * Arena layout.  Host arrays H[n], W[n] in, host offsets[n] out.  Returns the total number of
- * elements of one arena, or -1 on bad arguments.  No reference counterpart (the reference solves
- * one numpy array per call); it defines the packing of many uneven images for one launch.
+ * Arena layout.  Host arrays H[n], W[n] in; host offsets[n] and pitches[n] out (either may be
+ * NULL).  Returns the total number of elements of one arena, or -1 on bad arguments.  No
+ * reference counterpart (the reference solves one numpy array per call); it defines the packing
+ * of many uneven images for one launch.
  */
-int64_t irls2d_arena_layout(int n, const int64_t *H, const int64_t *W, int64_t *offsets);
+int64_t irls2d_arena_layout(int n, const int64_t *H, const int64_t *W, int64_t *offsets, int64_t *pitches);
 
 /*
  * Workspace bytes needed by irls2d_solve for this set of shapes (host arrays).  The workspace

@@ -78,19 +78,15 @@ def test_batched_equals_plain_bitwise(hip):
 def test_graph_replay_equals_eager(hip):
     import torch
     from core_amd import solve_device
-    from core_amd.unwrap2d import Workspace
+    from core_amd.unwrap2d import Workspace, image_view, pack
     shapes = [(60, 75), (41, 128)]
     ims = [_case(h, w, 20 + i) for i, (h, w) in enumerate(shapes)]
     ws = Workspace.get(shapes, hip)
-    ph = torch.zeros(ws.total, dtype=torch.float32)
-    co = torch.zeros(ws.total, dtype=torch.float32)
-    for (psi, coh), (h, w), o in zip(ims, shapes, ws.offsets):
-        ph[o:o + h * w] = torch.from_numpy(psi.ravel())
-        co[o:o + h * w] = torch.from_numpy(coh.ravel())
-    ph = ph.to(hip)
-    co = co.to(hip)
-    def images(u):  # the arena's alignment padding is never written by the solver
-        return torch.cat([u[o:o + h * w] for (h, w), o in zip(shapes, ws.offsets)])
+    ph = pack([im[0] for im in ims], ws, hip)
+    co = pack([im[1] for im in ims], ws, hip)
+
+    def images(u):  # the arena's row padding is never written by the solver
+        return torch.cat([image_view(u, ws, k).reshape(-1) for k in range(len(shapes))])
 
     u_e, info_e = solve_device(ph, co, shapes, graph=False, workspace=ws, **FIXED)
     u_e = images(u_e).clone()
@@ -99,10 +95,20 @@ def test_graph_replay_equals_eager(hip):
     u_e2, _ = solve_device(ph, co, shapes, graph=False, workspace=ws, **FIXED)
     assert torch.equal(images(u_e2), u_e)
     out = torch.full((ws.total,), float('nan'), device=hip)
+    def diff(a, b):
+        d = (a - b).abs()
+        msg = f"{int((d != 0).sum())} differing of {d.numel()}, max {float(d.max()):.3e}"
+        for k in range(len(shapes)):
+            v = image_view(out, ws, k)
+            nan = torch.isnan(v).nonzero().cpu().numpy()
+            if len(nan):
+                msg += f" | img{k} nan rows {sorted(set(nan[:, 0].tolist()))[:20]} cols {sorted(set(nan[:, 1].tolist()))[:40]}"
+        return msg
+
     for rep in range(4):
         u_g, info_g = solve_device(ph, co, shapes, graph=True, workspace=ws, out=out, **FIXED)
         torch.cuda.synchronize()
-        assert torch.equal(images(u_g), u_e), f"replay {rep}"
+        assert torch.equal(images(u_g), u_e), f"replay {rep}: {diff(images(u_g), u_e)} {info_g.tolist()} {info_e.tolist()}"
         assert torch.equal(info_g, info_e)
         # eager work between replays
         _ = (ph * 2).sum().item()
