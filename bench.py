@@ -1,0 +1,240 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X DCT+IRLS L1 unwrap (one JSON line on stdout).
+
+Workload (value): one plain 8192 x 8192 float32 interferogram (ramp 0-300 rad, 6 bumps, 3 pi
+fault line, 5% masked 16x16 blocks, seed 1), fixed 10 IRLS x 50 PCG iterations, inputs resident
+in HBM.  A step is one complete solve (set-up, DCT start, 10 IRLS steps of 50 PCG iterations),
+replayed from a captured hipGraph.  value = pixels / step time, in Mpx/s.
+
+Also reported:
+  roofline      the kernel with the largest total time in an event-timed solve: its
+                algorithmic bytes per launch / its mean launch duration vs 8 TB/s HBM3E;
+                traffic = measured HBM bytes per launch of that kernel (rocprofv3 FETCH_SIZE and
+                WRITE_SIZE, copy-calibrated) from the committed profile, when present.
+  solve_model   whole-solve algorithmic bytes N [100 + 10 (109 + 50 * 76)] / step time.
+  batched       48 uneven SBAS-burst interferograms (H ~ U{1000..1500}, W ~ U{4000..6000},
+                seeds 1000 + i) solved in one batched graph launch vs 48 sequential plain calls.
+  cpu_baseline  the float32 torch-CPU restatement of the reference solver (oracle.irls_torch32)
+                on a bounded sample of the same workload, on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpx/s unwrapped (10 IRLS iters) plain vs batched at ≤1e-3 rad; % HBM peak"
+HBM_PEAK_GBS = 8000.0
+N_IRLS, N_CG = 10, 50
+# Algorithmic bytes per pixel of one launch of each PCG kernel (reads + writes of float32 fields).
+ALG_BPX = {
+    "k_cg_pap": 24,           # z, p_old, wxi, wyi -> p_new, Ap
+    "k_row_dct2_update": 28,  # x, p, r, Ap -> x, r, T
+    "k_col": 8,               # Tt -> Tt
+    "k_row_dct3": 12,         # T, r -> z
+    "k_transpose": 8,         # T -> Tt / Tt -> T
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_8192.json")
+
+
+def model_bytes_per_px(n_irls=N_IRLS, n_cg=N_CG):
+    return 100 + n_irls * (109 + 76 * n_cg)
+
+
+def upload(ws, arrays, dev):
+    from core_amd.unwrap2d import pack
+    return pack(arrays, ws, dev)
+
+
+def time_steps(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return e0.elapsed_time(e1) / steps, wall / steps
+
+
+def plain_8192(args, dev):
+    import torch
+    from core_amd import solve_device, synthetic as S, _lib
+    from core_amd.unwrap2d import Workspace
+    wl = S.single_8192()
+    psi, coh, _ = S.generate(wl)
+    shapes = [(wl.H, wl.W)]
+    ws = Workspace(shapes, dev)
+    ph = upload(ws, [psi], dev)
+    co = upload(ws, [coh], dev)
+    out = torch.empty(ws.total, dtype=torch.float32, device=dev)
+    kw = dict(max_iter=N_IRLS, tol=0.0, cg_max_iter=N_CG, cg_tol=0.0, out=out, workspace=ws)
+    ms, wall_ms = time_steps(lambda: solve_device(ph, co, shapes, graph=True, **kw), args.steps, args.warmup)
+    info = solve_device(ph, co, shapes, graph=True, **kw)[1].cpu().tolist()[0]
+    # event-timed eager solve: per-kernel mean durations on the launch stream
+    _lib.kernel_stats_reset()
+    solve_device(ph, co, shapes, graph=False, time_kernels=True, **kw)
+    stats = _lib.kernel_stats()
+    ws.release()
+    del ws, ph, co, out
+    torch.cuda.empty_cache()
+    return psi, coh, ms, wall_ms, info, stats
+
+
+def roofline(stats, npx):
+    cand = {k: v for k, v in stats.items() if k in ALG_BPX}
+    dom = max(cand, key=lambda k: cand[k][0])
+    tot_ms, n = cand[dom]
+    avg_s = tot_ms / n / 1e3
+    alg = ALG_BPX[dom] * npx
+    achieved = alg / avg_s / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        if dom in t.get("kernels", {}):
+            traffic = t["kernels"][dom]["bytes_per_launch"]
+    return dom, {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+        "kernel": dom, "alg_bytes_per_launch": alg, "mean_launch_us": round(avg_s * 1e6, 2),
+        "launches": n,
+    }
+
+
+def batched_stack(args, dev):
+    import torch
+    from core_amd import solve_device, synthetic as S
+    from core_amd.unwrap2d import Workspace
+    wls = S.stack_48()
+    ims = [S.generate(wl)[:2] for wl in wls]
+    shapes = [(wl.H, wl.W) for wl in wls]
+    npx = sum(h * w for h, w in shapes)
+    kw = dict(max_iter=N_IRLS, tol=0.0, cg_max_iter=N_CG, cg_tol=0.0)
+    ws = Workspace(shapes, dev)
+    ph = upload(ws, [im[0] for im in ims], dev)
+    co = upload(ws, [im[1] for im in ims], dev)
+    out = torch.empty(ws.total, dtype=torch.float32, device=dev)
+    ms_b, _ = time_steps(lambda: solve_device(ph, co, shapes, graph=True, out=out, workspace=ws, **kw),
+                         args.batched_steps, 1)
+    ws.release()
+    del ws, ph, co, out
+    torch.cuda.empty_cache()
+    # 48 sequential plain calls (one image per call, eager launches), inputs already in HBM
+    singles = []
+    for (psi, coh), sh in zip(ims, shapes):
+        w1 = Workspace([sh], dev)
+        singles.append((w1, upload(w1, [psi], dev), upload(w1, [coh], dev),
+                        torch.empty(w1.total, dtype=torch.float32, device=dev)))
+
+    def seq():
+        for w1, p1, c1, o1 in singles:
+            solve_device(p1, c1, w1.shapes, out=o1, workspace=w1, **kw)
+
+    ms_s, _ = time_steps(seq, 1, 1)
+    for w1, *_ in singles:
+        w1.release()
+    singles.clear()
+    torch.cuda.empty_cache()
+    return {
+        "workload": "48 SBAS bursts H~U{1000..1500} W~U{4000..6000}, seeds 1000+i, 10 IRLS x 50 PCG",
+        "pixels": npx,
+        "batched_ms": round(ms_b, 2), "batched_mpx_s": round(npx / ms_b / 1e3, 2),
+        "sequential_plain_ms": round(ms_s, 2), "sequential_plain_mpx_s": round(npx / ms_s / 1e3, 2),
+        "speedup": round(ms_s / ms_b, 3),
+        "solve_model_frac_hbm": round(npx * model_bytes_per_px() / (ms_b / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+
+
+def cpu_baseline(psi, coh, rows=2048, cols=2048):
+    """Reference CPU path (float32 torch restatement) on the top-left rows x cols of the workload."""
+    import numpy as np
+    import torch
+    from oracle import irls_torch32, post
+    ncores = len(os.sched_getaffinity(0))
+    threads = min(16, ncores)
+    torch.set_num_threads(threads)
+    ph = np.ascontiguousarray(psi[:rows, :cols])
+    wt = np.ascontiguousarray(coh[:rows, :cols])
+    t0 = time.perf_counter()
+    pro = post.prologue(ph, wt)
+    _, ph_c, nm, pf, wf = pro
+    u = irls_torch32.solve(pf, wf, ~nm, N_IRLS, 0.0, N_CG, 0.0, 1e-2)
+    post.epilogue(u, ph_c, nm, 30)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(rows * cols / dt / 1e6, 4), "unit": "Mpx/s", "cores": threads, "kind": "port",
+        "sample": f"top-left {rows}x{cols} crop of the 8192x8192 workload, 10 IRLS x 50 PCG, float32 "
+                  f"torch-CPU restatement incl. host epilogue, {dt:.1f} s on {threads} of {ncores} host threads",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batched-steps", type=int, default=2)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the per-GPU shard is what one GPU shows; see the sharding notes)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP GPU", file=sys.stderr)
+        sys.exit(1)
+    import core_amd._lib as L
+    L.load()
+    dev = torch.device("cuda", 0)
+    npx = 8192 * 8192
+    psi, coh, ms, wall_ms, info, stats = plain_8192(args, dev)
+    dom, roof = roofline(stats, npx)
+    result = {
+        "metric": METRIC,
+        "value": round(npx / ms / 1e3, 3),
+        "unit": "Mpx/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain 8192x8192 float32, ramp 0-300 rad, 6 bumps, 3pi fault, 5% masked 16x16 blocks, seed 1",
+            "irls_iters": N_IRLS, "pcg_iters": N_CG, "pixels": npx, "graph": True,
+            "solver_info": {"irls_iters": info[1], "pcg_iters": info[2], "valid_px": info[3]},
+        },
+        "roofline": roof,
+        "solve_model": {
+            "bytes_per_px": model_bytes_per_px(),
+            "achieved_gbs": round(npx * model_bytes_per_px() / (ms / 1e3) / 1e9, 1),
+            "frac": round(npx * model_bytes_per_px() / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        },
+        "kernel_ms": {k: [round(v[0], 3), v[1]] for k, v in sorted(stats.items(), key=lambda kv: -kv[1][0])},
+    }
+    if not args.no_batched:
+        result["batched"] = batched_stack(args, dev)
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(psi, coh)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

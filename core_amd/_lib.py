@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libirls2d.so")
 
 OK, EINVAL, EUNSUPPORTED, EWORKSPACE, EHIP = 0, 1, 2, 3, 4
 FLAG_GRAPH = 1
+FLAG_TIME_KERNELS = 2
 
 EXPORTS = (
     "irls2d_version",
@@ -26,6 +27,8 @@ EXPORTS = (
     "irls2d_workspace_size",
     "irls2d_solve",
     "irls2d_release",
+    "irls2d_kernel_stats",
+    "irls2d_kernel_stats_reset",
 )
 
 _lib = None
@@ -62,6 +65,10 @@ def load():
     L.irls2d_solve.restype = ctypes.c_int
     L.irls2d_release.argtypes = [ctypes.c_void_p]
     L.irls2d_release.restype = ctypes.c_int
+    L.irls2d_kernel_stats.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), i64p]
+    L.irls2d_kernel_stats.restype = ctypes.c_int
+    L.irls2d_kernel_stats_reset.argtypes = []
+    L.irls2d_kernel_stats_reset.restype = None
     _lib = L
     return L
 
@@ -103,3 +110,22 @@ def workspace_size(shapes) -> int:
 
 def supported_length(n: int) -> bool:
     return bool(load().irls2d_supported_length(int(n)))
+
+
+def kernel_stats():
+    """{kernel name: (total ms, launches)} recorded by solves run with FLAG_TIME_KERNELS."""
+    L = load()
+    n = L.irls2d_kernel_stats(0, None, None, None)
+    names = ctypes.create_string_buffer(64 * max(n, 1))
+    tot = (ctypes.c_double * max(n, 1))()
+    cnt = (ctypes.c_int64 * max(n, 1))()
+    n = L.irls2d_kernel_stats(n, names, tot, cnt)
+    out = {}
+    for k in range(n):
+        nm = names.raw[64 * k:64 * (k + 1)].split(b"\0", 1)[0].decode()
+        out[nm] = (float(tot[k]), int(cnt[k]))
+    return out
+
+
+def kernel_stats_reset():
+    load().irls2d_kernel_stats_reset()

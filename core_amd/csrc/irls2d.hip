@@ -1,5 +1,6 @@
 // Host side of the MI355X IRLS unwrap solver: transform plans, workspace layout, the launch
 // sequence of one (batched) solve, hipGraph capture/replay, and the C ABI of include/irls2d.h.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -371,6 +372,45 @@ struct Params {
   double tol, cg_tol, epsilon;
 };
 
+// Optional per-kernel timing (IRLS2D_FLAG_TIME_KERNELS): a hipEvent pair brackets every launch
+// on the launch stream; durations are accumulated per kernel name after the solve.
+struct KTimer {
+  std::vector<std::tuple<const char *, hipEvent_t, hipEvent_t>> pending;
+};
+static thread_local KTimer *t_timer = nullptr;
+static std::mutex g_stats_mu;
+static std::map<std::string, std::pair<double, long long>> g_kstats;
+
+#define LAUNCH(name, kern, grid, block, lds, stream, ...)                                   \
+  do {                                                                                      \
+    if (t_timer) {                                                                          \
+      hipEvent_t e0_ = nullptr, e1_ = nullptr;                                              \
+      (void)hipEventCreate(&e0_);                                                           \
+      (void)hipEventCreate(&e1_);                                                           \
+      /* events attached to the dispatch packet: kernel start and end timestamps */         \
+      hipExtLaunchKernelGGL(kern, grid, block, lds, stream, e0_, e1_, 0, __VA_ARGS__);      \
+      t_timer->pending.emplace_back(name, e0_, e1_);                                        \
+    } else {                                                                                \
+      hipLaunchKernelGGL(kern, grid, block, lds, stream, __VA_ARGS__);                      \
+    }                                                                                       \
+  } while (0)
+
+static int collect_timer(KTimer &T, hipStream_t s) {
+  HIP_TRY(hipStreamSynchronize(s));
+  std::lock_guard<std::mutex> lk(g_stats_mu);
+  for (auto &e : T.pending) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, std::get<1>(e), std::get<2>(e)));
+    auto &acc = g_kstats[std::get<0>(e)];
+    acc.first += ms;
+    acc.second += 1;
+    (void)hipEventDestroy(std::get<1>(e));
+    (void)hipEventDestroy(std::get<2>(e));
+  }
+  T.pending.clear();
+  return IRLS2D_OK;
+}
+
 
 static Ctx make_ctx(const Layout &Lo, unsigned char *ws, float *u, const Params &p) {
   Ctx c;
@@ -403,7 +443,7 @@ static Ctx make_ctx(const Layout &Lo, unsigned char *ws, float *u, const Params 
 }
 
 static void fin(const Layout &Lo, const Ctx &c, int kind, int it, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize, dim3(Lo.n), dim3(256), 0, s, c, kind, it);
+  LAUNCH("k_finalize", k_finalize, dim3(Lo.n), dim3(256), 0, s, c, kind, it);
 }
 
 // Transform launches, dispatched on the elements-per-thread template parameter.
@@ -411,30 +451,30 @@ static void row_dct2(const Layout &Lo, const Ctx &c, bool fuse, float *src, floa
                      hipStream_t s) {
   const dim3 g(Lo.geo[1].back()), b(Lo.row_threads);
   if (fuse) {
-    if (Lo.row_E == 16) hipLaunchKernelGGL((k_row_dct2<true, 16>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
-    else hipLaunchKernelGGL((k_row_dct2<true, 24>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
+    if (Lo.row_E == 16) LAUNCH("k_row_dct2_update", (k_row_dct2<true, 16>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
+    else LAUNCH("k_row_dct2_update", (k_row_dct2<true, 24>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
   } else {
-    if (Lo.row_E == 16) hipLaunchKernelGGL((k_row_dct2<false, 16>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
-    else hipLaunchKernelGGL((k_row_dct2<false, 24>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
+    if (Lo.row_E == 16) LAUNCH("k_row_dct2", (k_row_dct2<false, 16>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
+    else LAUNCH("k_row_dct2", (k_row_dct2<false, 24>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
   }
 }
 
 static void row_dct3(const Layout &Lo, const Ctx &c, const float *src, float *dst, int dot, int guard, hipStream_t s) {
   const dim3 g(Lo.geo[1].back()), b(Lo.row_threads);
-  if (Lo.row_E == 16) hipLaunchKernelGGL((k_row_dct3<16>), g, b, Lo.row_lds, s, c, src, dst, dot, guard);
-  else hipLaunchKernelGGL((k_row_dct3<24>), g, b, Lo.row_lds, s, c, src, dst, dot, guard);
+  if (Lo.row_E == 16) LAUNCH("k_row_dct3", (k_row_dct3<16>), g, b, Lo.row_lds, s, c, src, dst, dot, guard);
+  else LAUNCH("k_row_dct3", (k_row_dct3<24>), g, b, Lo.row_lds, s, c, src, dst, dot, guard);
 }
 
 static void col_pass(const Layout &Lo, const Ctx &c, int guard, hipStream_t s) {
   const dim3 g(Lo.geo[2].back()), b(Lo.col_threads);
-  if (Lo.col_E == 16) hipLaunchKernelGGL((k_col<16>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
-  else hipLaunchKernelGGL((k_col<24>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
+  if (Lo.col_E == 16) LAUNCH("k_col", (k_col<16>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
+  else LAUNCH("k_col", (k_col<24>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
 }
 
 static void transpose(const Layout &Lo, const Ctx &c, int dir, int guard, hipStream_t s) {
   const float *src = dir == 0 ? c.f.T : c.f.Tt;
   float *dst = dir == 0 ? c.f.Tt : c.f.T;
-  hipLaunchKernelGGL(k_transpose, dim3(Lo.geo[3].back()), dim3(256), 0, s, c, src, dst, dir, guard);
+  LAUNCH("k_transpose", k_transpose, dim3(Lo.geo[3].back()), dim3(256), 0, s, c, src, dst, dir, guard);
 }
 
 // z = M^{-1} src (utils_unwrap2d.py:1390-1403); `dot` adds r.z (DOT_RZ_INIT / DOT_RZ).
@@ -452,27 +492,27 @@ static void enqueue(const Layout &Lo, const Ctx &c, const float *phase, const fl
                     const Params &p, hipStream_t s) {
   const int gt = Lo.geo[0].back();
   const dim3 tb(TILE_THREADS);
-  hipLaunchKernelGGL(k_state_init, dim3(1), dim3(64), 0, s, c);
-  hipLaunchKernelGGL(k_setup, dim3(gt), tb, 0, s, c, phase, weight);
+  LAUNCH("k_state_init", k_state_init, dim3(1), dim3(64), 0, s, c);
+  LAUNCH("k_setup", k_setup, dim3(gt), tb, 0, s, c, phase, weight);
   fin(Lo, c, FIN_VCOUNT, 0, s);
-  hipLaunchKernelGGL(k_rhs_static, dim3(gt), tb, 0, s, c, c.f.r);
+  LAUNCH("k_rhs_static", k_rhs_static, dim3(gt), tb, 0, s, c, c.f.r);
   precond(Lo, c, c.f.r, c.f.u, DOT_NONE, GUARD_NONE, s);
-  hipLaunchKernelGGL(k_center_reduce, dim3(gt), tb, 0, s, c);
+  LAUNCH("k_center_reduce", k_center_reduce, dim3(gt), tb, 0, s, c);
   fin(Lo, c, FIN_CENTER, 0, s);
-  hipLaunchKernelGGL(k_center_apply, dim3(gt), tb, 0, s, c);
+  LAUNCH("k_center_apply", k_center_apply, dim3(gt), tb, 0, s, c);
   for (int it = 0; it < p.max_iter; ++it) {
-    hipLaunchKernelGGL(k_irls_mean, dim3(gt), tb, 0, s, c);
+    LAUNCH("k_irls_mean", k_irls_mean, dim3(gt), tb, 0, s, c);
     fin(Lo, c, FIN_IRLS_MEAN, it, s);
-    hipLaunchKernelGGL(k_irls_update, dim3(gt), tb, 0, s, c);
+    LAUNCH("k_irls_update", k_irls_update, dim3(gt), tb, 0, s, c);
     fin(Lo, c, FIN_IRLS_UPDATE, it, s);
-    hipLaunchKernelGGL(k_irls_commit, dim3(gt), tb, 0, s, c, it);
-    hipLaunchKernelGGL(k_cg_init, dim3(gt), tb, 0, s, c);
+    LAUNCH("k_irls_commit", k_irls_commit, dim3(gt), tb, 0, s, c, it);
+    LAUNCH("k_cg_init", k_cg_init, dim3(gt), tb, 0, s, c);
     fin(Lo, c, FIN_CG_INIT, it, s);
     precond(Lo, c, c.f.r, c.f.z, DOT_RZ_INIT, GUARD_CG, s);
     for (int i = 0; i < p.cg_max_iter; ++i) {
       float *pold = (i & 1) ? c.f.p1 : c.f.p0;
       float *pnew = (i & 1) ? c.f.p0 : c.f.p1;
-      hipLaunchKernelGGL(k_cg_pap, dim3(gt), tb, 0, s, c, (const float *)pold, pnew);
+      LAUNCH("k_cg_pap", k_cg_pap, dim3(gt), tb, 0, s, c, (const float *)pold, pnew);
       fin(Lo, c, FIN_PAP, it, s);
       row_dct2(Lo, c, true, c.f.r, c.f.T, pnew, GUARD_CG, s);
       fin(Lo, c, FIN_UPDATE, it, s);
@@ -482,11 +522,11 @@ static void enqueue(const Layout &Lo, const Ctx &c, const float *phase, const fl
       row_dct3(Lo, c, c.f.T, c.f.z, DOT_RZ, GUARD_CG, s);
       fin(Lo, c, FIN_RZ, it, s);
     }
-    hipLaunchKernelGGL(k_irls_check, dim3(gt), tb, 0, s, c);
+    LAUNCH("k_irls_check", k_irls_check, dim3(gt), tb, 0, s, c);
     fin(Lo, c, FIN_IRLS_CHECK, it, s);
-    hipLaunchKernelGGL(k_restore, dim3(gt), tb, 0, s, c, it);
+    LAUNCH("k_restore", k_restore, dim3(gt), tb, 0, s, c, it);
   }
-  if (info) hipLaunchKernelGGL(k_info, dim3(1), dim3(64), 0, s, c, info);
+  if (info) LAUNCH("k_info", k_info, dim3(1), dim3(64), 0, s, c, info);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -585,8 +625,12 @@ int solve(int n, const int64_t *H, const int64_t *W, const float *phase, const f
   Params p{max_iter, cg_max_iter, tol, cg_tol, epsilon};
   Ctx c = make_ctx(*Lo, ws, out, p);
   if (!(flags & IRLS2D_FLAG_GRAPH)) {
+    KTimer timer;
+    if (flags & IRLS2D_FLAG_TIME_KERNELS) t_timer = &timer;
     enqueue(*Lo, c, phase, weight, info, p, s);
+    t_timer = nullptr;
     HIP_TRY(hipGetLastError());
+    if (flags & IRLS2D_FLAG_TIME_KERNELS) return collect_timer(timer, s);
     return IRLS2D_OK;
   }
   // Graph path: capture once on a private stream, replay with event ordering against `s`.
@@ -690,5 +734,27 @@ int irls2d_solve(int n, const int64_t *H, const int64_t *W, const float *phase, 
 }
 
 int irls2d_release(void *workspace) { return irls2d::release(workspace); }
+
+int irls2d_kernel_stats(int max, char *names, double *total_ms, int64_t *launches) {
+  std::lock_guard<std::mutex> lk(irls2d::g_stats_mu);
+  int k = 0;
+  for (const auto &e : irls2d::g_kstats) {
+    if (k < max) {
+      if (names) {
+        std::strncpy(names + (size_t)k * 64, e.first.c_str(), 63);
+        names[(size_t)k * 64 + 63] = 0;
+      }
+      if (total_ms) total_ms[k] = e.second.first;
+      if (launches) launches[k] = e.second.second;
+    }
+    ++k;
+  }
+  return k;
+}
+
+void irls2d_kernel_stats_reset(void) {
+  std::lock_guard<std::mutex> lk(irls2d::g_stats_mu);
+  irls2d::g_kstats.clear();
+}
 
 }  // extern "C"

@@ -90,7 +90,8 @@ class Workspace:
 
 
 def solve_device(phase: torch.Tensor, weight, shapes, max_iter=50, tol=1e-3, cg_max_iter=20, cg_tol=1e-4,
-                 epsilon=1e-2, out: torch.Tensor | None = None, graph: bool = False, workspace=None):
+                 epsilon=1e-2, out: torch.Tensor | None = None, graph: bool = False, workspace=None,
+                 time_kernels: bool = False):
     """Run the solver on device arenas.
 
     phase, weight: float32 device tensors holding the images at `Workspace.offsets`
@@ -114,7 +115,7 @@ def solve_device(phase: torch.Tensor, weight, shapes, max_iter=50, tol=1e-3, cg_
     rc = L.irls2d_solve(n, H, W, phase.data_ptr(), weight.data_ptr() if weight is not None else None,
                         out.data_ptr(), int(max_iter), float(tol), int(cg_max_iter), float(cg_tol),
                         float(epsilon), ws.buf.data_ptr(), ws.nbytes, ws.info.data_ptr(), stream,
-                        _lib.FLAG_GRAPH if graph else 0)
+                        (_lib.FLAG_GRAPH if graph else 0) | (_lib.FLAG_TIME_KERNELS if time_kernels else 0))
     _lib.check(rc, "irls2d_solve")
     return out, ws.info.view(n, 4)
 

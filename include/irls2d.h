@@ -41,7 +41,9 @@ extern "C" {
 #define IRLS2D_EHIP 4         /* HIP runtime error */
 
 /* flags for irls2d_solve */
-#define IRLS2D_FLAG_GRAPH 1u /* capture the whole launch sequence once in a hipGraph and replay it */
+#define IRLS2D_FLAG_GRAPH 1u        /* capture the whole launch sequence once in a hipGraph and replay it */
+#define IRLS2D_FLAG_TIME_KERNELS 2u /* eager only: hipEvent pairs around every launch, see irls2d_kernel_stats;
+                                       the call then synchronises `stream` before returning */
 
 /* info[i*4 + k] written by irls2d_solve (device int32 array of 4*n) */
 #define IRLS2D_INFO_STATUS 0   /* bit0: IRLS hit a non-finite u and restored u_best; bit1: stopped by tol */
@@ -94,6 +96,15 @@ int irls2d_solve(int n, const int64_t *H, const int64_t *W,
  * freeing a workspace that was used with IRLS2D_FLAG_GRAPH.
  */
 int irls2d_release(void *workspace);
+
+/*
+ * Per-kernel timing accumulated by calls made with IRLS2D_FLAG_TIME_KERNELS.  Fills up to `max`
+ * entries: names (64-byte NUL-terminated slots), total milliseconds and launch counts (any
+ * pointer may be NULL) and returns the number of distinct kernels recorded.  No reference
+ * counterpart (the reference prints phase timings with debug=True, utils_unwrap2d.py:1758-1764).
+ */
+int irls2d_kernel_stats(int max, char *names, double *total_ms, int64_t *launches);
+void irls2d_kernel_stats_reset(void);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,142 @@
+"""CPU tests of the boundary: the C ABI loads and exports every symbol of include/irls2d.h, its
+host-only entry points (layout, workspace size, argument errors) behave, and the host epilogue of
+the package matches the oracle's restatement of the reference epilogue."""
+import ctypes
+import os
+import re
+import warnings
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "irls2d.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(irls2d_\w+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import core_amd._lib as L
+    return L
+
+
+def test_library_exports_every_header_symbol(lib):
+    L = lib.load()
+    names = header_functions()
+    assert len(names) >= 9
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(lib.EXPORTS)
+    assert L.irls2d_version() >= 100
+
+
+def test_arena_layout_pitch_and_alignment(lib):
+    shapes = [(10, 20), (3, 65), (1, 1), (7, 128)]
+    off, pitch, total = lib.arena_layout(shapes)
+    assert pitch == [64, 128, 64, 128]
+    assert off[0] == 0 and all(o % 64 == 0 for o in off)
+    ends = [o + h * p for o, p, (h, w) in zip(off, pitch, shapes)]
+    assert all(e <= o2 for e, o2 in zip(ends, off[1:]))
+    assert total == ends[-1]
+
+
+def test_workspace_size_grows_with_pixels(lib):
+    a = lib.workspace_size([(64, 64)])
+    b = lib.workspace_size([(128, 128)])
+    c = lib.workspace_size([(64, 64), (128, 128)])
+    assert 0 < a < b < c
+    # 15 float fields + transposed buffer + valid bytes at least
+    assert b >= 16 * 128 * 128 * 4
+
+
+def test_supported_lengths(lib):
+    assert lib.supported_length(8192) and lib.supported_length(12288)
+    assert lib.supported_length(4999)            # Bluestein
+    assert not lib.supported_length(12289)       # beyond the LDS-resident lengths
+    assert not lib.supported_length(0)
+
+
+def test_argument_errors_without_gpu(lib):
+    L = lib.load()
+    n, H, W = lib.shapes_arrays([(16, 16)])
+    # null pointers -> EINVAL before any HIP call
+    rc = L.irls2d_solve(n, H, W, None, None, None, 10, 0.0, 50, 0.0, 1e-2, None, 0, None, None, 0)
+    assert rc == lib.EINVAL and b"null" in L.irls2d_last_error()
+    with pytest.raises(ValueError):
+        lib.check(rc, "irls2d_solve")
+    nb, Hb, Wb = lib.shapes_arrays([(0, 16)])
+    b = ctypes.c_size_t(0)
+    assert L.irls2d_workspace_size(nb, Hb, Wb, ctypes.byref(b)) == lib.EINVAL
+    nu, Hu, Wu = lib.shapes_arrays([(16, 20011)])
+    assert L.irls2d_workspace_size(nu, Hu, Wu, ctypes.byref(b)) == lib.EUNSUPPORTED
+    assert L.irls2d_arena_layout(0, H, W, None, None) == -1
+
+
+def test_resolve_device_rules():
+    from core_amd.unwrap2d import resolve_device
+    for d in ("cpu", "mps", "tpu"):
+        with pytest.raises(ValueError):
+            resolve_device(d)
+    with pytest.raises(ValueError):
+        resolve_device(3.0)
+
+
+def test_crop_box_matches_oracle_bbox():
+    from core_amd.postprocess import crop_box
+    from oracle import post
+    r = np.random.default_rng(0)
+    for _ in range(20):
+        a = r.normal(size=(13, 17)).astype(np.float32)
+        a[r.uniform(size=a.shape) < 0.5] = np.nan
+        a[:r.integers(0, 4)] = np.nan
+        a[:, -r.integers(1, 4):] = np.nan
+        nm, box = crop_box(a)
+        if np.isnan(a).all():
+            assert box is None
+        else:
+            assert box == post.bbox(np.isnan(a))
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_epilogue_matches_oracle(seed):
+    from core_amd.postprocess import epilogue
+    from oracle import post
+    r = np.random.default_rng(seed)
+    H, W = 37, 53
+    psi = r.uniform(-np.pi, np.pi, size=(H, W)).astype(np.float32)
+    psi[r.uniform(size=(H, W)) < 0.35] = np.nan           # many small components and ties
+    nm = np.isnan(psi)
+    u = (np.where(nm, 0, psi) + 2 * np.pi * r.integers(-3, 4) + r.normal(0, 0.4, size=(H, W))).astype(np.float32)
+    for size in (1, 4, 30):
+        a = epilogue(u.copy(), psi, nm, size)
+        b = post.epilogue(u.copy(), psi, nm, size)
+        np.testing.assert_array_equal(a[1], b[1])
+        np.testing.assert_allclose(a[0], b[0], atol=2e-6, equal_nan=True)
+
+
+def test_epilogue_nonfinite_warns():
+    from core_amd.postprocess import epilogue
+    psi = np.zeros((4, 4), np.float32)
+    u = np.zeros((4, 4), np.float32)
+    u[1, 1] = np.inf
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        assert epilogue(u, psi, np.isnan(psi), 30) is None
+    assert any(issubclass(x.category, RuntimeWarning) for x in w)
+
+
+def test_synthetic_generator_is_deterministic_and_banded():
+    from core_amd import synthetic as S
+    wl = S.Workload(70, 90, 5, ramp_max=40.0, fault=True, mask_frac=0.1)
+    a = S.generate(wl, band_rows=2048)
+    b = S.generate(wl, band_rows=7)           # banded draws = one whole-field draw
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    psi, coh, true = a
+    assert np.nanmax(np.abs(psi)) <= np.pi + 1e-6
+    assert np.isnan(psi).any() and (coh[np.isnan(psi)] == 0).all()
+    assert S.stack_shapes()[:2] == [(1419, 5954), (1131, 4636)]
