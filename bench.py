@@ -14,6 +14,7 @@ Also reported:
   solve_model   whole-solve algorithmic bytes N [100 + 10 (109 + 50 * 76)] / step time.
   batched       48 uneven SBAS-burst interferograms (H ~ U{1000..1500}, W ~ U{4000..6000},
                 seeds 1000 + i) solved in one batched graph launch vs 48 sequential plain calls.
+  adjoint       16 x 2048 x 2048 forward solve and implicit adjoint (dL/dphase, dL/dweight).
   cpu_baseline  the float32 torch-CPU restatement of the reference solver (oracle.irls_torch32)
                 on a bounded sample of the same workload, on this host's cores.
 """
@@ -157,6 +158,47 @@ def batched_stack(args, dev):
     }
 
 
+def adjoint_batch(args, dev):
+    """16 x 2048 x 2048 differentiable unwrap: forward solve and implicit adjoint (50 PCG)."""
+    import ctypes
+    import numpy as np
+    import torch
+    from core_amd import solve_device, synthetic as S, _lib
+    from core_amd.unwrap2d import Workspace
+    wls = S.adjoint_batch()
+    ims = [S.generate(wl)[:2] for wl in wls]
+    shapes = [(wl.H, wl.W) for wl in wls]
+    npx = sum(h * w for h, w in shapes)
+    ws = Workspace(shapes, dev)
+    ph = upload(ws, [im[0] for im in ims], dev)
+    co = upload(ws, [im[1] for im in ims], dev)
+    out = torch.empty(ws.total, dtype=torch.float32, device=dev)
+    kw = dict(max_iter=N_IRLS, tol=0.0, cg_max_iter=N_CG, cg_tol=0.0, out=out, workspace=ws)
+    ms_f, _ = time_steps(lambda: solve_device(ph, co, shapes, graph=True, **kw), 1, 1)
+    g = torch.randn(ws.total, device=dev)
+    gph = torch.empty_like(g)
+    gw = torch.empty_like(g)
+    n, H, W = _lib.shapes_arrays(shapes)
+    L = _lib.load()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def adj():
+        _lib.check(L.irls2d_adjoint(n, H, W, ph.data_ptr(), co.data_ptr(), out.data_ptr(), g.data_ptr(),
+                                    gph.data_ptr(), gw.data_ptr(), 1e-2, N_CG, 0.0, ws.buf.data_ptr(), ws.nbytes,
+                                    ws.info.data_ptr(), stream, 0), "irls2d_adjoint")
+
+    ms_a, _ = time_steps(adj, 1, 1)
+    ok = bool(torch.isfinite(gph).all() and torch.isfinite(gw).all())
+    ws.release()
+    del ws, ph, co, out, g, gph, gw
+    torch.cuda.empty_cache()
+    return {
+        "workload": "16 x 2048x2048, 1024-case generator seeds 3+i; forward 10 IRLS x 50 PCG; adjoint 50 PCG",
+        "pixels": npx, "forward_ms": round(ms_f, 2), "forward_mpx_s": round(npx / ms_f / 1e3, 2),
+        "adjoint_ms": round(ms_a, 2), "adjoint_mpx_s": round(npx / ms_a / 1e3, 2), "grads_finite": ok,
+    }
+
+
 def cpu_baseline(psi, coh, rows=2048, cols=2048):
     """Reference CPU path (float32 torch restatement) on the top-left rows x cols of the workload."""
     import numpy as np
@@ -188,6 +230,7 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=2)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -231,6 +274,8 @@ def main():
     }
     if not args.no_batched:
         result["batched"] = batched_stack(args, dev)
+    if not args.no_adjoint:
+        result["adjoint"] = adjoint_batch(args, dev)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(psi, coh)
     print(json.dumps(result), flush=True)

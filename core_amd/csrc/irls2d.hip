@@ -488,6 +488,29 @@ static void precond(const Layout &Lo, const Ctx &c, float *src, float *dst, int 
   if (dot == DOT_RZ) fin(Lo, c, FIN_RZ, 0, s);
 }
 
+// PCG on A = D^T W D with W from c.f.wxi / c.f.wyi, right-hand side c.f.b, x = c.f.u updated in
+// place (utils_unwrap2d.py:1405-1474): r = b - A x, z = M^{-1} r, then cg_max_iter iterations.
+static void pcg(const Layout &Lo, const Ctx &c, int cg_max_iter, int it, hipStream_t s) {
+  const int gt = Lo.geo[0].back();
+  const dim3 tb(TILE_THREADS);
+  LAUNCH("k_cg_init", k_cg_init, dim3(gt), tb, 0, s, c);
+  fin(Lo, c, FIN_CG_INIT, it, s);
+  precond(Lo, c, c.f.r, c.f.z, DOT_RZ_INIT, GUARD_CG, s);
+  for (int i = 0; i < cg_max_iter; ++i) {
+    float *pold = (i & 1) ? c.f.p1 : c.f.p0;
+    float *pnew = (i & 1) ? c.f.p0 : c.f.p1;
+    LAUNCH("k_cg_pap", k_cg_pap, dim3(gt), tb, 0, s, c, (const float *)pold, pnew);
+    fin(Lo, c, FIN_PAP, it, s);
+    row_dct2(Lo, c, true, c.f.r, c.f.T, pnew, GUARD_CG, s);
+    fin(Lo, c, FIN_UPDATE, it, s);
+    transpose(Lo, c, 0, GUARD_CG, s);
+    col_pass(Lo, c, GUARD_CG, s);
+    transpose(Lo, c, 1, GUARD_CG, s);
+    row_dct3(Lo, c, c.f.T, c.f.z, DOT_RZ, GUARD_CG, s);
+    fin(Lo, c, FIN_RZ, it, s);
+  }
+}
+
 static void enqueue(const Layout &Lo, const Ctx &c, const float *phase, const float *weight, int32_t *info,
                     const Params &p, hipStream_t s) {
   const int gt = Lo.geo[0].back();
@@ -506,22 +529,7 @@ static void enqueue(const Layout &Lo, const Ctx &c, const float *phase, const fl
     LAUNCH("k_irls_update", k_irls_update, dim3(gt), tb, 0, s, c);
     fin(Lo, c, FIN_IRLS_UPDATE, it, s);
     LAUNCH("k_irls_commit", k_irls_commit, dim3(gt), tb, 0, s, c, it);
-    LAUNCH("k_cg_init", k_cg_init, dim3(gt), tb, 0, s, c);
-    fin(Lo, c, FIN_CG_INIT, it, s);
-    precond(Lo, c, c.f.r, c.f.z, DOT_RZ_INIT, GUARD_CG, s);
-    for (int i = 0; i < p.cg_max_iter; ++i) {
-      float *pold = (i & 1) ? c.f.p1 : c.f.p0;
-      float *pnew = (i & 1) ? c.f.p0 : c.f.p1;
-      LAUNCH("k_cg_pap", k_cg_pap, dim3(gt), tb, 0, s, c, (const float *)pold, pnew);
-      fin(Lo, c, FIN_PAP, it, s);
-      row_dct2(Lo, c, true, c.f.r, c.f.T, pnew, GUARD_CG, s);
-      fin(Lo, c, FIN_UPDATE, it, s);
-      transpose(Lo, c, 0, GUARD_CG, s);
-      col_pass(Lo, c, GUARD_CG, s);
-      transpose(Lo, c, 1, GUARD_CG, s);
-      row_dct3(Lo, c, c.f.T, c.f.z, DOT_RZ, GUARD_CG, s);
-      fin(Lo, c, FIN_RZ, it, s);
-    }
+    pcg(Lo, c, p.cg_max_iter, it, s);
     LAUNCH("k_irls_check", k_irls_check, dim3(gt), tb, 0, s, c);
     fin(Lo, c, FIN_IRLS_CHECK, it, s);
     LAUNCH("k_restore", k_restore, dim3(gt), tb, 0, s, c, it);
@@ -591,6 +599,24 @@ static int set_attrs() {
   return IRLS2D_OK;
 }
 
+// Uploads the plans, tables and descriptors of a layout unless this workspace already holds them.
+static int upload_meta(const Layout &Lo, void *workspace, hipStream_t s) {
+  bool need_upload;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_uploaded.find(workspace);
+    need_upload = (it == g_uploaded.end() || it->second != Lo.sig);
+  }
+  if (need_upload) {
+    std::vector<unsigned char> meta = pack_meta(Lo);
+    HIP_TRY(hipMemcpyAsync((unsigned char *)workspace + Lo.o_meta, meta.data(), meta.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_uploaded[workspace] = Lo.sig;
+  }
+  return IRLS2D_OK;
+}
+
 int solve(int n, const int64_t *H, const int64_t *W, const float *phase, const float *weight, float *out,
           int max_iter, double tol, int cg_max_iter, double cg_tol, double epsilon, void *workspace,
           size_t ws_bytes, int32_t *info, void *stream, unsigned flags) {
@@ -609,19 +635,8 @@ int solve(int n, const int64_t *H, const int64_t *W, const float *phase, const f
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   unsigned char *ws = (unsigned char *)workspace;
-  bool need_upload;
-  {
-    std::lock_guard<std::mutex> lk(g_mu);
-    auto it = g_uploaded.find(workspace);
-    need_upload = (it == g_uploaded.end() || it->second != Lo->sig);
-  }
-  if (need_upload) {
-    std::vector<unsigned char> meta = pack_meta(*Lo);
-    HIP_TRY(hipMemcpyAsync(ws + Lo->o_meta, meta.data(), meta.size(), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    std::lock_guard<std::mutex> lk(g_mu);
-    g_uploaded[workspace] = Lo->sig;
-  }
+  rc = upload_meta(*Lo, workspace, s);
+  if (rc) return rc;
   Params p{max_iter, cg_max_iter, tol, cg_tol, epsilon};
   Ctx c = make_ctx(*Lo, ws, out, p);
   if (!(flags & IRLS2D_FLAG_GRAPH)) {
@@ -684,6 +699,51 @@ int solve(int n, const int64_t *H, const int64_t *W, const float *phase, const f
   return IRLS2D_OK;
 }
 
+static void enqueue_adjoint(const Layout &Lo, const Ctx &c, const float *phase, const float *weight,
+                            const float *u, const float *g, float *gphase, float *gweight, int32_t *info,
+                            int cg_max_iter, hipStream_t s) {
+  const int gt = Lo.geo[0].back();
+  const dim3 tb(TILE_THREADS);
+  LAUNCH("k_state_init", k_state_init, dim3(1), dim3(64), 0, s, c);
+  LAUNCH("k_setup", k_setup, dim3(gt), tb, 0, s, c, phase, weight);
+  fin(Lo, c, FIN_VCOUNT, 0, s);
+  LAUNCH("k_adj_gsum", k_adj_gsum, dim3(gt), tb, 0, s, c, g);
+  fin(Lo, c, FIN_GSUM, 0, s);
+  LAUNCH("k_adj_prep", k_adj_prep, dim3(gt), tb, 0, s, c, u, g, gphase);
+  pcg(Lo, c, cg_max_iter, 0, s);  // lambda accumulates in gphase (c.f.u)
+  LAUNCH("k_adj_grads", k_adj_grads, dim3(gt), tb, 0, s, c, (const float *)gphase, weight, c.f.T, gweight);
+  LAUNCH("k_copy", k_copy, dim3(gt), tb, 0, s, c, (const float *)c.f.T, gphase);
+  if (info) LAUNCH("k_info", k_info, dim3(1), dim3(64), 0, s, c, info);
+}
+
+int adjoint(int n, const int64_t *H, const int64_t *W, const float *phase, const float *weight, const float *u,
+            const float *g, float *gphase, float *gweight, double epsilon, int cg_max_iter, double cg_tol,
+            void *workspace, size_t ws_bytes, int32_t *info, void *stream, unsigned flags) {
+  g_err.clear();
+  if (!phase || !u || !g || !gphase || !workspace) return fail(IRLS2D_EINVAL, "null pointer argument");
+  if (cg_max_iter < 0) return fail(IRLS2D_EINVAL, "cg_max_iter must be >= 0");
+  if (!(epsilon >= 0) || !std::isfinite(epsilon)) return fail(IRLS2D_EINVAL, "epsilon must be finite and >= 0");
+  std::shared_ptr<Layout> Lo;
+  int rc = get_layout(n, H, W, Lo);
+  if (rc) return rc;
+  if (ws_bytes < Lo->bytes) return fail(IRLS2D_EWORKSPACE, "workspace too small");
+  rc = set_attrs();
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  unsigned char *ws = (unsigned char *)workspace;
+  rc = upload_meta(*Lo, workspace, s);
+  if (rc) return rc;
+  Params p{0, cg_max_iter, 0.0, cg_tol, epsilon};
+  Ctx c = make_ctx(*Lo, ws, gphase, p);
+  KTimer timer;
+  if (flags & IRLS2D_FLAG_TIME_KERNELS) t_timer = &timer;
+  enqueue_adjoint(*Lo, c, phase, weight, u, g, gphase, gweight, info, cg_max_iter, s);
+  t_timer = nullptr;
+  HIP_TRY(hipGetLastError());
+  if (flags & IRLS2D_FLAG_TIME_KERNELS) return collect_timer(timer, s);
+  return IRLS2D_OK;
+}
+
 int release(void *workspace) {
   std::lock_guard<std::mutex> lk(g_mu);
   for (auto it = g_graphs.begin(); it != g_graphs.end();) {
@@ -734,6 +794,14 @@ int irls2d_solve(int n, const int64_t *H, const int64_t *W, const float *phase, 
 }
 
 int irls2d_release(void *workspace) { return irls2d::release(workspace); }
+
+int irls2d_adjoint(int n, const int64_t *H, const int64_t *W, const float *phase, const float *weight,
+                   const float *unwrapped, const float *grad_unwrapped, float *grad_phase, float *grad_weight,
+                   double epsilon, int cg_max_iter, double cg_tol, void *workspace, size_t workspace_bytes,
+                   int32_t *info, void *stream, unsigned flags) {
+  return irls2d::adjoint(n, H, W, phase, weight, unwrapped, grad_unwrapped, grad_phase, grad_weight, epsilon,
+                         cg_max_iter, cg_tol, workspace, workspace_bytes, info, stream, flags);
+}
 
 int irls2d_kernel_stats(int max, char *names, double *total_ms, int64_t *launches) {
   std::lock_guard<std::mutex> lk(irls2d::g_stats_mu);

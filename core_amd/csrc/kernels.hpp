@@ -24,6 +24,7 @@ enum FinKind {
   FIN_RZ_INIT,      // k_row_dct3 DOT_RZ_INIT -> rz
   FIN_RZ,           // k_row_dct3 DOT_RZ      -> beta or rz break
   FIN_IRLS_CHECK,   // k_irls_check     -> restore / relative change
+  FIN_GSUM,         // k_adj_gsum       -> mean of the incoming gradient over valid pixels
 };
 
 __device__ __forceinline__ bool guard_off(const ImgState *st, int guard) {
@@ -667,6 +668,7 @@ __global__ __launch_bounds__(256) void k_finalize(Ctx c, int kind, int it) {
       break;
     case FIN_CENTER:
     case FIN_IRLS_MEAN:
+    case FIN_GSUM:
       st->mean = st->vcount > 0 ? acc[0] / st->vcount : 0.0;
       break;
     case FIN_IRLS_UPDATE:
@@ -727,6 +729,94 @@ __global__ __launch_bounds__(256) void k_finalize(Ctx c, int kind, int it) {
     default:
       break;
   }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Implicit adjoint of the IRLS fixed point D^T (w_hat(r) r) = 0, r = D u - t (no reference
+// counterpart).  With H = D^T W' D, W'_e = w_e eps^2 / (r_e^2 + eps^2)^(3/2) on unclamped edges and
+// the clamp value on clamped ones: H lam = C^T g (C = masked-mean centring),
+// dL/dpsi = valid (H lam), dL/dw_e = -(D lam)_e r_e / sqrt(r_e^2 + eps^2) (0 when clamped),
+// chained through the edge average (w_i + w_j)/2 and the clip(coh, 0.01, 1).
+// ---------------------------------------------------------------------------------------------
+
+// sum of the incoming gradient over all pixels
+__global__ __launch_bounds__(TILE_THREADS) void k_adj_gsum(Ctx c, const float *__restrict__ g) {
+  const TilePos t = tile_pos(c);
+  const ImgDesc d = c.img[t.img];
+  double acc[1] = {0.0};
+  for_tile(d, t, [&](int i, int j, long long idx) { acc[0] += (double)g[idx]; });
+  block_partial<1>(acc, c.partials, t.blk);
+}
+
+// W' (-> wxi, wyi), s' = d(w_hat r)/dw (-> u_prev, u_best), b = C^T g (-> b), lambda = 0 (-> lam)
+__global__ __launch_bounds__(TILE_THREADS) void k_adj_prep(Ctx c, const float *__restrict__ u,
+                                                            const float *__restrict__ g, float *__restrict__ lam) {
+  const TilePos t = tile_pos(c);
+  const ImgDesc d = c.img[t.img];
+  const float gm = (float)c.st[t.img].mean;
+  const float eps_sq = c.eps_sq;
+  for_tile(d, t, [&](int i, int j, long long idx) {
+    float wpx = 0.f, spx = 0.f, wpy = 0.f, spy = 0.f;
+    const float uc = u[idx];
+    if (j < d.W - 1) {
+      const float r = (u[idx + 1] - uc) - c.f.tx[idx];
+      const float q = r * r + eps_sq, sq = sqrtf(q), w = c.f.wx[idx];
+      const float what = w / sq;
+      const bool clamped = !(what >= 1e-6f && what <= 1e6f);
+      wpx = clamped ? clampf_nan(what, 1e-6f, 1e6f) : w * eps_sq / (q * sq);
+      spx = clamped ? 0.f : r / sq;
+    }
+    if (i < d.H - 1) {
+      const float r = (u[idx + d.P] - uc) - c.f.ty[idx];
+      const float q = r * r + eps_sq, sq = sqrtf(q), w = c.f.wy[idx];
+      const float what = w / sq;
+      const bool clamped = !(what >= 1e-6f && what <= 1e6f);
+      wpy = clamped ? clampf_nan(what, 1e-6f, 1e6f) : w * eps_sq / (q * sq);
+      spy = clamped ? 0.f : r / sq;
+    }
+    c.f.wxi[idx] = wpx;
+    c.f.wyi[idx] = wpy;
+    c.f.u_prev[idx] = spx;
+    c.f.u_best[idx] = spy;
+    c.f.b[idx] = g[idx] - (c.f.valid[idx] ? gm : 0.f);
+    lam[idx] = 0.f;
+  });
+}
+
+// dL/dpsi into Tout and dL/dweight (optional) from D lam and s'.  The fixed point is
+// translation-equivariant (u*(psi + delta) = u*(psi) + delta for delta vanishing on invalid pixels:
+// every residual D u - t is unchanged), so H lam = C^T g exactly and dL/dpsi = valid C^T g = valid b;
+// it is written from b directly instead of from the float32 product H lam.
+__global__ __launch_bounds__(TILE_THREADS) void k_adj_grads(Ctx c, const float *__restrict__ lam,
+                                                             const float *__restrict__ weight,
+                                                             float *__restrict__ Tout, float *__restrict__ gw) {
+  const TilePos t = tile_pos(c);
+  const ImgDesc d = c.img[t.img];
+  const float *sx = c.f.u_prev, *sy = c.f.u_best;
+  const unsigned char *v = c.f.valid;
+  for_tile(d, t, [&](int i, int j, long long idx) {
+    const bool vc = v[idx] != 0;
+    Tout[idx] = vc ? c.f.b[idx] : 0.f;
+    if (gw) {
+      const float lc = lam[idx];
+      float acc = 0.f;
+      if (vc) {
+        if (j < d.W - 1 && v[idx + 1]) acc += (lam[idx + 1] - lc) * sx[idx];
+        if (j > 0 && v[idx - 1]) acc += (lc - lam[idx - 1]) * sx[idx - 1];
+        if (i < d.H - 1 && v[idx + d.P]) acc += (lam[idx + d.P] - lc) * sy[idx];
+        if (i > 0 && v[idx - d.P]) acc += (lc - lam[idx - d.P]) * sy[idx - d.P];
+      }
+      const float raw = weight ? weight[idx] : 0.f;
+      const bool pass = vc && weight && raw > 0.01f && raw < 1.0f;  // clip(coh, 0.01, 1) derivative
+      gw[idx] = pass ? -0.5f * acc : 0.f;
+    }
+  });
+}
+
+__global__ __launch_bounds__(TILE_THREADS) void k_copy(Ctx c, const float *__restrict__ src, float *__restrict__ dst) {
+  const TilePos t = tile_pos(c);
+  const ImgDesc d = c.img[t.img];
+  for_tile(d, t, [&](int i, int j, long long idx) { dst[idx] = src[idx]; });
 }
 
 __global__ void k_info(Ctx c, int32_t *info) {

@@ -92,6 +92,23 @@ int irls2d_solve(int n, const int64_t *H, const int64_t *W,
                  void *stream, unsigned flags);
 
 /*
+ * Implicit adjoint of the solve (no reference counterpart: the reference is not differentiable).
+ * Given the inputs of irls2d_solve, its output `unwrapped` (the IRLS fixed point u*) and
+ * grad_unwrapped = dL/d(C u*) with C the centring u - mean_valid(u), writes
+ *   grad_phase  = dL/dphase  (0 at NaN pixels),
+ *   grad_weight = dL/dweight (may be NULL; 0 where the clip(weight, 0.01, 1) is flat),
+ * by solving (D^T W' D) lambda = C^T grad with the same DCT-preconditioned CG (cg_max_iter
+ * iterations, absolute tolerance cg_tol), W' = d(w_hat r)/dr of the IRLS weights at u*.
+ * Same arenas and workspace layout as irls2d_solve.
+ */
+int irls2d_adjoint(int n, const int64_t *H, const int64_t *W,
+                   const float *phase, const float *weight, const float *unwrapped,
+                   const float *grad_unwrapped, float *grad_phase, float *grad_weight,
+                   double epsilon, int cg_max_iter, double cg_tol,
+                   void *workspace, size_t workspace_bytes, int32_t *info,
+                   void *stream, unsigned flags);
+
+/*
  * Release cached hipGraphs and plans that refer to `workspace` (NULL = all).  Call before
  * freeing a workspace that was used with IRLS2D_FLAG_GRAPH.
  */
