@@ -27,6 +27,7 @@ EXPORTS = (
     "irls2d_workspace_size",
     "irls2d_solve",
     "irls2d_adjoint",
+    "irls2d_precondition",
     "irls2d_release",
     "irls2d_kernel_stats",
     "irls2d_kernel_stats_reset",
@@ -71,6 +72,9 @@ def load():
         ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint,
     ]
     L.irls2d_adjoint.restype = ctypes.c_int
+    L.irls2d_precondition.argtypes = [ctypes.c_int, i64p, i64p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    L.irls2d_precondition.restype = ctypes.c_int
     L.irls2d_release.argtypes = [ctypes.c_void_p]
     L.irls2d_release.restype = ctypes.c_int
     L.irls2d_kernel_stats.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), i64p]

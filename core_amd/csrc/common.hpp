@@ -28,6 +28,15 @@ constexpr int TW_LO = 64;
 constexpr int TW_LDS = TW_LO + MAX_FFT_LEN / 64;
 // Padded LDS length of a sequence of L complex values (one pad slot per 16).
 constexpr int lds_seq_len(int L) { return L + (L >> 4) + 1; }
+// Lines longer than MAX_FFT_LEN: four-step transform N = N1 N2 in two global passes, each a batch
+// of B short FFTs in LDS (B * factor <= LONG_E * LONG_THREADS).
+constexpr int LONG_THREADS = 512;
+constexpr int LONG_E = 16;
+constexpr int LONG_MAX_FACTOR = 1024;
+constexpr int LONG_BATCH_ELEMS = LONG_E * LONG_THREADS;  // 8192
+constexpr int LONG_MAX_BATCH = 32;
+// LDS of a long pass in float2: twiddles of the factor, then factor x (B + 1) values.
+constexpr int LONG_LDS = TW_LDS + LONG_BATCH_ELEMS + LONG_MAX_FACTOR;
 
 // One Stockham pass, precomputed on the host so the device does no integer division.
 struct PassDesc {
@@ -62,14 +71,36 @@ struct DctPlan {
   FftPlan fft;
 };
 
+// DCT of a line longer than MAX_FFT_LEN: the complex FFT of length N = N1 N2 (Makhoul-packed
+// pair of real lines) as N2 FFTs of length N1 over n1 of x[N2 n1 + n2] (pass 1), the twiddle
+// exp(-2 pi i n2 k1 / N), then N1 FFTs of length N2 over n2 (pass 2); the result for k1 + N1 k2
+// sits at row k1, column k2 of the scratch line.  Pass 1 blocks take a set of columns closed
+// under the reflection that pairs their inputs, pass 2 blocks a set of rows closed under the one
+// that pairs their outputs, so every global element is read and written once, coalesced.
+struct LongPlan {
+  int N, N1, N2;
+  int f1, f2;      // DctPlan indices whose FftPlans transform lengths N1 and N2
+  int tw_off;      // split twiddle table of length N: exp(-2 pi i b / N), b < 64, then exp(-2 pi i 64 a / N)
+  int mk_off;      // N entries exp(-i pi k / 2N)
+  int lb1, lb2;    // log2 of the FFTs per block (slots) in pass 1 and pass 2
+  int nb1, nb2;    // blocks per line pair in pass 1 and pass 2
+  int spitch;      // float2 elements per line pair in the scratch arena
+  float invN;
+  float inv_n2;    // 1 / N2 for float quotients
+  int pad;
+};
+
 struct ImgDesc {
   long long off;   // element offset of the image in every field arena (multiple of 64)
   long long offT;  // element offset of the image in the transposed arena (multiple of 64)
+  long long offS;  // float2 offset of the image in the long-transform scratch arena
   int H, W;
   int P;           // row pitch in elements: W rounded up to a multiple of 64
   int PT;          // row pitch of the transposed image: H rounded up to a multiple of 64
-  int rowplan;     // DctPlan index for length W
-  int colplan;     // DctPlan index for length H
+  int rowplan;     // DctPlan index for length W (-1 when W > MAX_FFT_LEN)
+  int colplan;     // DctPlan index for length H (-1 when H > MAX_FFT_LEN)
+  int rowlong;     // LongPlan index for length W when W > MAX_FFT_LEN, else -1
+  int collong;     // LongPlan index for length H when H > MAX_FFT_LEN, else -1
   int cosH_off;    // double offset of cos(pi i / H), i < H
   int cosW_off;    // double offset of cos(pi j / W), j < W
 };
@@ -105,15 +136,20 @@ struct Fields {
 
 struct Geo {  // block start of each image per launch geometry, [n + 1] each
   const int *tile, *rowpair, *colpair, *trans;
+  const int *rowL1, *rowL2, *colL1, *colL2;  // long-line passes (images with W or H > MAX_FFT_LEN)
 };
+constexpr int NGEO = 8;
 
 struct Ctx {
   const ImgDesc *img;
   ImgState *st;
   const DctPlan *plans;
+  const LongPlan *lplans;
   const float2 *tab;
   const double *dtab;
   double *partials;
+  double *partials2;  // block partials of the long-line passes (their own block numbering)
+  float2 *S;          // long-transform scratch arena
   Geo geo;
   Fields f;
   int n;

@@ -157,6 +157,32 @@ __device__ __forceinline__ float2 csqr(float2 a) {
   return make_float2(a.x * a.x - a.y * a.y, 2.f * a.x * a.y);
 }
 
+// Butterfly j of a Stockham pass works on k = j mod Ns (float quotient when Ns is no power of two).
+__device__ __forceinline__ int stockham_k(int j, const PassDesc &pd) {
+  if (pd.nsmask >= 0) return j & pd.nsmask;
+  const int qt = (int)((float)j * pd.inv_ns);
+  int k = j - __umul24(qt, pd.Ns);
+  if (k < 0) k += pd.Ns;
+  if (k >= pd.Ns) k -= pd.Ns;
+  return k;
+}
+
+// Multiplies butterfly input t by exp(-2 pi i t k / (Ns R)) (no-op in the first pass).
+template <int R>
+__device__ __forceinline__ void stockham_twiddle(float2 *v, const float2 *tws, const PassDesc &pd, int k) {
+  if (pd.Ns == 1) return;
+  const int m1 = __umul24(k, pd.twmul);
+  float2 w[R];
+  w[1] = twiddle(tws, m1);
+#pragma unroll
+  for (int t = 2; t < R; ++t) {
+    // restart from the table every 8 powers, otherwise a short product tree
+    w[t] = (t % 8 == 0) ? twiddle(tws, __umul24(t, m1)) : ((t % 2 == 0) ? csqr(w[t / 2]) : cmul(w[t - 1], w[1]));
+  }
+#pragma unroll
+  for (int t = 1; t < R; ++t) v[t] = cmul(v[t], w[t]);
+}
+
 // One Stockham pass of radix R over a length-L sequence in LDS, Ns = product of earlier radices.
 // Butterfly j reads x[j + t L/R] (t < R), twiddles by exp(-2 pi i t k / (Ns R)) with k = j mod Ns,
 // and writes y[(j - k) R + k + t Ns].  Index arithmetic uses 24-bit multiplies and, when the
@@ -182,27 +208,7 @@ __device__ __forceinline__ void stockham_pass(float2 *buf, const float2 *tws, co
 #pragma unroll
         for (int t = 0; t < R; ++t) v[q][t] = buf[pidx(j + __umul24(t, nb))];
       }
-      if (Ns > 1) {
-        int k;
-        if (pd.nsmask >= 0) {
-          k = j & pd.nsmask;
-        } else {
-          int qt = (int)((float)j * pd.inv_ns);
-          k = j - __umul24(qt, Ns);
-          if (k < 0) k += Ns;
-          if (k >= Ns) k -= Ns;
-        }
-        const int m1 = __umul24(k, pd.twmul);
-        float2 w[R];
-        w[1] = twiddle(tws, m1);
-#pragma unroll
-        for (int t = 2; t < R; ++t) {
-          // restart from the table every 8 powers, otherwise a short product tree
-          w[t] = (t % 8 == 0) ? twiddle(tws, __umul24(t, m1)) : ((t % 2 == 0) ? csqr(w[t / 2]) : cmul(w[t - 1], w[1]));
-        }
-#pragma unroll
-        for (int t = 1; t < R; ++t) v[q][t] = cmul(v[q][t], w[t]);
-      }
+      if (Ns > 1) stockham_twiddle<R>(v[q], tws, pd, stockham_k(j, pd));
       Dft<R>::run(v[q]);
     }
   }
@@ -211,17 +217,7 @@ __device__ __forceinline__ void stockham_pass(float2 *buf, const float2 *tws, co
   for (int q = 0; q < MB; ++q) {
     const int j = tid + q * T;
     if (j < nb) {
-      int k;
-      if (Ns == 1) {
-        k = 0;
-      } else if (pd.nsmask >= 0) {
-        k = j & pd.nsmask;
-      } else {
-        int qt = (int)((float)j * pd.inv_ns);
-        k = j - __umul24(qt, Ns);
-        if (k < 0) k += Ns;
-        if (k >= Ns) k -= Ns;
-      }
+      const int k = (Ns == 1) ? 0 : stockham_k(j, pd);
       const int base = __umul24(j - k, R) + k;
       if (pd.wstride) {
         int a = pidx(base);
@@ -327,29 +323,38 @@ __device__ __forceinline__ void dct2_pair_load(const DctCtx &d, const float *a, 
   dct2_pair_run<E>(d);
 }
 
-// After dct2_pair_run: DCT-II coefficients of both lines at k and at nk = (N - k) mod N, from the
-// two spectrum values Z_k, Z_nk (Hermitian separation + Makhoul post-twiddle).
-__device__ __forceinline__ void dct2_coef_pair(const DctCtx &d, int k, int nk, float &Xak, float &Xbk,
-                                               float &Xan, float &Xbn) {
-  const float2 Zk = dct_Z(d, k);
-  const float2 Zn = dct_Z(d, nk);
-  const float2 wk = d.tab[d.P->mk_off + k];
-  const float2 wn = d.tab[d.P->mk_off + nk];
+// DCT-II coefficients of both lines at k and nk = (N - k) mod N from the spectrum values Z_k, Z_nk
+// and the post-twiddles wk = exp(-i pi k / 2N), wn = exp(-i pi nk / 2N).
+__device__ __forceinline__ void dct2_coef_w(float2 Zk, float2 Zn, float2 wk, float2 wn, float &Xak, float &Xbk,
+                                            float &Xan, float &Xbn) {
   Xak = wk.x * (Zk.x + Zn.x) - wk.y * (Zk.y - Zn.y);
   Xbk = wk.x * (Zk.y + Zn.y) + wk.y * (Zk.x - Zn.x);
   Xan = wn.x * (Zn.x + Zk.x) - wn.y * (Zn.y - Zk.y);
   Xbn = wn.x * (Zn.y + Zk.y) + wn.y * (Zn.x - Zk.x);
 }
 
+// After dct2_pair_run: DCT-II coefficients of both lines at k and at nk = (N - k) mod N, from the
+// two spectrum values Z_k, Z_nk (Hermitian separation + Makhoul post-twiddle).
+__device__ __forceinline__ void dct2_coef_pair(const DctCtx &d, int k, int nk, float &Xak, float &Xbk,
+                                               float &Xan, float &Xbn) {
+  const float2 Zk = dct_Z(d, k);
+  const float2 Zn = dct_Z(d, nk);
+  dct2_coef_w(Zk, Zn, d.tab[d.P->mk_off + k], d.tab[d.P->mk_off + nk], Xak, Xbk, Xan, Xbn);
+}
+
 // DCT-III input step at index k: from X_k and X_{N-k} (zero for k = 0) of both lines, the value
 // conj(V_a + i V_b) (chirp-weighted for Bluestein) whose forward DFT gives N * conj(output).
-__device__ __forceinline__ float2 dct3_pre(const DctCtx &d, int k, float Xak, float Xbk, float Xan, float Xbn) {
+// w = exp(-i pi k / 2N) = (cos t, -sin t), t = pi k / 2N.
+__device__ __forceinline__ float2 dct3_pre_w(float2 w, float Xak, float Xbk, float Xan, float Xbn) {
   const float ya = 0.5f * Xak, yb = 0.5f * Xbk, yan = 0.5f * Xan, ybn = 0.5f * Xbn;
-  const float2 w = d.tab[d.P->mk_off + k];  // (cos t, -sin t), t = pi k / 2N
   const float cx = w.x, cy = -w.y;
   const float vax = cx * ya + cy * yan, vay = cy * ya - cx * yan;
   const float vbx = cx * yb + cy * ybn, vby = cy * yb - cx * ybn;
-  float2 cz = make_float2(vax - vby, -(vay + vbx));
+  return make_float2(vax - vby, -(vay + vbx));
+}
+
+__device__ __forceinline__ float2 dct3_pre(const DctCtx &d, int k, float Xak, float Xbk, float Xan, float Xbn) {
+  float2 cz = dct3_pre_w(d.tab[d.P->mk_off + k], Xak, Xbk, Xan, Xbn);
   if (d.P->blue) cz = cmul(cz, d.tab[d.P->chirp_off + k]);
   return cz;
 }

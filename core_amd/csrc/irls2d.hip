@@ -18,6 +18,7 @@
 
 #include "irls2d.h"
 #include "kernels.hpp"
+#include "long_dct.hpp"
 
 namespace irls2d {
 
@@ -106,9 +107,30 @@ static int64_t bluestein_len(int64_t N) {
   return best;
 }
 
+// Four-step split of a 5-smooth length N > MAX_FFT_LEN: N = N1 N2, both <= LONG_MAX_FACTOR, the
+// most balanced such pair (N1 >= N2).
+static bool long_split(int64_t N, int &N1, int &N2) {
+  if (N <= MAX_FFT_LEN || !smooth235(N)) return false;
+  int64_t best = -1;
+  for (int64_t b = 2; b <= LONG_MAX_FACTOR; ++b) {
+    if (N % b) continue;
+    const int64_t a = N / b;
+    if (a > LONG_MAX_FACTOR || a < b) continue;
+    if (best < 0 || a < best) best = a;
+  }
+  if (best < 0) return false;
+  N1 = (int)best;
+  N2 = (int)(N / best);
+  return true;
+}
+
 int supported(int64_t N) {
   if (N < 1) return 0;
-  if (smooth235(N)) return N <= MAX_FFT_LEN;
+  if (N > MAX_FFT_LEN) {
+    int a, b;
+    return long_split(N, a, b) ? 1 : 0;
+  }
+  if (smooth235(N)) return 1;
   return bluestein_len(N) <= MAX_FFT_LEN;
 }
 
@@ -182,6 +204,56 @@ static std::shared_ptr<HostDct> get_plan(int N) {
   return h;
 }
 
+struct HostLong {
+  LongPlan plan;                // tw_off / mk_off relative to `tab`; f1, f2 set per layout
+  std::vector<float2> tab;
+};
+static std::map<int, std::shared_ptr<HostLong>> g_lplans;
+
+// FFTs per block of a long pass: the largest power of two <= LONG_MAX_BATCH whose batch of
+// length-L sequences fits LONG_BATCH_ELEMS.
+static int batch_log2(int L) {
+  int lb = 0;
+  while ((2 << lb) <= LONG_MAX_BATCH && (int64_t)(2 << lb) * L <= LONG_BATCH_ELEMS) ++lb;
+  return lb;
+}
+
+static std::shared_ptr<HostLong> get_long_plan(int N) {
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  auto it = g_lplans.find(N);
+  if (it != g_lplans.end()) return it->second;
+  auto h = std::make_shared<HostLong>();
+  LongPlan &P = h->plan;
+  std::memset(&P, 0, sizeof(P));
+  long_split(N, P.N1, P.N2);
+  P.N = N;
+  P.invN = (float)(1.0 / (double)N);
+  P.inv_n2 = (float)(1.0 / (double)P.N2);
+  P.tw_off = 0;
+  for (int b = 0; b < TW_LO; ++b) {
+    const double ang = -2.0 * M_PI * (double)b / (double)N;
+    h->tab.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+  }
+  for (int a = 0; a < (N + 63) / 64; ++a) {
+    const double ang = -2.0 * M_PI * (double)(64 * (int64_t)a) / (double)N;
+    h->tab.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+  }
+  P.mk_off = (int)h->tab.size();
+  for (int k = 0; k < N; ++k) {
+    const double ang = -M_PI * (double)k / (2.0 * (double)N);
+    h->tab.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+  }
+  P.lb1 = batch_log2(P.N1);
+  P.lb2 = batch_log2(P.N2);
+  // representatives per block = half the slots; the reflection k -> N - k has the most of them
+  const int C1 = (1 << P.lb1) / 2, C2 = (1 << P.lb2) / 2;
+  P.nb1 = (P.N2 / 2 + 1 + C1 - 1) / C1;
+  P.nb2 = (P.N1 / 2 + 1 + C2 - 1) / C2;
+  P.spitch = (N + 31) / 32 * 32;
+  g_lplans[N] = h;
+  return h;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Batch layout
 // ---------------------------------------------------------------------------------------------
@@ -200,16 +272,19 @@ struct Layout {
   int64_t fstride = 0;  // elements between consecutive workspace fields
   // metadata (host images of the device region)
   std::vector<ImgDesc> desc;
-  std::vector<int> geo[4];
+  std::vector<int> geo[NGEO];  // tile, rowpair, colpair, trans, rowL1, rowL2, colL1, colL2
   std::vector<DctPlan> plans;
+  std::vector<LongPlan> lplans;
   std::vector<float2> tab;
   std::vector<double> dtab;
-  int max_blocks = 0;
+  int max_blocks = 0, max_blocks2 = 0;
+  int64_t totalS = 0;  // float2 elements of the long-transform scratch arena
   int row_threads = 64, col_threads = 64, row_E = 16, col_E = 16;
-  size_t row_lds = 0, col_lds = 0;
+  size_t row_lds = 0, col_lds = 0, long_lds = 0;
   // byte offsets inside the workspace
-  size_t o_fields = 0, o_Tt = 0, o_valid = 0, o_meta = 0, o_desc = 0, o_state = 0, o_geo[4] = {0, 0, 0, 0},
-         o_plans = 0, o_tab = 0, o_dtab = 0, o_partials = 0, bytes = 0, meta_bytes = 0;
+  size_t o_fields = 0, o_Tt = 0, o_S = 0, o_valid = 0, o_meta = 0, o_desc = 0, o_state = 0, o_geo[NGEO] = {},
+         o_plans = 0, o_lplans = 0, o_tab = 0, o_dtab = 0, o_partials = 0, o_partials2 = 0, bytes = 0,
+         meta_bytes = 0;
   uint64_t sig = 0;
 };
 
@@ -260,10 +335,12 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     if (H[i] > (int64_t)1 << 30 || W[i] > (int64_t)1 << 30) return fail(IRLS2D_EINVAL, "dimension too large");
     if (!supported(H[i]) || !supported(W[i]))
       return fail(IRLS2D_EUNSUPPORTED, "grid " + std::to_string(H[i]) + "x" + std::to_string(W[i]) +
-                                           " exceeds the LDS-resident DCT lengths");
+                                           ": sides above " + std::to_string(MAX_FFT_LEN) +
+                                           " must be 2^a 3^b 5^c with two factors <= " +
+                                           std::to_string(LONG_MAX_FACTOR));
   }
   Lo.sig = shape_sig(n, H, W);
-  std::map<int, int> plan_idx, tab_base, cos_off;
+  std::map<int, int> plan_idx, long_idx, cos_off;
   auto add_plan = [&](int N) -> int {
     auto it = plan_idx.find(N);
     if (it != plan_idx.end()) return it->second;
@@ -279,6 +356,25 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     plan_idx[N] = idx;
     return idx;
   };
+  auto add_long = [&](int N) -> int {
+    auto it = long_idx.find(N);
+    if (it != long_idx.end()) return it->second;
+    auto h = get_long_plan(N);
+    LongPlan P = h->plan;
+    const int base = (int)Lo.tab.size();
+    Lo.tab.insert(Lo.tab.end(), h->tab.begin(), h->tab.end());
+    P.tw_off += base;
+    P.mk_off += base;
+    P.f1 = add_plan(P.N1);
+    P.f2 = add_plan(P.N2);
+    Lo.lplans.push_back(P);
+    const int idx = (int)Lo.lplans.size() - 1;
+    long_idx[N] = idx;
+    const size_t l1 = (size_t)(TW_LDS + P.N1 * ((1 << P.lb1) + 1)) * sizeof(float2);
+    const size_t l2 = (size_t)(TW_LDS + P.N2 * ((1 << P.lb2) + 1)) * sizeof(float2);
+    Lo.long_lds = std::max(Lo.long_lds, std::max(l1, l2));
+    return idx;
+  };
   auto add_cos = [&](int N) -> int {
     auto it = cos_off.find(N);
     if (it != cos_off.end()) return it->second;
@@ -287,7 +383,7 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     cos_off[N] = o;
     return o;
   };
-  for (int g = 0; g < 4; ++g) Lo.geo[g].assign(1, 0);
+  for (int g = 0; g < NGEO; ++g) Lo.geo[g].assign(1, 0);
   int maxM_row = 1, maxM_col = 1;
   for (int i = 0; i < n; ++i) {
     const int h = (int)H[i], w = (int)W[i];
@@ -298,23 +394,43 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     d.W = w;
     d.P = (int)Lo.pitch[i];
     d.PT = (int)pitch_of(h);
-    d.rowplan = add_plan(w);
-    d.colplan = add_plan(h);
+    const bool lw = w > MAX_FFT_LEN, lh = h > MAX_FFT_LEN;
+    d.rowplan = lw ? -1 : add_plan(w);
+    d.colplan = lh ? -1 : add_plan(h);
+    d.rowlong = lw ? add_long(w) : -1;
+    d.collong = lh ? add_long(h) : -1;
     d.cosH_off = add_cos(h);
     d.cosW_off = add_cos(w);
+    int64_t sz = 0;
+    int64_t lr1 = 0, lr2 = 0, lc1 = 0, lc2 = 0;
+    if (lw) {
+      const LongPlan &P = Lo.lplans[d.rowlong];
+      sz = std::max<int64_t>(sz, (int64_t)((h + 1) / 2) * P.spitch);
+      lr1 = (int64_t)((h + 1) / 2) * P.nb1;
+      lr2 = (int64_t)((h + 1) / 2) * P.nb2;
+    }
+    if (lh) {
+      const LongPlan &P = Lo.lplans[d.collong];
+      sz = std::max<int64_t>(sz, (int64_t)((w + 1) / 2) * P.spitch);
+      lc1 = (int64_t)((w + 1) / 2) * P.nb1;
+      lc2 = (int64_t)((w + 1) / 2) * P.nb2;
+    }
+    d.offS = Lo.totalS;
+    Lo.totalS += (sz + 31) / 32 * 32;
     Lo.desc.push_back(d);
-    maxM_row = std::max(maxM_row, Lo.plans[d.rowplan].M);
-    maxM_col = std::max(maxM_col, Lo.plans[d.colplan].M);
+    if (!lw) maxM_row = std::max(maxM_row, Lo.plans[d.rowplan].M);
+    if (!lh) maxM_col = std::max(maxM_col, Lo.plans[d.colplan].M);
     const int64_t tiles = ((int64_t)(h + TILE_H - 1) / TILE_H) * ((w + TILE_W - 1) / TILE_W);
     const int64_t ttiles = ((int64_t)(h + TT - 1) / TT) * ((w + TT - 1) / TT);
-    const int64_t geo_add[4] = {tiles, (h + 1) / 2, (w + 1) / 2, ttiles};
-    for (int g = 0; g < 4; ++g) {
+    const int64_t geo_add[NGEO] = {tiles, lw ? 0 : (h + 1) / 2, lh ? 0 : (w + 1) / 2, ttiles, lr1, lr2, lc1, lc2};
+    for (int g = 0; g < NGEO; ++g) {
       const int64_t v = (int64_t)Lo.geo[g].back() + geo_add[g];
       if (v > INT32_MAX) return fail(IRLS2D_EUNSUPPORTED, "too many blocks in one launch");
       Lo.geo[g].push_back((int)v);
     }
   }
   for (int g = 0; g < 4; ++g) Lo.max_blocks = std::max(Lo.max_blocks, Lo.geo[g].back());
+  for (int g = 4; g < NGEO; ++g) Lo.max_blocks2 = std::max(Lo.max_blocks2, Lo.geo[g].back());
   fft_shape(maxM_row, Lo.row_E, Lo.row_threads);
   fft_shape(maxM_col, Lo.col_E, Lo.col_threads);
   Lo.row_lds = (size_t)(TW_LDS + lds_seq_len(maxM_row)) * sizeof(float2);
@@ -326,21 +442,27 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
   o = align_up(o + (size_t)NFIELD * Lo.fstride * sizeof(float), 256);
   Lo.o_Tt = o;
   o = align_up(o + (size_t)(Lo.totalT + FIELD_PAD) * sizeof(float), 256);
+  Lo.o_S = o;
+  o = align_up(o + (size_t)Lo.totalS * sizeof(float2), 256);
   Lo.o_valid = o;
   o = align_up(o + (size_t)Lo.total, 256);
   Lo.o_state = o;
   o = align_up(o + n * sizeof(ImgState), 256);
   Lo.o_partials = o;
   o = align_up(o + (size_t)Lo.max_blocks * MAX_NV * sizeof(double), 256);
+  Lo.o_partials2 = o;
+  o = align_up(o + (size_t)Lo.max_blocks2 * MAX_NV * sizeof(double), 256);
   Lo.o_meta = o;  // everything below is uploaded from the host in one copy
   Lo.o_desc = o;
   o = align_up(o + n * sizeof(ImgDesc), 256);
-  for (int g = 0; g < 4; ++g) {
+  for (int g = 0; g < NGEO; ++g) {
     Lo.o_geo[g] = o;
     o = align_up(o + (n + 1) * sizeof(int), 256);
   }
   Lo.o_plans = o;
   o = align_up(o + Lo.plans.size() * sizeof(DctPlan), 256);
+  Lo.o_lplans = o;
+  o = align_up(o + Lo.lplans.size() * sizeof(LongPlan), 256);
   Lo.o_tab = o;
   o = align_up(o + Lo.tab.size() * sizeof(float2), 256);
   Lo.o_dtab = o;
@@ -356,8 +478,9 @@ static std::vector<unsigned char> pack_meta(const Layout &Lo) {
     if (nbytes) std::memcpy(buf.data() + (off - Lo.o_meta), src, nbytes);
   };
   put(Lo.o_desc, Lo.desc.data(), Lo.desc.size() * sizeof(ImgDesc));
-  for (int g = 0; g < 4; ++g) put(Lo.o_geo[g], Lo.geo[g].data(), Lo.geo[g].size() * sizeof(int));
+  for (int g = 0; g < NGEO; ++g) put(Lo.o_geo[g], Lo.geo[g].data(), Lo.geo[g].size() * sizeof(int));
   put(Lo.o_plans, Lo.plans.data(), Lo.plans.size() * sizeof(DctPlan));
+  put(Lo.o_lplans, Lo.lplans.data(), Lo.lplans.size() * sizeof(LongPlan));
   put(Lo.o_tab, Lo.tab.data(), Lo.tab.size() * sizeof(float2));
   put(Lo.o_dtab, Lo.dtab.data(), Lo.dtab.size() * sizeof(double));
   return buf;
@@ -418,6 +541,9 @@ static Ctx make_ctx(const Layout &Lo, unsigned char *ws, float *u, const Params 
   c.img = (const ImgDesc *)(ws + Lo.o_desc);
   c.st = (ImgState *)(ws + Lo.o_state);
   c.plans = (const DctPlan *)(ws + Lo.o_plans);
+  c.lplans = (const LongPlan *)(ws + Lo.o_lplans);
+  c.partials2 = (double *)(ws + Lo.o_partials2);
+  c.S = (float2 *)(ws + Lo.o_S);
   c.tab = (const float2 *)(ws + Lo.o_tab);
   c.dtab = (const double *)(ws + Lo.o_dtab);
   c.partials = (double *)(ws + Lo.o_partials);
@@ -425,6 +551,10 @@ static Ctx make_ctx(const Layout &Lo, unsigned char *ws, float *u, const Params 
   c.geo.rowpair = (const int *)(ws + Lo.o_geo[1]);
   c.geo.colpair = (const int *)(ws + Lo.o_geo[2]);
   c.geo.trans = (const int *)(ws + Lo.o_geo[3]);
+  c.geo.rowL1 = (const int *)(ws + Lo.o_geo[4]);
+  c.geo.rowL2 = (const int *)(ws + Lo.o_geo[5]);
+  c.geo.colL1 = (const int *)(ws + Lo.o_geo[6]);
+  c.geo.colL2 = (const int *)(ws + Lo.o_geo[7]);
   float *F = (float *)(ws + Lo.o_fields);
   float *fld[NFIELD];
   for (int k = 0; k < NFIELD; ++k) fld[k] = F + (size_t)k * Lo.fstride;
@@ -447,8 +577,40 @@ static void fin(const Layout &Lo, const Ctx &c, int kind, int it, hipStream_t s)
 }
 
 // Transform launches, dispatched on the elements-per-thread template parameter.
+// Long lines (W or H > MAX_FFT_LEN) run the two four-step passes instead; images of both kinds
+// can share a batch, each launch covering the images of its kind.
+static void long_pass1(const Layout &Lo, const Ctx &c, int dir, int mode, float *src, const float *p, int guard,
+                       hipStream_t s) {
+  const dim3 g(Lo.geo[dir ? 6 : 4].back()), b(LONG_THREADS);
+  const size_t l = Lo.long_lds;
+  switch (mode) {
+    case LONG_DCT2: LAUNCH("k_long_p1_dct2", (k_long_p1<LONG_DCT2, LONG_E>), g, b, l, s, c, dir, src, p, guard); break;
+    case LONG_DCT2_UPDATE:
+      LAUNCH("k_long_p1_dct2_update", (k_long_p1<LONG_DCT2_UPDATE, LONG_E>), g, b, l, s, c, dir, src, p, guard);
+      break;
+    case LONG_DCT3: LAUNCH("k_long_p1_dct3", (k_long_p1<LONG_DCT3, LONG_E>), g, b, l, s, c, dir, src, p, guard); break;
+    default:
+      LAUNCH("k_long_p1_dct3_scale", (k_long_p1<LONG_DCT3_SCALE, LONG_E>), g, b, l, s, c, dir, src, p, guard);
+      break;
+  }
+}
+
+static void long_pass2(const Layout &Lo, const Ctx &c, int dir, int mode, float *dst, int dot, int guard,
+                       hipStream_t s) {
+  const dim3 g(Lo.geo[dir ? 7 : 5].back()), b(LONG_THREADS);
+  if (mode == LONG_DCT2)
+    LAUNCH("k_long_p2_dct2", (k_long_p2<LONG_DCT2, LONG_E>), g, b, Lo.long_lds, s, c, dir, dst, dot, guard);
+  else
+    LAUNCH("k_long_p2_dct3", (k_long_p2<LONG_DCT3, LONG_E>), g, b, Lo.long_lds, s, c, dir, dst, dot, guard);
+}
+
 static void row_dct2(const Layout &Lo, const Ctx &c, bool fuse, float *src, float *dst, const float *p, int guard,
                      hipStream_t s) {
+  if (Lo.geo[4].back() > 0) {
+    long_pass1(Lo, c, 0, fuse ? LONG_DCT2_UPDATE : LONG_DCT2, src, p, guard, s);
+    long_pass2(Lo, c, 0, LONG_DCT2, dst, 0, guard, s);
+  }
+  if (Lo.geo[1].back() == 0) return;
   const dim3 g(Lo.geo[1].back()), b(Lo.row_threads);
   if (fuse) {
     if (Lo.row_E == 16) LAUNCH("k_row_dct2_update", (k_row_dct2<true, 16>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
@@ -460,12 +622,24 @@ static void row_dct2(const Layout &Lo, const Ctx &c, bool fuse, float *src, floa
 }
 
 static void row_dct3(const Layout &Lo, const Ctx &c, const float *src, float *dst, int dot, int guard, hipStream_t s) {
+  if (Lo.geo[4].back() > 0) {
+    long_pass1(Lo, c, 0, LONG_DCT3, const_cast<float *>(src), nullptr, guard, s);
+    long_pass2(Lo, c, 0, LONG_DCT3, dst, dot, guard, s);
+  }
+  if (Lo.geo[1].back() == 0) return;
   const dim3 g(Lo.geo[1].back()), b(Lo.row_threads);
   if (Lo.row_E == 16) LAUNCH("k_row_dct3", (k_row_dct3<16>), g, b, Lo.row_lds, s, c, src, dst, dot, guard);
   else LAUNCH("k_row_dct3", (k_row_dct3<24>), g, b, Lo.row_lds, s, c, src, dst, dot, guard);
 }
 
 static void col_pass(const Layout &Lo, const Ctx &c, int guard, hipStream_t s) {
+  if (Lo.geo[6].back() > 0) {
+    long_pass1(Lo, c, 1, LONG_DCT2, c.f.Tt, nullptr, guard, s);
+    long_pass2(Lo, c, 1, LONG_DCT2, c.f.Tt, 0, guard, s);
+    long_pass1(Lo, c, 1, LONG_DCT3_SCALE, c.f.Tt, nullptr, guard, s);
+    long_pass2(Lo, c, 1, LONG_DCT3, c.f.Tt, 0, guard, s);
+  }
+  if (Lo.geo[2].back() == 0) return;
   const dim3 g(Lo.geo[2].back()), b(Lo.col_threads);
   if (Lo.col_E == 16) LAUNCH("k_col", (k_col<16>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
   else LAUNCH("k_col", (k_col<24>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
@@ -595,6 +769,11 @@ static int set_attrs() {
                        (const void *)k_col<16>,             (const void *)k_col<24>,
                        (const void *)k_row_dct3<16>,        (const void *)k_row_dct3<24>};
   for (const void *f : fns) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+  const int llds = LONG_LDS * (int)sizeof(float2);
+  const void *lfns[] = {(const void *)k_long_p1<LONG_DCT2, LONG_E>, (const void *)k_long_p1<LONG_DCT2_UPDATE, LONG_E>,
+                        (const void *)k_long_p1<LONG_DCT3, LONG_E>, (const void *)k_long_p1<LONG_DCT3_SCALE, LONG_E>,
+                        (const void *)k_long_p2<LONG_DCT2, LONG_E>, (const void *)k_long_p2<LONG_DCT3, LONG_E>};
+  for (const void *f : lfns) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, llds));
   if (dev >= 0 && dev < 64) g_attr_done[dev] = true;
   return IRLS2D_OK;
 }
@@ -744,6 +923,28 @@ int adjoint(int n, const int64_t *H, const int64_t *W, const float *phase, const
   return IRLS2D_OK;
 }
 
+int precondition(int n, const int64_t *H, const int64_t *W, const float *r, float *z, void *workspace,
+                 size_t ws_bytes, void *stream) {
+  g_err.clear();
+  if (!r || !z || !workspace) return fail(IRLS2D_EINVAL, "null pointer argument");
+  if (r == z) return fail(IRLS2D_EINVAL, "r and z must not alias");
+  std::shared_ptr<Layout> Lo;
+  int rc = get_layout(n, H, W, Lo);
+  if (rc) return rc;
+  if (ws_bytes < Lo->bytes) return fail(IRLS2D_EWORKSPACE, "workspace too small");
+  rc = set_attrs();
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  rc = upload_meta(*Lo, workspace, s);
+  if (rc) return rc;
+  Params p{0, 0, 0.0, 0.0, 0.0};
+  Ctx c = make_ctx(*Lo, (unsigned char *)workspace, z, p);
+  // the row DCT-II only reads its source
+  precond(*Lo, c, const_cast<float *>(r), z, DOT_NONE, GUARD_NONE, s);
+  HIP_TRY(hipGetLastError());
+  return IRLS2D_OK;
+}
+
 int release(void *workspace) {
   std::lock_guard<std::mutex> lk(g_mu);
   for (auto it = g_graphs.begin(); it != g_graphs.end();) {
@@ -794,6 +995,11 @@ int irls2d_solve(int n, const int64_t *H, const int64_t *W, const float *phase, 
 }
 
 int irls2d_release(void *workspace) { return irls2d::release(workspace); }
+
+int irls2d_precondition(int n, const int64_t *H, const int64_t *W, const float *r, float *z, void *workspace,
+                        size_t workspace_bytes, void *stream) {
+  return irls2d::precondition(n, H, W, r, z, workspace, workspace_bytes, stream);
+}
 
 int irls2d_adjoint(int n, const int64_t *H, const int64_t *W, const float *phase, const float *weight,
                    const float *unwrapped, const float *grad_unwrapped, float *grad_phase, float *grad_weight,

@@ -649,7 +649,11 @@ __global__ __launch_bounds__(256) void k_finalize(Ctx c, int kind, int it) {
     guard = GUARD_CG;
   if (guard_off(st, guard)) return;
   const bool rowgeo = (kind == FIN_UPDATE || kind == FIN_RZ_INIT || kind == FIN_RZ);
-  const int *geo = rowgeo ? c.geo.rowpair : c.geo.tile;
+  // rows longer than MAX_FFT_LEN: the partials come from the long passes (update: pass 1 of the
+  // DCT-II, r.z: pass 2 of the DCT-III) in their own region
+  const bool lng = rowgeo && c.img[img].rowlong >= 0;
+  const int *geo = rowgeo ? (lng ? (kind == FIN_UPDATE ? c.geo.rowL1 : c.geo.rowL2) : c.geo.rowpair) : c.geo.tile;
+  const double *parts = lng ? c.partials2 : c.partials;
   const int b0 = geo[img], b1 = geo[img + 1];
   int nv = 1;
   if (kind == FIN_IRLS_UPDATE || kind == FIN_UPDATE) nv = 2;
@@ -658,7 +662,7 @@ __global__ __launch_bounds__(256) void k_finalize(Ctx c, int kind, int it) {
   for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
 #pragma unroll
     for (int q = 0; q < MAX_NV; ++q)
-      if (q < nv) acc[q] += c.partials[(long long)b * MAX_NV + q];
+      if (q < nv) acc[q] += parts[(long long)b * MAX_NV + q];
   }
   block_sum<MAX_NV>(acc);
   if (threadIdx.x != 0) return;

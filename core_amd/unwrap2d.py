@@ -19,7 +19,8 @@ import torch
 from . import _lib
 from . import postprocess as pp
 
-__all__ = ["irls_unwrap_2d", "irls_unwrap_2d_batch", "solve_device", "resolve_device", "Workspace"]
+__all__ = ["irls_unwrap_2d", "irls_unwrap_2d_batch", "solve_device", "precondition_device", "resolve_device",
+           "Workspace"]
 
 # The reference serialises GPU solves per process with one lock (Stack_unwrap2d.py:20, :63);
 # the same lock guards the shared workspaces here.
@@ -118,6 +119,26 @@ def solve_device(phase: torch.Tensor, weight, shapes, max_iter=50, tol=1e-3, cg_
                         (_lib.FLAG_GRAPH if graph else 0) | (_lib.FLAG_TIME_KERNELS if time_kernels else 0))
     _lib.check(rc, "irls2d_solve")
     return out, ws.info.view(n, 4)
+
+
+def precondition_device(r: torch.Tensor, shapes, out: torch.Tensor | None = None, workspace=None):
+    """z = IDCT2(-DCT2(r) / lambda), DC zeroed, per image of an arena (the reference's
+    apply_preconditioner closure, utils_unwrap2d.py:1390-1403).  Returns the z arena."""
+    if r.dtype != torch.float32 or r.device.type != 'cuda' or not r.is_contiguous():
+        raise ValueError("r must be a contiguous float32 HIP tensor")
+    L = _lib.load()
+    ws = workspace or Workspace.get(shapes, r.device)
+    if r.numel() < ws.total:
+        raise ValueError(f"r arena has {r.numel()} elements, layout needs {ws.total}")
+    if out is None:
+        out = torch.empty(ws.total, dtype=torch.float32, device=r.device)
+    elif out.numel() < ws.total or out.dtype != torch.float32 or not out.is_contiguous():
+        raise ValueError("out arena too small or not contiguous float32")
+    n, H, W = _lib.shapes_arrays(ws.shapes)
+    stream = torch.cuda.current_stream(r.device).cuda_stream
+    _lib.check(L.irls2d_precondition(n, H, W, r.data_ptr(), out.data_ptr(), ws.buf.data_ptr(), ws.nbytes, stream),
+               "irls2d_precondition")
+    return out
 
 
 def image_view(arena, ws, k):

@@ -57,7 +57,9 @@ int irls2d_version(void);
 /* Message of the last error on the calling thread ("" if none). */
 const char *irls2d_last_error(void);
 
-/* Longest grid side the solver handles for a DCT length N (1 = supported). */
+/* 1 if a grid side of length N is supported: N <= 12288 (LDS-resident DCT, Bluestein for prime
+   factors above 5), or a larger N = 2^a 3^b 5^c that splits into two factors <= 1024 (two-pass
+   four-step DCT). */
 int irls2d_supported_length(int64_t n);
 
 /*
@@ -107,6 +109,17 @@ int irls2d_adjoint(int n, const int64_t *H, const int64_t *W,
                    double epsilon, int cg_max_iter, double cg_tol,
                    void *workspace, size_t workspace_bytes, int32_t *info,
                    void *stream, unsigned flags);
+
+/*
+ * Poisson preconditioner alone: z = IDCT2(-DCT2(r) / lambda) with the DC term zeroed, for every
+ * image (replaces apply_preconditioner, utils_unwrap2d.py:1390-1403, with the DCT pair of
+ * :1321-1365 and the eigenvalues of :1284-1289).  r and z are device arenas laid out by
+ * irls2d_arena_layout (they may not alias); same workspace as irls2d_solve.  This is the
+ * transform step irls2d_solve applies twice per PCG iteration; exported for callers that need
+ * the unweighted Poisson solve and to test the DCT engine on its own.
+ */
+int irls2d_precondition(int n, const int64_t *H, const int64_t *W, const float *r, float *z,
+                        void *workspace, size_t workspace_bytes, void *stream);
 
 /*
  * Release cached hipGraphs and plans that refer to `workspace` (NULL = all).  Call before

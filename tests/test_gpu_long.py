@@ -1,0 +1,150 @@
+"""GPU parity of the four-step (two-pass) DCT used for sides longer than the LDS-resident limit
+(12288): rows, columns, odd factors, odd line counts (a last line without a partner), mixed
+batches of long and short images, graph replay.
+
+Bars: cycle counts agree with the float32 reference-CPU path on >= 99.99% of valid pixels, and
+RMS(phi_gpu - phi_fp64) is within the float32 envelope: <= max(1e-3, 2 x the largest
+RMS(phi_fp32 - phi_fp64) over the input and six copies perturbed by uniform +-1e-6 rad noise).
+On grids this elongated the smoothest modes along the long side have eigenvalues ~(pi/N)^2 ~ 6e-8
+and a faulted field makes the IRLS weights span many decades; a float32 solve (the reference's own
+CPU path included) then moves by up to ~3e-2 rad when its input moves by a few ulp, and more
+PCG iterations move it further from FP64, not closer; a fixed 1e-3 bar would fail the reference
+itself.  On square grids the same perturbation moves it by ~2e-5 and the plain 1e-3 bar of
+test_gpu_parity.py applies.  The transform itself is pinned without this noise by
+test_preconditioner_matches_scipy_fp64 (relative error ~1e-6 for every length class)."""
+import numpy as np
+import pytest
+
+from core_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+ITERS = dict(max_iter=5, tol=0.0, cg_max_iter=40, cg_tol=0.0)
+RMS_TOL = 1e-3
+K_AGREE = 0.9999
+
+
+def _case(H, W, seed, fault=False, mask=0.0, ramp=60.0):
+    # the ramp runs across the width: keep its gradient <= 0.5 rad/px so narrow grids stay
+    # unwrappable (no aliased gradients)
+    wl = S.Workload(H, W, seed, ramp_max=min(ramp, 0.5 * W), n_bumps=6, fault=fault, mask_frac=mask)
+    psi, coh, _ = S.generate(wl)
+    return psi, coh
+
+
+def _rms(u, ref, psi):
+    v = ~np.isnan(psi)
+    d = u[v].astype(np.float64) - ref[v].astype(np.float64)
+    return float(np.sqrt(np.mean(d * d)))
+
+
+def _kagree(u, ref, psi):
+    v = ~np.isnan(psi)
+    k1 = np.round((u[v].astype(np.float64) - psi[v]) / (2 * np.pi))
+    k2 = np.round((ref[v].astype(np.float64) - psi[v]) / (2 * np.pi))
+    return float(np.mean(k1 == k2))
+
+
+LONG_CASES = [
+    (24, 12500, 30, True, 0.05),   # rows 125 x 100, fault + masked blocks
+    (9, 15625, 31, False, 0.0),    # rows 125 x 125 (both factors odd), odd row count
+    (12800, 20, 32, False, 0.0),   # columns 128 x 100
+    (13122, 15, 33, True, 0.0),    # columns 162 x 81 (odd factor), odd column count
+]
+
+
+@pytest.mark.parametrize("H,W,seed,fault,mask", LONG_CASES)
+def test_long_side_parity(hip, H, W, seed, fault, mask):
+    from core_amd import irls_unwrap_2d
+    from oracle import irls_fp64, irls_torch32
+    print(f"case {H}x{W} seed={seed}", flush=True)
+    psi, coh = _case(H, W, seed, fault, mask)
+    u_gpu, cc_gpu = irls_unwrap_2d(psi, coh, device='cuda', **ITERS)
+    u64, cc64 = irls_fp64.irls_unwrap_2d(psi, coh, **ITERS)
+    u32, _ = irls_torch32.irls_unwrap_2d(psi, coh, **ITERS)
+    envelope = _rms(u32, u64, psi)
+    rng = np.random.default_rng(seed)
+    for _ in range(6):  # the float32 path's own spread under +-1e-6 rad input perturbations
+        psi_p = (psi + rng.uniform(-1e-6, 1e-6, size=psi.shape)).astype(np.float32)
+        up, _ = irls_torch32.irls_unwrap_2d(psi_p, coh, **ITERS)
+        envelope = max(envelope, _rms(up, u64, psi))
+    rms = _rms(u_gpu, u64, psi)
+    ka = _kagree(u_gpu, u32, psi)
+    print(f"  rms_vs_fp64={rms:.3e} k_agree_vs_fp32={ka:.6f} rms_fp32_vs_fp64={_rms(u32, u64, psi):.3e} "
+          f"fp32_envelope={envelope:.3e}", flush=True)
+    assert np.array_equal(np.isnan(u_gpu), np.isnan(psi))
+    assert rms <= max(RMS_TOL, 2 * envelope)
+    assert ka >= K_AGREE
+    assert np.array_equal(cc_gpu, cc64)
+
+
+def test_long_mixed_batch_equals_plain(hip):
+    from core_amd import irls_unwrap_2d, irls_unwrap_2d_batch
+    cases = [_case(20, 12800, 40), _case(12500, 18, 41), _case(50, 70, 42, True, 0.05)]
+    print("mixed batch 20x12800, 12500x18, 50x70", flush=True)
+    batch = irls_unwrap_2d_batch([c[0] for c in cases], [c[1] for c in cases], device='cuda', **ITERS)
+    for (psi, coh), (ub, cb) in zip(cases, batch):
+        up, cp = irls_unwrap_2d(psi, coh, device='cuda', **ITERS)
+        assert np.array_equal(ub, up, equal_nan=True)
+        assert np.array_equal(cb, cp)
+
+
+def test_long_graph_equals_eager(hip):
+    import torch
+    from core_amd import solve_device
+    from core_amd.unwrap2d import Workspace, image_view, pack
+    shapes = [(16, 13500), (12960, 12)]
+    print(f"graph replay {shapes}", flush=True)
+    ims = [_case(h, w, 50 + i) for i, (h, w) in enumerate(shapes)]
+    ws = Workspace.get(shapes, hip)
+    ph = pack([im[0] for im in ims], ws, hip)
+    co = pack([im[1] for im in ims], ws, hip)
+
+    def images(u):
+        return torch.cat([image_view(u, ws, k).reshape(-1) for k in range(len(shapes))])
+
+    u_e, info_e = solve_device(ph, co, shapes, graph=False, workspace=ws, **ITERS)
+    u_e = images(u_e).clone()
+    info_e = info_e.clone()
+    out = torch.empty(ws.total, device=hip)
+    for rep in range(3):
+        u_g, info_g = solve_device(ph, co, shapes, graph=True, workspace=ws, out=out, **ITERS)
+        torch.cuda.synchronize()
+        assert torch.equal(images(u_g), u_e), f"replay {rep}"
+        assert torch.equal(info_g, info_e)
+        _ = (ph * 2).sum().item()
+        out.fill_(-1.0)
+    ws.release()
+
+
+PRECOND_SHAPES = [(64, 48), (127, 211), (1, 1000), (16, 12500), (12800, 8), (9, 15625), (12, 40000)]
+
+
+def test_preconditioner_matches_scipy_fp64(hip):
+    """The DCT engine alone (LDS-resident, Bluestein and four-step lengths) against scipy in FP64:
+    z = idctn(-dctn(r) / lambda), DC = 0, lambda = 2cos(pi i/H) + 2cos(pi j/W) - 4."""
+    import scipy.fft as sf
+    import torch
+    from core_amd.unwrap2d import Workspace, image_view, pack, precondition_device
+    rng = np.random.default_rng(5)
+    rs = []
+    for h, w in PRECOND_SHAPES:
+        y, x = np.mgrid[0:h, 0:w]
+        smooth = np.sin(2 * np.pi * x / max(w, 2) * 3) + np.cos(2 * np.pi * y / max(h, 2) * 2)
+        rs.append((rng.normal(size=(h, w)) + 5 * smooth).astype(np.float32))
+    print(f"preconditioner {PRECOND_SHAPES}", flush=True)
+    ws = Workspace.get(PRECOND_SHAPES, hip)
+    z = precondition_device(pack(rs, ws, hip), PRECOND_SHAPES, workspace=ws)
+    zh = z.cpu().numpy()
+    for k, ((h, w), r) in enumerate(zip(PRECOND_SHAPES, rs)):
+        lam = (2 * np.cos(np.pi * np.arange(h) / h)[:, None] + 2 * np.cos(np.pi * np.arange(w) / w)[None, :] - 4)
+        lam[0, 0] = 1.0
+        X = sf.dctn(r.astype(np.float64), type=2) / -lam
+        X[0, 0] = 0.0
+        z64 = sf.idctn(X, type=2)
+        zg = image_view(zh, ws, k).astype(np.float64)
+        # max error relative to max |z|; scipy's own float32 dctn/idctn reaches 2e-7 .. 6e-7 here
+        err = np.abs(zg - z64).max() / np.abs(z64).max()
+        print(f"  {h}x{w}: max rel err {err:.2e}", flush=True)
+        assert err < 2e-5, (h, w, err)
+    ws.release()

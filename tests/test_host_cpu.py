@@ -56,8 +56,22 @@ def test_workspace_size_grows_with_pixels(lib):
 def test_supported_lengths(lib):
     assert lib.supported_length(8192) and lib.supported_length(12288)
     assert lib.supported_length(4999)            # Bluestein
-    assert not lib.supported_length(12289)       # beyond the LDS-resident lengths
+    assert not lib.supported_length(12289)       # prime beyond the LDS-resident lengths
+    assert lib.supported_length(40000)           # four-step 200 x 200
+    assert lib.supported_length(12500) and lib.supported_length(15625) and lib.supported_length(13122)
+    assert not lib.supported_length(2 * 12289)   # long and not 5-smooth
+    assert not lib.supported_length(2 ** 21)     # no split into two factors <= 1024
+    assert lib.supported_length(2 ** 20)
     assert not lib.supported_length(0)
+
+
+def test_workspace_size_long_sides(lib):
+    # a long side adds a complex scratch line per line pair (8 B per pair element = 4 B per pixel)
+    short = lib.workspace_size([(24, 12000)])
+    long_ = lib.workspace_size([(24, 12500)])
+    assert long_ > short * 12500 / 12000 + 12 * 12500 * 8 * 0.99
+    both = lib.workspace_size([(12500, 12800)])
+    assert both >= 17 * 12500 * 12800 * 4
 
 
 def test_argument_errors_without_gpu(lib):
