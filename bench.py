@@ -15,6 +15,8 @@ Also reported:
   batched       48 uneven SBAS-burst interferograms (H ~ U{1000..1500}, W ~ U{4000..6000},
                 seeds 1000 + i) solved in one batched graph launch vs 48 sequential plain calls.
   adjoint       16 x 2048 x 2048 forward solve and implicit adjoint (dL/dphase, dL/dweight).
+  max_scale     (--max-scale / --only-max-scale) 40000 x 40000 solve with two-pass transforms,
+                per-kernel GB/s, and cycle-correctness against the generator's noiseless phase.
   cpu_baseline  the float32 torch-CPU restatement of the reference solver (oracle.irls_torch32)
                 on a bounded sample of the same workload, on this host's cores.
 """
@@ -39,6 +41,20 @@ ALG_BPX = {
     "k_col": 8,               # Tt -> Tt
     "k_row_dct3": 12,         # T, r -> z
     "k_transpose": 8,         # T -> Tt / Tt -> T
+}
+# The same for the two-pass transforms of sides above 12288 (the 40000 x 40000 workload); the
+# scratch line holds one complex value per pixel pair, i.e. 4 B per pixel each way.
+ALG_BPX_LONG = {
+    "k_long_p1_dct2_update": 28,  # x, p, r, Ap -> x, r, scratch
+    "k_long_p2_dct2_rows": 8,     # scratch -> T
+    "k_long_p1_dct2_cols": 8,     # Tt -> scratch
+    "k_long_p2_dct2_cols": 8,     # scratch -> Tt
+    "k_long_p1_dct3_scale": 8,    # Tt -> scratch
+    "k_long_p2_dct3_cols": 8,     # scratch -> Tt
+    "k_long_p1_dct3_rows": 8,     # T -> scratch
+    "k_long_p2_dct3_rows": 12,    # scratch, r -> z
+    "k_cg_pap": 24,
+    "k_transpose": 8,
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_8192.json")
 
@@ -199,6 +215,73 @@ def adjoint_batch(args, dev):
     }
 
 
+def max_scale(args, dev):
+    """40000 x 40000 plain solve (sides above the LDS-resident DCT lengths: two-pass transforms),
+    10 IRLS x 50 PCG from a captured graph, plus an event-timed eager solve for per-kernel GB/s.
+    No oracle runs at this size; the result is checked against the generator's noiseless phase:
+    after removing one constant, the share of valid pixels within pi of it (cycle-correct)."""
+    import numpy as np
+    import torch
+    from core_amd import solve_device, synthetic as S, _lib
+    from core_amd.unwrap2d import Workspace
+    wl = S.single_40000()
+    t0 = time.time()
+
+    def progress(done, total):
+        if done % 4096 < 1024 or done == total:
+            print(f"max_scale: generated {done}/{total} rows ({time.time() - t0:.0f} s)", file=sys.stderr, flush=True)
+
+    psi, coh, true = S.generate(wl, band_rows=1024, progress=progress)
+    gen_s = time.time() - t0
+    shapes = [(wl.H, wl.W)]
+    ws = Workspace(shapes, dev)
+    ph = upload(ws, [psi], dev)
+    co = upload(ws, [coh], dev)
+    tr = upload(ws, [true], dev)
+    valid = upload(ws, [(~np.isnan(psi)).astype(np.float32)], dev) > 0.5
+    del psi, coh, true
+    out = torch.empty(ws.total, dtype=torch.float32, device=dev)
+    kw = dict(max_iter=N_IRLS, tol=0.0, cg_max_iter=N_CG, cg_tol=0.0, out=out, workspace=ws)
+    print(f"max_scale: inputs in HBM, workspace {ws.nbytes / 2**30:.1f} GiB; graph solve", file=sys.stderr, flush=True)
+    ms, _ = time_steps(lambda: solve_device(ph, co, shapes, graph=True, **kw), 1, 1)
+    info = ws.info.cpu().tolist()[:4]
+    d = (out - tr)[valid]
+    sub = d[torch.randint(0, d.numel(), (4_000_000,), device=dev)]
+    c = sub.median()
+    dd = d - c
+    cyc = float((dd.abs() < np.pi).float().mean())
+    rms = float(dd.pow(2).mean().sqrt())
+    finite = bool(torch.isfinite(out[valid]).all())
+    del d, dd, sub
+    print(f"max_scale: {ms / 1e3:.1f} s per solve; event-timed eager solve", file=sys.stderr, flush=True)
+    _lib.kernel_stats_reset()
+    solve_device(ph, co, shapes, graph=False, time_kernels=True, **kw)
+    stats = _lib.kernel_stats()
+    npx = wl.H * wl.W
+    kernels = {}
+    for k, (tot, n) in sorted(stats.items(), key=lambda kv: -kv[1][0]):
+        e = {"total_ms": round(tot, 1), "launches": n, "mean_us": round(tot / n * 1e3, 1)}
+        if k in ALG_BPX_LONG:
+            e["gbs"] = round(ALG_BPX_LONG[k] * npx / (tot / n / 1e3) / 1e9, 1)
+        kernels[k] = e
+    peak_gib = torch.cuda.max_memory_allocated(dev) / 2**30
+    ws.release()
+    del ws, ph, co, tr, valid, out
+    torch.cuda.empty_cache()
+    return {
+        "workload": "40000x40000, ramp 0-1500 rad, 20 bumps, 3 pi fault, 5% masked blocks, seed 4; "
+                    "10 IRLS x 50 PCG; four-step two-pass DCT (200 x 200) both ways",
+        "pixels": npx, "ms_per_solve": round(ms, 1), "mpx_s": round(npx / ms / 1e3, 2),
+        "solve_model_frac_hbm": round(npx * model_bytes_per_px() / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "info": info, "finite": finite,
+        "vs_truth": {"cycle_correct_frac": round(cyc, 6), "rms_rad": round(rms, 4),
+                     "note": "the 3 pi fault wraps to a pi step, so any unwrapper leaves one side of it "
+                             "2 pi from the noiseless phase; the FP64 oracle scores 0.73 on a faulted "
+                             "512x512 grid of this generator and 1.0 without the fault"},
+        "generate_s": round(gen_s, 1), "torch_peak_gib": round(peak_gib, 1), "kernels": kernels,
+    }
+
+
 def cpu_baseline(psi, coh, rows=2048, cols=2048):
     """Reference CPU path (float32 torch restatement) on the top-left rows x cols of the workload."""
     import numpy as np
@@ -231,6 +314,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--max-scale", action="store_true",
+                    help="also solve the 40000 x 40000 workload (about 4 minutes, ~140 GB of HBM)")
+    ap.add_argument("--only-max-scale", action="store_true", help="run only the 40000 x 40000 workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -243,6 +329,9 @@ def main():
     import core_amd._lib as L
     L.load()
     dev = torch.device("cuda", 0)
+    if args.only_max_scale:
+        print(json.dumps({"max_scale": max_scale(args, dev)}), flush=True)
+        return
     npx = 8192 * 8192
     psi, coh, ms, wall_ms, info, stats = plain_8192(args, dev)
     dom, roof = roofline(stats, npx)
@@ -276,6 +365,8 @@ def main():
         result["batched"] = batched_stack(args, dev)
     if not args.no_adjoint:
         result["adjoint"] = adjoint_batch(args, dev)
+    if args.max_scale:
+        result["max_scale"] = max_scale(args, dev)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(psi, coh)
     print(json.dumps(result), flush=True)

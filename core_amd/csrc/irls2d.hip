@@ -584,11 +584,17 @@ static void long_pass1(const Layout &Lo, const Ctx &c, int dir, int mode, float 
   const dim3 g(Lo.geo[dir ? 6 : 4].back()), b(LONG_THREADS);
   const size_t l = Lo.long_lds;
   switch (mode) {
-    case LONG_DCT2: LAUNCH("k_long_p1_dct2", (k_long_p1<LONG_DCT2, LONG_E>), g, b, l, s, c, dir, src, p, guard); break;
+    case LONG_DCT2:
+      LAUNCH(dir ? "k_long_p1_dct2_cols" : "k_long_p1_dct2_rows", (k_long_p1<LONG_DCT2, LONG_E>), g, b, l, s, c,
+             dir, src, p, guard);
+      break;
     case LONG_DCT2_UPDATE:
       LAUNCH("k_long_p1_dct2_update", (k_long_p1<LONG_DCT2_UPDATE, LONG_E>), g, b, l, s, c, dir, src, p, guard);
       break;
-    case LONG_DCT3: LAUNCH("k_long_p1_dct3", (k_long_p1<LONG_DCT3, LONG_E>), g, b, l, s, c, dir, src, p, guard); break;
+    case LONG_DCT3:
+      LAUNCH(dir ? "k_long_p1_dct3_cols" : "k_long_p1_dct3_rows", (k_long_p1<LONG_DCT3, LONG_E>), g, b, l, s, c,
+             dir, src, p, guard);
+      break;
     default:
       LAUNCH("k_long_p1_dct3_scale", (k_long_p1<LONG_DCT3_SCALE, LONG_E>), g, b, l, s, c, dir, src, p, guard);
       break;
@@ -599,9 +605,11 @@ static void long_pass2(const Layout &Lo, const Ctx &c, int dir, int mode, float 
                        hipStream_t s) {
   const dim3 g(Lo.geo[dir ? 7 : 5].back()), b(LONG_THREADS);
   if (mode == LONG_DCT2)
-    LAUNCH("k_long_p2_dct2", (k_long_p2<LONG_DCT2, LONG_E>), g, b, Lo.long_lds, s, c, dir, dst, dot, guard);
+    LAUNCH(dir ? "k_long_p2_dct2_cols" : "k_long_p2_dct2_rows", (k_long_p2<LONG_DCT2, LONG_E>), g, b, Lo.long_lds,
+           s, c, dir, dst, dot, guard);
   else
-    LAUNCH("k_long_p2_dct3", (k_long_p2<LONG_DCT3, LONG_E>), g, b, Lo.long_lds, s, c, dir, dst, dot, guard);
+    LAUNCH(dir ? "k_long_p2_dct3_cols" : "k_long_p2_dct3_rows", (k_long_p2<LONG_DCT3, LONG_E>), g, b, Lo.long_lds,
+           s, c, dir, dst, dot, guard);
 }
 
 static void row_dct2(const Layout &Lo, const Ctx &c, bool fuse, float *src, float *dst, const float *p, int guard,

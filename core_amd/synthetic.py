@@ -29,6 +29,8 @@ from __future__ import annotations
 
 from dataclasses import dataclass, field
 
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 TWO_PI = 2.0 * np.pi
@@ -57,8 +59,10 @@ def wrap_f64(x: np.ndarray) -> np.ndarray:
     return w
 
 
-def generate(wl: Workload, band_rows: int = 2048):
-    """Return (psi float32[H,W] with NaN holes, coh float32[H,W], true float32[H,W])."""
+def generate(wl: Workload, band_rows: int = 2048, workers: int | None = None, progress=None):
+    """Return (psi float32[H,W] with NaN holes, coh float32[H,W], true float32[H,W]).
+
+    `progress(rows_done, H)` is called after each band of the phase field when given."""
     H, W = int(wl.H), int(wl.W)
     rng = np.random.default_rng(wl.seed)
     bumps = []
@@ -79,19 +83,51 @@ def generate(wl: Workload, band_rows: int = 2048):
     true32 = np.empty((H, W), np.float32)
     x = np.arange(W, dtype=np.float64)[None, :]
     ramp = wl.ramp_max * x / max(W - 1, 1)
-    for r0 in range(0, H, band_rows):
-        r1 = min(H, r0 + band_rows)
+
+    def deterministic(r0, r1):
         y = np.arange(r0, r1, dtype=np.float64)[:, None]
         t = np.broadcast_to(ramp, (r1 - r0, W)).copy()
         for (A, s, cy, cx) in bumps:
-            t += A * np.exp(-((y - cy) ** 2 + (x - cx) ** 2) / (2.0 * s * s))
+            # exp(-d^2 / 2s^2) is exactly 0.0 in float64 beyond d = 38.6 s (argument < -745), so
+            # evaluating it only inside that window leaves every value bit-identical
+            R = 39.0 * s + 1.0
+            i0, i1 = max(r0, int(np.floor(cy - R))), min(r1, int(np.ceil(cy + R)) + 1)
+            j0, j1 = max(0, int(np.floor(cx - R))), min(W, int(np.ceil(cx + R)) + 1)
+            if i0 >= i1 or j0 >= j1:
+                continue
+            yy = y[i0 - r0:i1 - r0]
+            xx = x[:, j0:j1]
+            t[i0 - r0:i1 - r0, j0:j1] += A * np.exp(-((yy - cy) ** 2 + (xx - cx) ** 2) / (2.0 * s * s))
         if fault is not None:
             theta, fy, fx = fault
             t += 3.0 * np.pi * (((x - fx) * np.sin(theta) - (y - fy) * np.cos(theta)) > 0)
         true32[r0:r1] = t
-        t += rng.normal(0.0, wl.noise, size=(r1 - r0, W))
+        return t
+
+    def finish(r0, r1, t, noise):
+        t += noise
         psi[r0:r1] = wrap_f64(t)
-        del t
+
+    # The random draws stay in stream order on this thread; the deterministic field and the
+    # wrap run on worker threads (numpy releases the GIL), a bounded number of bands ahead.
+    bands = [(r0, min(H, r0 + band_rows)) for r0 in range(0, H, band_rows)]
+    nw = max(1, min(workers or 8, len(bands)))
+    with ThreadPoolExecutor(nw) as ex:
+        det = {}
+        fin = []
+        for k, (r0, r1) in enumerate(bands):
+            for kk in range(k, min(len(bands), k + nw)):
+                if kk not in det:
+                    det[kk] = ex.submit(deterministic, *bands[kk])
+            t = det.pop(k).result()
+            noise = rng.normal(0.0, wl.noise, size=(r1 - r0, W))
+            fin.append(ex.submit(finish, r0, r1, t, noise))
+            while len(fin) > nw:
+                fin.pop(0).result()
+            if progress is not None:
+                progress(r1, H)
+        for f in fin:
+            f.result()
 
     coh = np.empty((H, W), np.float32)
     for r0 in range(0, H, band_rows):
