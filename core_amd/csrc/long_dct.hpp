@@ -99,6 +99,8 @@ __device__ __forceinline__ float2 long_twiddle(const float2 *__restrict__ tw, in
   return cmul(tw[TW_LO + (m >> 6)], tw[m & 63]);
 }
 
+static_assert(LONG_THREADS % LONG_MAX_BATCH == 0, "a thread must keep its slot across rows");
+
 struct LongLine {
   const LongPlan *L;
   int img, lp, blk;  // image, line pair, block within the line pair
@@ -145,6 +147,10 @@ __device__ __forceinline__ void st2(float *p, long long m, float2 v, bool two) {
 // partial sums of r.r and of non-finite x.  LONG_DCT3(_SCALE): DCT-III pre-step from the
 // coefficients at k and N - k; _SCALE first divides them by -lambda (the column pass of the
 // preconditioner, utils_unwrap2d.py:1395-1398) with lambda's DC entry giving 0.
+// Global loads are issued in chunks of LONG_U rows before any of them is used, so that each
+// thread keeps several requests in flight.
+constexpr int LONG_U = 4;
+
 template <int MODE, int E>
 __global__ __launch_bounds__(LONG_THREADS) void k_long_p1(Ctx c, int dir, float *src, const float *__restrict__ p,
                                                           int guard) {
@@ -157,82 +163,144 @@ __global__ __launch_bounds__(LONG_THREADS) void k_long_p1(Ctx c, int dir, float 
   float2 *tws = lds, *buf = lds + TW_LDS;
   load_tws(tws, c.tab, F);
   constexpr bool dct2 = (MODE == LONG_DCT2 || MODE == LONG_DCT2_UPDATE);
+  constexpr int U = LONG_U;
   const int o = dct2 ? N2 - 1 : 0;
   const float *a = src + ln.la;
   const float *b = src + ln.la + ln.pitch;
   const int T = blockDim.x;
   double acc[2] = {0.0, 0.0};
-  for (int idx = threadIdx.x; idx < (N1 << lb); idx += T) {
-    const int s = idx & (B - 1), n1 = idx >> lb;
-    const int col = slot_index(s, C, ln.blk, N2, o);
-    if (col < 0) continue;
-    const int sp = slot_partner(s, C, col, N2, o);
-    const int n = N2 * n1 + col;
-    if (dct2) {
-      if (2 * n >= N) continue;  // filled by the partner element N - 1 - n
-      const long long m = 2 * (long long)n;
-      const bool two = 2 * n + 1 < N;
-      float2 va = ld2(a, m), vb = ln.hasb ? ld2(b, m) : make_float2(0.f, 0.f);
-      if (MODE == LONG_DCT2_UPDATE) {
-        const ImgState *st = &c.st[ln.img];
-        const float alpha = (float)st->alpha;
-        float *x = c.f.u;
-        const float *Ap = c.f.Ap;
+  // blockDim.x is a multiple of B, so a thread keeps its slot (one FFT of the batch) and walks
+  // rows by T / B.  Only rows n1 <= N1 / 2 hold canonical elements (2n < N, or k <= N - k); the
+  // others are filled through the partner.
+  const int s = threadIdx.x & (B - 1), rstep = T >> lb;
+  const int col = slot_index(s, C, ln.blk, N2, o);
+  const int sp = col >= 0 ? slot_partner(s, C, col, N2, o) : 0;
+  const int n1_end = col >= 0 ? N1 / 2 + 1 : 0;
+  if (dct2) {
+    const float alpha = (MODE == LONG_DCT2_UPDATE) ? (float)c.st[ln.img].alpha : 0.f;
+    float *x = c.f.u;
+    const float *Ap = c.f.Ap;
+    for (int n0 = threadIdx.x >> lb; n0 < n1_end; n0 += U * rstep) {
+      float2 v[U][2], xv[U][2], pv[U][2], av[U][2];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int n1 = n0 + u * rstep;
+        const int n = N2 * n1 + col;
+        const long long m = 2 * (long long)n;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-          if (h == 1 && !ln.hasb) break;
-          const long long e = ln.la + (long long)h * ln.pitch + m;
-          const float2 xv = ld2(x, e), pv = ld2(p, e), av = ld2(Ap, e);
-          float2 rv = h ? vb : va;
-          const float2 xn = make_float2(xv.x + alpha * pv.x, xv.y + alpha * pv.y);
-          rv = make_float2(rv.x - alpha * av.x, rv.y - alpha * av.y);
-          st2(x, e, xn, two);
-          st2(src, e, rv, two);
-          acc[0] += (double)rv.x * rv.x + (two ? (double)rv.y * rv.y : 0.0);
-          acc[1] += (isfinite(xn.x) ? 0.0 : 1.0) + ((two && !isfinite(xn.y)) ? 1.0 : 0.0);
-          if (h) vb = rv; else va = rv;
+          v[u][h] = xv[u][h] = pv[u][h] = av[u][h] = make_float2(0.f, 0.f);
+          if (n1 < n1_end && 2 * n < N && (h == 0 || ln.hasb)) {
+            const long long e = ln.la + (long long)h * ln.pitch + m;
+            v[u][h] = ld2(src, e);
+            if (MODE == LONG_DCT2_UPDATE) {
+              xv[u][h] = ld2(x, e);
+              pv[u][h] = ld2(p, e);
+              av[u][h] = ld2(Ap, e);
+            }
+          }
         }
       }
-      buf[n1 * BS + s] = make_float2(va.x, vb.x);
-      if (two) buf[(N1 - 1 - n1) * BS + sp] = make_float2(va.y, vb.y);
-    } else {
-      const int k = n;
-      const int nk = k ? N - k : 0;
-      if (nk < k) continue;  // done by the partner
-      float Xak = a[k], Xbk = ln.hasb ? b[k] : 0.f;
-      float Xan = k ? a[nk] : 0.f, Xbn = (k && ln.hasb) ? b[nk] : 0.f;
-      if (MODE == LONG_DCT3_SCALE) {
-        // lines are image columns ja, ja + 1 of length H = N; lambda = 2cos(pi k/H) + 2cos(pi j/W) - 4
-        const ImgDesc &d = c.img[ln.img];
-        const double *cosH = c.dtab + d.cosH_off;
-        const int ja = 2 * ln.lp;
-        const double cwa = 2.0 * c.dtab[d.cosW_off + ja];
-        const double cwb = ln.hasb ? 2.0 * c.dtab[d.cosW_off + ja + 1] : 0.0;
-        const double chk = 2.0 * cosH[k], chn = 2.0 * cosH[nk];
-        Xak = (k == 0 && ja == 0) ? 0.f : -(Xak * __builtin_amdgcn_rcpf((float)(chk + cwa - 4.0)));
-        Xbk = ln.hasb ? -(Xbk * __builtin_amdgcn_rcpf((float)(chk + cwb - 4.0))) : 0.f;
-        if (k) {
-          Xan = -(Xan * __builtin_amdgcn_rcpf((float)(chn + cwa - 4.0)));
-          Xbn = ln.hasb ? -(Xbn * __builtin_amdgcn_rcpf((float)(chn + cwb - 4.0))) : 0.f;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int n1 = n0 + u * rstep;
+        const int n = N2 * n1 + col;
+        if (!(n1 < n1_end && 2 * n < N)) continue;
+        const long long m = 2 * (long long)n;
+        const bool two = 2 * n + 1 < N;
+        if (MODE == LONG_DCT2_UPDATE) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            if (h == 1 && !ln.hasb) break;
+            const long long e = ln.la + (long long)h * ln.pitch + m;
+            const float2 xn = make_float2(xv[u][h].x + alpha * pv[u][h].x, xv[u][h].y + alpha * pv[u][h].y);
+            const float2 rv = make_float2(v[u][h].x - alpha * av[u][h].x, v[u][h].y - alpha * av[u][h].y);
+            st2(x, e, xn, two);
+            st2(src, e, rv, two);
+            acc[0] += (double)rv.x * rv.x + (two ? (double)rv.y * rv.y : 0.0);
+            acc[1] += (isfinite(xn.x) ? 0.0 : 1.0) + ((two && !isfinite(xn.y)) ? 1.0 : 0.0);
+            v[u][h] = rv;
+          }
+        }
+        buf[n1 * BS + s] = make_float2(v[u][0].x, v[u][1].x);
+        if (two) buf[(N1 - 1 - n1) * BS + sp] = make_float2(v[u][0].y, v[u][1].y);
+      }
+    }
+  } else {
+    const float2 *mk = c.tab + L.mk_off;
+    const ImgDesc &d = c.img[ln.img];
+    const double *cosH = c.dtab + d.cosH_off;
+    const int ja = 2 * ln.lp;
+    // lines are image columns ja, ja + 1 of length H = N; lambda = 2cos(pi k/H) + 2cos(pi j/W) - 4
+    const double cwa = (MODE == LONG_DCT3_SCALE) ? 2.0 * c.dtab[d.cosW_off + ja] : 0.0;
+    const double cwb = (MODE == LONG_DCT3_SCALE && ln.hasb) ? 2.0 * c.dtab[d.cosW_off + ja + 1] : 0.0;
+    for (int n0 = threadIdx.x >> lb; n0 < n1_end; n0 += U * rstep) {
+      float X[U][4];
+      float2 wk[U], wn[U];
+      double ck[U], cn[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int n1 = n0 + u * rstep;
+        const int k = N2 * n1 + col;
+        const int nk = k ? N - k : 0;
+        X[u][0] = X[u][1] = X[u][2] = X[u][3] = 0.f;
+        wk[u] = wn[u] = make_float2(1.f, 0.f);
+        ck[u] = cn[u] = 0.0;
+        if (n1 < n1_end && (k == 0 || 2 * k <= N)) {
+          X[u][0] = a[k];
+          X[u][1] = ln.hasb ? b[k] : 0.f;
+          X[u][2] = k ? a[nk] : 0.f;
+          X[u][3] = (k && ln.hasb) ? b[nk] : 0.f;
+          wk[u] = mk[k];
+          wn[u] = mk[nk];
+          if (MODE == LONG_DCT3_SCALE) {
+            ck[u] = 2.0 * cosH[k];
+            cn[u] = 2.0 * cosH[nk];
+          }
         }
       }
-      const float2 *mk = c.tab + L.mk_off;
-      buf[n1 * BS + s] = dct3_pre_w(mk[k], Xak, Xbk, Xan, Xbn);
-      if (nk != k) {
-        const int n1p = (col == 0) ? N1 - n1 : N1 - 1 - n1;  // row of nk = N2 n1' + refl(col)
-        buf[n1p * BS + sp] = dct3_pre_w(mk[nk], Xan, Xbn, Xak, Xbk);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int n1 = n0 + u * rstep;
+        const int k = N2 * n1 + col;
+        if (!(n1 < n1_end && (k == 0 || 2 * k <= N))) continue;
+        const int nk = k ? N - k : 0;
+        float Xak = X[u][0], Xbk = X[u][1], Xan = X[u][2], Xbn = X[u][3];
+        if (MODE == LONG_DCT3_SCALE) {
+          Xak = (k == 0 && ja == 0) ? 0.f : -(Xak * __builtin_amdgcn_rcpf((float)(ck[u] + cwa - 4.0)));
+          Xbk = ln.hasb ? -(Xbk * __builtin_amdgcn_rcpf((float)(ck[u] + cwb - 4.0))) : 0.f;
+          if (k) {
+            Xan = -(Xan * __builtin_amdgcn_rcpf((float)(cn[u] + cwa - 4.0)));
+            Xbn = ln.hasb ? -(Xbn * __builtin_amdgcn_rcpf((float)(cn[u] + cwb - 4.0))) : 0.f;
+          }
+        }
+        buf[n1 * BS + s] = dct3_pre_w(wk[u], Xak, Xbk, Xan, Xbn);
+        if (nk != k) {
+          const int n1p = (col == 0) ? N1 - n1 : N1 - 1 - n1;  // row of nk = N2 n1' + refl(col)
+          buf[n1p * BS + sp] = dct3_pre_w(wn[u], Xan, Xbn, Xak, Xbk);
+        }
       }
     }
   }
   __syncthreads();
   batch_fft<E>(buf, BS, lb, tws, F);
   const float2 *tw = c.tab + L.tw_off;
-  for (int idx = threadIdx.x; idx < (N1 << lb); idx += T) {
-    const int s = idx & (B - 1), k1 = idx >> lb;
-    const int col = slot_index(s, C, ln.blk, N2, o);
-    if (col < 0) continue;
-    const float2 v = buf[k1 * BS + s];
-    ln.S[k1 * N2 + col] = (k1 && col) ? cmul(v, long_twiddle(tw, k1 * col)) : v;
+  for (int k0 = col >= 0 ? (int)(threadIdx.x >> lb) : N1; k0 < N1; k0 += U * rstep) {
+    float2 v[U], w[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k1 = k0 + u * rstep;
+      v[u] = w[u] = make_float2(1.f, 0.f);
+      if (k1 < N1) {
+        v[u] = buf[k1 * BS + s];
+        if (k1 && col) w[u] = long_twiddle(tw, k1 * col);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k1 = k0 + u * rstep;
+      if (k1 < N1) ln.S[k1 * N2 + col] = (k1 && col) ? cmul(v[u], w[u]) : v[u];
+    }
   }
   if (MODE == LONG_DCT2_UPDATE) block_partial<2>(acc, c.partials2);
 }
@@ -251,61 +319,104 @@ __global__ __launch_bounds__(LONG_THREADS) void k_long_p2(Ctx c, int dir, float 
   float2 *tws = lds, *buf = lds + TW_LDS;
   load_tws(tws, c.tab, F);
   constexpr bool dct2 = (MODE == LONG_DCT2);
+  constexpr int U = LONG_U;
   const int o = dct2 ? 0 : N1 - 1;
   const int T = blockDim.x;
   // rows of the scratch line, each a contiguous run of N2 values
-  for (int idx = threadIdx.x; idx < (N2 << lb); idx += T) {
-    int s = (int)((float)idx * L.inv_n2);
-    int i = idx - s * N2;
-    if (i < 0) { --s; i += N2; }
-    if (i >= N2) { ++s; i -= N2; }
-    const int k1 = slot_index(s, C, ln.blk, N1, o);
-    if (k1 < 0) continue;
-    buf[i * BS + s] = ln.S[k1 * N2 + i];
+  for (int i0 = threadIdx.x; i0 < (N2 << lb); i0 += U * T) {
+    float2 v[U];
+    int pos[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int idx = i0 + u * T;
+      pos[u] = -1;
+      v[u] = make_float2(0.f, 0.f);
+      if (idx < (N2 << lb)) {
+        int q = (int)((float)idx * L.inv_n2);
+        int i = idx - q * N2;
+        if (i < 0) { --q; i += N2; }
+        if (i >= N2) { ++q; i -= N2; }
+        const int row = slot_index(q, C, ln.blk, N1, o);
+        if (row >= 0) {
+          v[u] = ln.S[row * N2 + i];
+          pos[u] = i * BS + q;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (pos[u] >= 0) buf[pos[u]] = v[u];
   }
   __syncthreads();
   batch_fft<E>(buf, BS, lb, tws, F);
   float *da = dst + ln.la, *db = dst + ln.la + ln.pitch;
   double acc[1] = {0.0};
-  for (int idx = threadIdx.x; idx < (N2 << lb); idx += T) {
-    const int s = idx & (B - 1), k2 = idx >> lb;
-    const int k1 = slot_index(s, C, ln.blk, N1, o);
-    if (k1 < 0) continue;
-    const int sp = slot_partner(s, C, k1, N1, o);
-    const int k = k1 + N1 * k2;
+  const int s = threadIdx.x & (B - 1), rstep = T >> lb;
+  const int k1 = slot_index(s, C, ln.blk, N1, o);
+  const int sp = k1 >= 0 ? slot_partner(s, C, k1, N1, o) : 0;
+  // canonical elements (k <= N - k, or 2n < N) all have k2 <= N2 / 2
+  const int k2_end = k1 >= 0 ? N2 / 2 + 1 : 0;
+  const float2 *mk = c.tab + L.mk_off;
+  for (int q0 = threadIdx.x >> lb; q0 < k2_end; q0 += U * rstep) {
     if (dct2) {
-      const int nk = k ? N - k : 0;
-      if (nk < k) continue;
-      // nk = k1' + N1 k2' with k1' = refl(k1) and k2' = -k2 mod N2 (k1 = 0) or N2 - 1 - k2
-      const int k2p = k1 ? N2 - 1 - k2 : (k2 ? N2 - k2 : 0);
-      const float2 Zk = buf[k2 * BS + s], Zn = buf[k2p * BS + sp];
-      const float2 *mk = c.tab + L.mk_off;
-      float Xak, Xbk, Xan, Xbn;
-      dct2_coef_w(Zk, Zn, mk[k], mk[nk], Xak, Xbk, Xan, Xbn);
-      da[k] = Xak;
-      if (ln.hasb) db[k] = Xbk;
-      if (nk != k) {
-        da[nk] = Xan;
-        if (ln.hasb) db[nk] = Xbn;
+      float2 wk[U], wn[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k2 = q0 + u * rstep;
+        const int k = k1 + N1 * k2;
+        wk[u] = wn[u] = make_float2(1.f, 0.f);
+        if (k2 < k2_end && (k == 0 || 2 * k <= N)) {
+          wk[u] = mk[k];
+          wn[u] = mk[k ? N - k : 0];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k2 = q0 + u * rstep;
+        const int k = k1 + N1 * k2;
+        if (!(k2 < k2_end && (k == 0 || 2 * k <= N))) continue;
+        const int nk = k ? N - k : 0;
+        // nk = k1' + N1 k2' with k1' = refl(k1) and k2' = -k2 mod N2 (k1 = 0) or N2 - 1 - k2
+        const int k2p = k1 ? N2 - 1 - k2 : (k2 ? N2 - k2 : 0);
+        const float2 Zk = buf[k2 * BS + s], Zn = buf[k2p * BS + sp];
+        float Xak, Xbk, Xan, Xbn;
+        dct2_coef_w(Zk, Zn, wk[u], wn[u], Xak, Xbk, Xan, Xbn);
+        da[k] = Xak;
+        if (ln.hasb) db[k] = Xbk;
+        if (nk != k) {
+          da[nk] = Xan;
+          if (ln.hasb) db[nk] = Xbn;
+        }
       }
     } else {
-      const int n = k;
-      if (2 * n >= N) continue;  // written with its partner N - 1 - n
-      const bool two = 2 * n + 1 < N;
-      const float2 w0 = buf[k2 * BS + s];
-      const float2 w1 = two ? buf[(N2 - 1 - k2) * BS + sp] : make_float2(0.f, 0.f);
-      const float2 va = make_float2(w0.x * L.invN, w1.x * L.invN);
-      const float2 vb = make_float2(-w0.y * L.invN, -w1.y * L.invN);
-      const long long m = 2 * (long long)n;
-      st2(da, m, va, two);
-      if (ln.hasb) st2(db, m, vb, two);
-      if (dot) {
-        const float *r = c.f.r + ln.la;
-        const float2 ra = ld2(r, m);
-        acc[0] += (double)ra.x * va.x + (two ? (double)ra.y * va.y : 0.0);
-        if (ln.hasb) {
-          const float2 rb = ld2(r + ln.pitch, m);
-          acc[0] += (double)rb.x * vb.x + (two ? (double)rb.y * vb.y : 0.0);
+      float2 ra[U], rb[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k2 = q0 + u * rstep;
+        const int n = k1 + N1 * k2;
+        ra[u] = rb[u] = make_float2(0.f, 0.f);
+        if (dot && k2 < k2_end && 2 * n < N) {
+          const float *r = c.f.r + ln.la;
+          ra[u] = ld2(r, 2 * (long long)n);
+          if (ln.hasb) rb[u] = ld2(r + ln.pitch, 2 * (long long)n);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k2 = q0 + u * rstep;
+        const int n = k1 + N1 * k2;
+        if (!(k2 < k2_end && 2 * n < N)) continue;  // written with its partner N - 1 - n
+        const bool two = 2 * n + 1 < N;
+        const float2 w0 = buf[k2 * BS + s];
+        const float2 w1 = two ? buf[(N2 - 1 - k2) * BS + sp] : make_float2(0.f, 0.f);
+        const float2 va = make_float2(w0.x * L.invN, w1.x * L.invN);
+        const float2 vb = make_float2(-w0.y * L.invN, -w1.y * L.invN);
+        const long long m = 2 * (long long)n;
+        st2(da, m, va, two);
+        if (ln.hasb) st2(db, m, vb, two);
+        if (dot) {
+          acc[0] += (double)ra[u].x * va.x + (two ? (double)ra[u].y * va.y : 0.0);
+          if (ln.hasb) acc[0] += (double)rb[u].x * vb.x + (two ? (double)rb[u].y * vb.y : 0.0);
         }
       }
     }
