@@ -329,6 +329,8 @@ def main():
     import core_amd._lib as L
     L.load()
     dev = torch.device("cuda", 0)
+    from core_amd import autotune
+    n_tuned = autotune.apply()  # plans measured by `python -m core_amd.autotune` on this GPU type
     if args.only_max_scale:
         print(json.dumps({"max_scale": max_scale(args, dev)}), flush=True)
         return
@@ -350,7 +352,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "plain 8192x8192 float32, ramp 0-300 rad, 6 bumps, 3pi fault, 5% masked 16x16 blocks, seed 1",
-            "irls_iters": N_IRLS, "pcg_iters": N_CG, "pixels": npx, "graph": True,
+            "irls_iters": N_IRLS, "pcg_iters": N_CG, "pixels": npx, "graph": True, "autotuned_plans": n_tuned,
             "solver_info": {"irls_iters": info[1], "pcg_iters": info[2], "valid_px": info[3]},
         },
         "roofline": roof,

@@ -28,6 +28,9 @@ EXPORTS = (
     "irls2d_solve",
     "irls2d_adjoint",
     "irls2d_precondition",
+    "irls2d_plan_override",
+    "irls2d_plan_describe",
+    "irls2d_plan_reset",
     "irls2d_release",
     "irls2d_kernel_stats",
     "irls2d_kernel_stats_reset",
@@ -75,6 +78,13 @@ def load():
     L.irls2d_precondition.argtypes = [ctypes.c_int, i64p, i64p, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
     L.irls2d_precondition.restype = ctypes.c_int
+    i32p = ctypes.POINTER(ctypes.c_int)
+    L.irls2d_plan_override.argtypes = [ctypes.c_int64, ctypes.c_int64, i32p, ctypes.c_int, ctypes.c_int]
+    L.irls2d_plan_override.restype = ctypes.c_int
+    L.irls2d_plan_describe.argtypes = [ctypes.c_int64, i64p, i32p, i32p, i32p]
+    L.irls2d_plan_describe.restype = ctypes.c_int
+    L.irls2d_plan_reset.argtypes = []
+    L.irls2d_plan_reset.restype = ctypes.c_int
     L.irls2d_release.argtypes = [ctypes.c_void_p]
     L.irls2d_release.restype = ctypes.c_int
     L.irls2d_kernel_stats.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), i64p]
@@ -141,3 +151,26 @@ def kernel_stats():
 
 def kernel_stats_reset():
     load().irls2d_kernel_stats_reset()
+
+
+def plan_describe(n: int):
+    """(fft_len, radices, ept) of the transform plan used for DCT length n."""
+    L = load()
+    fl = ctypes.c_int64()
+    rad = (ctypes.c_int * 16)()
+    nr = ctypes.c_int()
+    ept = ctypes.c_int()
+    check(L.irls2d_plan_describe(int(n), ctypes.byref(fl), rad, ctypes.byref(nr), ctypes.byref(ept)),
+          "irls2d_plan_describe")
+    return int(fl.value), [int(rad[i]) for i in range(nr.value)], int(ept.value)
+
+
+def plan_override(n: int, fft_len: int, radices=(), ept: int = 0):
+    """Set (fft_len > 0) or remove (fft_len = 0) the transform plan of DCT length n."""
+    L = load()
+    rad = (ctypes.c_int * max(1, len(radices)))(*[int(r) for r in radices])
+    check(L.irls2d_plan_override(int(n), int(fft_len), rad, len(radices), int(ept)), "irls2d_plan_override")
+
+
+def plan_reset():
+    check(load().irls2d_plan_reset(), "irls2d_plan_reset")

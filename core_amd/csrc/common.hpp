@@ -67,7 +67,7 @@ struct DctPlan {
   int bhat_off;   // M entries FFT_M(conj chirp, wrapped)/M (Bluestein)
   int mk_off;     // N entries exp(-i pi k / (2N))          (Makhoul post-twiddle)
   float invN;     // 1/N
-  int pad;
+  int ept;        // elements per thread the plan asks for (16 or 24)
   FftPlan fft;
 };
 

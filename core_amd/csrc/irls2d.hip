@@ -89,6 +89,15 @@ struct HostDct {
 static std::mutex g_plan_mu;
 static std::map<int, std::shared_ptr<HostDct>> g_plans;
 
+// Plan choices set by an autotuner (irls2d_plan_override): FFT length (N, or the Bluestein length),
+// Stockham radix order and elements per thread, per DCT length.
+struct PlanOverride {
+  int fft_len;
+  std::vector<int> rad;
+  int ept;
+};
+static std::map<int, PlanOverride> g_over;
+
 // Bluestein length: the 5-smooth M >= 2N - 1 (within 30%) with the lowest M * (passes + 1).
 static int64_t bluestein_len(int64_t N) {
   const int64_t lo = 2 * N - 1;
@@ -134,6 +143,8 @@ int supported(int64_t N) {
   return bluestein_len(N) <= MAX_FFT_LEN;
 }
 
+static int default_ept(int M) { return M <= 16 * MAX_FFT_THREADS ? 16 : 24; }
+
 static std::shared_ptr<HostDct> get_plan(int N) {
   std::lock_guard<std::mutex> lk(g_plan_mu);
   auto it = g_plans.find(N);
@@ -142,9 +153,16 @@ static std::shared_ptr<HostDct> get_plan(int N) {
   DctPlan &P = h->plan;
   std::memset(&P, 0, sizeof(P));
   P.N = N;
-  P.blue = smooth235(N) ? 0 : 1;
-  P.M = P.blue ? (int)bluestein_len(N) : N;
+  auto ov = g_over.find(N);
+  if (ov != g_over.end()) {
+    P.M = ov->second.fft_len;
+    P.blue = P.M != N ? 1 : 0;
+  } else {
+    P.blue = smooth235(N) ? 0 : 1;
+    P.M = P.blue ? (int)bluestein_len(N) : N;
+  }
   P.invN = (float)(1.0 / (double)N);
+  P.ept = (ov != g_over.end() && ov->second.ept) ? ov->second.ept : default_ept(P.M);
   auto push = [&](const std::vector<cd> &v) {
     int off = (int)h->tab.size();
     for (const cd &z : v) h->tab.push_back(make_float2((float)z.real(), (float)z.imag()));
@@ -163,7 +181,7 @@ static std::shared_ptr<HostDct> get_plan(int N) {
   }
   P.fft.L = M;
   P.fft.tw_off = push(tw);
-  auto rad = radices(M);
+  const std::vector<int> rad = (ov != g_over.end() && !ov->second.rad.empty()) ? ov->second.rad : radices(M);
   P.fft.npass = (int)rad.size();
   int Ns = 1;
   for (size_t s = 0; s < rad.size(); ++s) {
@@ -310,9 +328,10 @@ static int64_t arena_layout(int n, const int64_t *H, const int64_t *W, int64_t *
   return tot;
 }
 
-// Elements per thread (16 or 32) and threads per workgroup for transforms of length <= L.
-static void fft_shape(int L, int &E, int &T) {
-  E = (L <= 16 * MAX_FFT_THREADS) ? 16 : 24;
+// Elements per thread (16 or 24: the largest any plan of the launch asks for, and enough for L
+// within MAX_FFT_THREADS threads) and threads per workgroup for transforms of length <= L.
+static void fft_shape(int L, int ept, int &E, int &T) {
+  E = std::max(ept, default_ept(L));
   int t = (L + E - 1) / E;
   t = (t + 63) / 64 * 64;
   T = std::max(64, std::min(MAX_FFT_THREADS, t));
@@ -384,7 +403,7 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     return o;
   };
   for (int g = 0; g < NGEO; ++g) Lo.geo[g].assign(1, 0);
-  int maxM_row = 1, maxM_col = 1;
+  int maxM_row = 1, maxM_col = 1, ept_row = 16, ept_col = 16;
   for (int i = 0; i < n; ++i) {
     const int h = (int)H[i], w = (int)W[i];
     ImgDesc d;
@@ -418,8 +437,14 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     d.offS = Lo.totalS;
     Lo.totalS += (sz + 31) / 32 * 32;
     Lo.desc.push_back(d);
-    if (!lw) maxM_row = std::max(maxM_row, Lo.plans[d.rowplan].M);
-    if (!lh) maxM_col = std::max(maxM_col, Lo.plans[d.colplan].M);
+    if (!lw) {
+      maxM_row = std::max(maxM_row, Lo.plans[d.rowplan].M);
+      ept_row = std::max(ept_row, Lo.plans[d.rowplan].ept);
+    }
+    if (!lh) {
+      maxM_col = std::max(maxM_col, Lo.plans[d.colplan].M);
+      ept_col = std::max(ept_col, Lo.plans[d.colplan].ept);
+    }
     const int64_t tiles = ((int64_t)(h + TILE_H - 1) / TILE_H) * ((w + TILE_W - 1) / TILE_W);
     const int64_t ttiles = ((int64_t)(h + TT - 1) / TT) * ((w + TT - 1) / TT);
     const int64_t geo_add[NGEO] = {tiles, lw ? 0 : (h + 1) / 2, lh ? 0 : (w + 1) / 2, ttiles, lr1, lr2, lc1, lc2};
@@ -431,8 +456,8 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
   }
   for (int g = 0; g < 4; ++g) Lo.max_blocks = std::max(Lo.max_blocks, Lo.geo[g].back());
   for (int g = 4; g < NGEO; ++g) Lo.max_blocks2 = std::max(Lo.max_blocks2, Lo.geo[g].back());
-  fft_shape(maxM_row, Lo.row_E, Lo.row_threads);
-  fft_shape(maxM_col, Lo.col_E, Lo.col_threads);
+  fft_shape(maxM_row, ept_row, Lo.row_E, Lo.row_threads);
+  fft_shape(maxM_col, ept_col, Lo.col_E, Lo.col_threads);
   Lo.row_lds = (size_t)(TW_LDS + lds_seq_len(maxM_row)) * sizeof(float2);
   Lo.col_lds = (size_t)(TW_LDS + lds_seq_len(maxM_col)) * sizeof(float2);
   // workspace map
@@ -953,6 +978,76 @@ int precondition(int n, const int64_t *H, const int64_t *W, const float *r, floa
   return IRLS2D_OK;
 }
 
+// Drops every cached layout, uploaded table set and graph (plans changed).
+static void drop_caches() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (auto &g : g_graphs) (void)hipGraphExecDestroy(g.second);
+  g_graphs.clear();
+  g_layouts.clear();
+  g_uploaded.clear();
+}
+
+static bool valid_radix(int r) { return r == 2 || r == 3 || r == 4 || r == 5 || r == 8 || r == 16; }
+
+int plan_override(int64_t N, int64_t fft_len, const int *rad, int nrad, int ept) {
+  g_err.clear();
+  if (N < 1 || N > MAX_FFT_LEN) return fail(IRLS2D_EINVAL, "plan overrides apply to lengths 1 .. 12288");
+  if (fft_len == 0) {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    g_over.erase((int)N);
+    g_plans.erase((int)N);
+  } else {
+    if (fft_len < N || fft_len > MAX_FFT_LEN || !smooth235(fft_len))
+      return fail(IRLS2D_EINVAL, "fft_len must be 5-smooth, >= N and <= 12288");
+    if (fft_len != N && fft_len < 2 * N - 1) return fail(IRLS2D_EINVAL, "a Bluestein length must be >= 2N - 1");
+    if (fft_len == N && !smooth235(N)) return fail(IRLS2D_EINVAL, "N has a prime factor > 5: needs Bluestein");
+    if (ept != 0 && ept != 16 && ept != 24) return fail(IRLS2D_EINVAL, "ept must be 0, 16 or 24");
+    if (ept == 16 && fft_len > 16 * MAX_FFT_THREADS) return fail(IRLS2D_EINVAL, "ept 16 covers lengths <= 8192");
+    PlanOverride o;
+    o.fft_len = (int)fft_len;
+    o.ept = ept;
+    if (nrad > 0) {
+      if (!rad || nrad > MAX_PASS) return fail(IRLS2D_EINVAL, "bad radix list");
+      int64_t prod = 1;
+      for (int i = 0; i < nrad; ++i) {
+        if (!valid_radix(rad[i])) return fail(IRLS2D_EINVAL, "radices must be 2, 3, 4, 5, 8 or 16");
+        prod *= rad[i];
+      }
+      if (prod != fft_len) return fail(IRLS2D_EINVAL, "the radices must multiply to fft_len");
+      o.rad.assign(rad, rad + nrad);
+    } else if (radices((int)fft_len).size() > MAX_PASS) {
+      return fail(IRLS2D_EINVAL, "too many passes");
+    }
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    g_over[(int)N] = o;
+    g_plans.erase((int)N);
+  }
+  drop_caches();
+  return IRLS2D_OK;
+}
+
+int plan_describe(int64_t N, int64_t *fft_len, int *rad, int *nrad, int *ept) {
+  g_err.clear();
+  if (N < 1 || N > MAX_FFT_LEN || !supported(N)) return fail(IRLS2D_EINVAL, "no LDS-resident plan for this length");
+  auto h = get_plan((int)N);
+  if (fft_len) *fft_len = h->plan.M;
+  if (ept) *ept = h->plan.ept;
+  if (nrad) *nrad = h->plan.fft.npass;
+  if (rad)
+    for (int i = 0; i < h->plan.fft.npass; ++i) rad[i] = h->plan.fft.pass[i].R;
+  return IRLS2D_OK;
+}
+
+int plan_reset() {
+  {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    g_over.clear();
+    g_plans.clear();
+  }
+  drop_caches();
+  return IRLS2D_OK;
+}
+
 int release(void *workspace) {
   std::lock_guard<std::mutex> lk(g_mu);
   for (auto it = g_graphs.begin(); it != g_graphs.end();) {
@@ -1003,6 +1098,16 @@ int irls2d_solve(int n, const int64_t *H, const int64_t *W, const float *phase, 
 }
 
 int irls2d_release(void *workspace) { return irls2d::release(workspace); }
+
+int irls2d_plan_override(int64_t n, int64_t fft_len, const int *radices, int nradices, int ept) {
+  return irls2d::plan_override(n, fft_len, radices, nradices, ept);
+}
+
+int irls2d_plan_describe(int64_t n, int64_t *fft_len, int *radices, int *nradices, int *ept) {
+  return irls2d::plan_describe(n, fft_len, radices, nradices, ept);
+}
+
+int irls2d_plan_reset(void) { return irls2d::plan_reset(); }
 
 int irls2d_precondition(int n, const int64_t *H, const int64_t *W, const float *r, float *z, void *workspace,
                         size_t workspace_bytes, void *stream) {

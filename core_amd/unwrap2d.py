@@ -71,6 +71,9 @@ class Workspace:
     def get(cls, shapes, device):
         key = (device.index, tuple((int(h), int(w)) for h, w in shapes))
         ws = cls._cache.get(key)
+        if ws is not None and _lib.workspace_size(ws.shapes) > ws.nbytes:  # plans changed (autotune)
+            cls._cache.pop(key).release()
+            ws = None
         if ws is None:
             if len(cls._cache) >= 4:
                 for k in list(cls._cache)[:len(cls._cache) - 3]:

@@ -122,6 +122,21 @@ int irls2d_precondition(int n, const int64_t *H, const int64_t *W, const float *
                         void *workspace, size_t workspace_bytes, void *stream);
 
 /*
+ * Autotuning hooks (no reference counterpart: the reference calls torch-dct's fixed plans).
+ * irls2d_plan_override sets the transform plan of DCT length n (1 .. 12288): fft_len = n for a
+ * direct FFT (n 5-smooth) or a 5-smooth Bluestein length >= 2n - 1, `radices` the Stockham pass
+ * order (nradices entries from {2, 3, 4, 5, 8, 16} multiplying to fft_len; 0 entries = the
+ * default order), ept the elements per thread (16, 24, or 0 = default; 16 needs fft_len <= 8192).
+ * fft_len = 0 removes the override.  Changing plans drops every cached layout, uploaded table set
+ * and hipGraph (calls must not be in flight).  irls2d_plan_describe reports the plan in use
+ * (radices: room for 16 entries); irls2d_plan_reset removes all overrides.
+ * core_amd.autotune times candidate plans with irls2d_precondition and applies the fastest.
+ */
+int irls2d_plan_override(int64_t n, int64_t fft_len, const int *radices, int nradices, int ept);
+int irls2d_plan_describe(int64_t n, int64_t *fft_len, int *radices, int *nradices, int *ept);
+int irls2d_plan_reset(void);
+
+/*
  * Release cached hipGraphs and plans that refer to `workspace` (NULL = all).  Call before
  * freeing a workspace that was used with IRLS2D_FLAG_GRAPH.
  */

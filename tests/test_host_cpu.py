@@ -154,3 +154,43 @@ def test_synthetic_generator_is_deterministic_and_banded():
     assert np.nanmax(np.abs(psi)) <= np.pi + 1e-6
     assert np.isnan(psi).any() and (coh[np.isnan(psi)] == 0).all()
     assert S.stack_shapes()[:2] == [(1419, 5954), (1131, 4636)]
+
+
+def test_plan_override_validation_and_roundtrip(lib):
+    try:
+        fl, rad, ept = lib.plan_describe(8192)
+        assert fl == 8192 and np.prod(rad) == 8192 and ept == 16
+        lib.plan_override(8192, 8192, [8, 8, 8, 8, 2], 24)
+        assert lib.plan_describe(8192) == (8192, [8, 8, 8, 8, 2], 24)
+        b0 = lib.workspace_size([(16, 4999)])
+        lib.plan_override(4999, 12288, [], 0)               # a longer Bluestein length
+        assert lib.plan_describe(4999)[0] == 12288
+        assert lib.workspace_size([(16, 4999)]) > b0        # bigger chirp-kernel table
+        for bad in [(8192, 8192, [16, 16, 16], 0),           # product != fft_len
+                    (8192, 8192, [16, 16, 16, 2], 32),       # ept
+                    (8192, 8192, [16, 16, 7, 2], 0),         # radix
+                    (4999, 9000, [], 0),                     # Bluestein length < 2N - 1 ... and not 5-smooth
+                    (4999, 4999, [], 0),                     # prime length without Bluestein
+                    (12289, 30000, [], 0),                   # beyond the LDS-resident lengths
+                    (8192, 12288, [], 16)]:                  # ept 16 with a length > 8192
+            with pytest.raises(ValueError):
+                lib.plan_override(*bad)
+        lib.plan_override(8192, 0)                          # remove one override
+        assert lib.plan_describe(8192) == (8192, [16, 16, 16, 2], 16)
+    finally:
+        lib.plan_reset()
+    assert lib.plan_describe(4999)[0] == 10240   # default: lowest M * (passes + 1)
+
+
+def test_autotune_candidates_are_valid(lib):
+    from core_amd import autotune as A
+    try:
+        for n in [40, 127, 1331, 1500, 2048, 4999, 5000, 6000, 8192, 12288]:
+            cands = A.candidates(n)
+            assert cands, n
+            for fl, rad, ept in cands:
+                assert int(np.prod(rad)) == fl and (fl == n or fl >= 2 * n - 1) and fl <= 12288
+                lib.plan_override(n, fl, rad, ept)   # the library accepts every candidate
+                assert lib.plan_describe(n) == (fl, rad, ept)
+    finally:
+        lib.plan_reset()
