@@ -60,3 +60,30 @@ def test_autograd_batch_shapes_and_loss(hip):
     assert torch.isfinite(ph.grad).all() and torch.isfinite(wt.grad).all()
     # centred output: masked mean zero per image
     assert float(u.detach().mean(dim=(1, 2)).abs().max()) < 1e-3
+
+
+@pytest.mark.parametrize("noise,mask", [(1.2, False), (1.0, True)])
+def test_gpu_adjoint_matches_unrolled_autograd(hip, noise, mask):
+    """The GPU implicit adjoint against autograd through the unrolled float64 torch restatement of
+    the reference solver (oracle/irls_unrolled.py), both run long enough to converge on 10 x 12."""
+    import sys, os
+    import torch
+    from core_amd.autograd import irls_unwrap_2d_diff
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_adjoint_unrolled import EPS, case, unrolled_grads
+    psi, coh, g = case(noise, mask)
+    print(f"unrolled-autograd case noise={noise} mask={mask}", flush=True)
+    v = ~np.isnan(psi)
+    _, ga_phi, ga_w = unrolled_grads(psi, coh, g)
+    ph = torch.from_numpy(psi)[None].to(hip).requires_grad_(True)
+    wt = torch.from_numpy(coh)[None].to(hip).requires_grad_(True)
+    u = irls_unwrap_2d_diff(ph, wt, max_iter=250, cg_max_iter=200, epsilon=EPS, adj_cg_iter=3000)
+    (u * torch.from_numpy(g.astype(np.float32))[None].to(hip)).sum().backward()
+    gphi = ph.grad[0].cpu().numpy().astype(np.float64)
+    gw = wt.grad[0].cpu().numpy().astype(np.float64)
+    scale = max(np.abs(ga_phi[v]).max(), np.abs(ga_w[v]).max())
+    ephi = np.abs(gphi[v] - ga_phi[v]).max() / np.abs(ga_phi[v]).max()
+    ew = np.abs(gw[v] - ga_w[v]).max() / scale
+    print(f"  dL/dphase rel err {ephi:.2e}; dL/dweight err {ew:.2e} (of the largest gradient entry)", flush=True)
+    assert ephi < 1e-4
+    assert ew < 2e-3
