@@ -1,0 +1,212 @@
+"""Stack-level unwrapping on the MI355X: the reference's `Stack_unwrap2d` entry points for whole
+SBAS stacks (insardev/insardev/Stack_unwrap2d.py).
+
+The reference runs one dask task per pair, each taking a process-wide GPU lock and calling
+`utils_unwrap2d.irls_unwrap_2d` (:17-29, :32-71, :753-990).  Here every pair of a burst variable
+goes through ONE batched device solve (`irls_unwrap_2d_batch`), with the same per-pair results.
+
+* `unwrap_stack(phase[pair, y, x] or [y, x], weight)` -> (unwrapped float32, conncomp uint16):
+  the array core.
+* `unwrap2d_irls`, `unwrap2d`, `unwrap2d_dataset`: the reference's methods over burst mappings
+  of xarray Datasets (keys -> Dataset), with the same arguments, defaults, errors and output
+  structure (dims ('pair', 'y', 'x') or ('y', 'x'), y/x coords, the pair dimension's
+  non-dimension coords such as ref/rep/BPR, `units = 'radians'`, dataset attrs, CRS when
+  rioxarray is present, the input's spatial chunks when it was dask-backed).  xarray is an
+  optional dependency: the functions import it on use (or take the module as `xr`).  Unlike the
+  reference they compute eagerly: each burst variable is one device call over all its pairs.
+* `filter_small_components`: the first step of the reference's component linking
+  (Stack_unwrap2d.py:343-375), which NaNs out labelled components below max(conncomp_size, 4).
+  The integer-program linking of the remaining components (:377-397, utils_unwrap2d.py:195-444)
+  is not part of this build: `unwrap2d(conncomp=False)` warns and returns them unlinked when two
+  or more remain.
+"""
+from __future__ import annotations
+
+import importlib
+import warnings
+
+import numpy as np
+
+from .unwrap2d import irls_unwrap_2d_batch
+
+__all__ = ["unwrap_stack", "filter_small_components", "unwrap2d_irls", "unwrap2d", "unwrap2d_dataset"]
+
+
+def unwrap_stack(phase, weight=None, device='auto', max_iter=50, tol=1e-2, cg_max_iter=10, cg_tol=1e-3,
+                 epsilon=1e-2, conncomp_size=30, debug=False, graph=False):
+    """Unwrap a (pair, y, x) or (y, x) stack in one device call.
+
+    Defaults are unwrap2d_irls's (Stack_unwrap2d.py:753-755).  Per pair the result equals
+    irls_unwrap_2d (an all-NaN pair gives NaN and zeros, as the reference task does,
+    Stack_unwrap2d.py:54-59).  Returns (unwrapped float32, conncomp uint16) of the input's shape.
+    """
+    ph = np.asarray(phase)
+    if ph.ndim not in (2, 3):
+        raise ValueError(f"Expected 2D or 3D phase, got shape {ph.shape}")
+    squeeze = ph.ndim == 2
+    ph3 = ph[None] if squeeze else ph
+    w3 = None
+    if weight is not None:
+        w3 = np.asarray(weight)
+        w3 = w3[None] if w3.ndim == 2 else w3
+        if w3.shape != ph3.shape:
+            raise ValueError(f"weight shape {w3.shape} does not match phase shape {ph3.shape}")
+    res = irls_unwrap_2d_batch([p for p in ph3], None if w3 is None else [w for w in w3], device=device,
+                               max_iter=max_iter, tol=tol, cg_max_iter=cg_max_iter, cg_tol=cg_tol, epsilon=epsilon,
+                               conncomp_size=conncomp_size, debug=debug, graph=graph)
+    unw = np.stack([r[0].astype(np.float32) for r in res])
+    cc = np.stack([r[1].astype(np.uint16) for r in res])
+    return (unw[0], cc[0]) if squeeze else (unw, cc)
+
+
+def filter_small_components(unwrapped, conncomp, conncomp_size=1000):
+    """NaN out the labelled components smaller than max(conncomp_size, 4) pixels.
+
+    Restates Stack_unwrap2d.py:343-375 for one 2-D grid: with the size-ordered IRLS labels, the
+    components with label >= 1 and at least min_size pixels are kept, label-0 pixels (invalid, or
+    components already below the solver's conncomp_size) are left as they are, every other
+    labelled pixel becomes NaN.  Returns (filtered copy, number of kept components); a grid
+    without valid pixels is returned unchanged.
+    """
+    u = np.asarray(unwrapped)
+    if not np.any(~np.isnan(u)):
+        return u, 0
+    labeled = np.asarray(conncomp).astype(np.int32)
+    min_size = max(conncomp_size, 4)
+    n_total = int(labeled.max())
+    sizes = np.bincount(labeled.ravel(), minlength=n_total + 1)
+    kept = np.array([i for i in range(1, n_total + 1) if sizes[i] >= min_size], dtype=np.int32)
+    is_linked = np.isin(labeled, kept) | (labeled == 0)
+    out = u.copy()
+    out[~is_linked] = np.nan
+    return out, int(kept.size)
+
+
+def _xr(xr):
+    return xr if xr is not None else importlib.import_module("xarray")
+
+
+def _pair_coords(da):
+    """Non-dimension coords along 'pair' (ref, rep, BPR, ...), Stack_unwrap2d.py:873-885."""
+    out = {}
+    if 'pair' in da.dims:
+        for k, v in da.coords.items():
+            if k != 'pair' and getattr(v, 'dims', None) == ('pair',):
+                out[k] = ('pair', np.asarray(v.values if hasattr(v, 'values') else v))
+    return out
+
+
+def _spatial_chunks(da):
+    chunks = getattr(getattr(da, 'data', None), 'chunks', None)
+    if isinstance(chunks, tuple) and len(chunks) >= 2:
+        return chunks[-2], chunks[-1]
+    return None
+
+
+def unwrap2d_irls(phase, weight=None, device='auto', max_iter=50, tol=1e-2, cg_max_iter=10, cg_tol=1e-3,
+                  epsilon=1e-2, conncomp_size=30, semaphore=8, debug=False, xr=None):
+    """Stack_unwrap2d.unwrap2d_irls (:753-990) over a burst mapping {key: xr.Dataset}.
+
+    Returns a tuple (unwrapped {key: Dataset}, conncomp {key: Dataset}) like the reference's
+    Batches((Batch, BatchUnit)).  `semaphore` only limits CPU tasks in the reference and has no
+    effect here (no CPU path).
+    """
+    xr = _xr(xr)
+    unwrap_result, conncomp_result = {}, {}
+    for key in phase.keys():
+        ds = phase[key]
+        wds = weight[key] if weight is not None and key in weight else None
+        data_vars = [v for v in ds.data_vars if 'y' in ds[v].dims and 'x' in ds[v].dims]
+        if debug:
+            print(f'unwrap2d_irls: burst {key!r}, variables {data_vars}')
+        uvars, cvars = {}, {}
+        for var in data_vars:
+            pda = ds[var]
+            wda = wds[var] if wds is not None else None
+            chunks = _spatial_chunks(pda)
+            arr = np.asarray(pda.values, dtype=np.float32)
+            warr = None if wda is None else np.asarray(wda.values, dtype=np.float32)
+            unw, cc = unwrap_stack(arr, warr, device=device, max_iter=max_iter, tol=tol, cg_max_iter=cg_max_iter,
+                                   cg_tol=cg_tol, epsilon=epsilon, conncomp_size=conncomp_size, debug=debug)
+            dims = ('pair', 'y', 'x') if 'pair' in pda.dims else ('y', 'x')
+            coords = {'y': pda.y, 'x': pda.x}
+            uda = xr.DataArray(unw, dims=dims, coords=coords)
+            uda.attrs['units'] = 'radians'
+            cda = xr.DataArray(cc, dims=dims, coords=coords)
+            pc = _pair_coords(pda)
+            if pc:
+                uda = uda.assign_coords(**pc)
+                cda = cda.assign_coords(**pc)
+            if chunks is not None and (len(chunks[0]) > 1 or len(chunks[1]) > 1):
+                uda = uda.chunk({'y': chunks[0], 'x': chunks[1]})
+                cda = cda.chunk({'y': chunks[0], 'x': chunks[1]})
+            uvars[var] = uda
+            cvars[var] = cda
+        unwrap_result[key] = xr.Dataset(uvars, attrs=dict(ds.attrs))
+        conncomp_result[key] = xr.Dataset(cvars, attrs=dict(ds.attrs))
+        rio = getattr(ds, 'rio', None)
+        crs = getattr(rio, 'crs', None) if rio is not None else None
+        if crs is not None:
+            unwrap_result[key].rio.write_crs(crs, inplace=True)
+            conncomp_result[key].rio.write_crs(crs, inplace=True)
+    return unwrap_result, conncomp_result
+
+
+def unwrap2d(phase, weight=None, conncomp=False, conncomp_size=1_000, conncomp_gap=None, conncomp_linksize=5,
+             conncomp_linkcount=30, device='auto', debug=False, xr=None, **kwargs):
+    """Stack_unwrap2d.unwrap2d (:645-759) over a burst mapping {key: xr.Dataset}.
+
+    conncomp=True returns (unwrapped, conncomp).  conncomp=False applies the small-component
+    filter of the reference's linking step and returns the unwrapped mapping; linking two or more
+    remaining components (the reference's integer program) is not part of this build, so they are
+    returned unlinked with a RuntimeWarning.
+    """
+    if not conncomp and conncomp_linksize > conncomp_size:
+        raise ValueError(
+            f'conncomp_linksize ({conncomp_linksize}) cannot be greater than conncomp_size ({conncomp_size}). '
+            f'Components must have at least conncomp_linksize pixels for reliable offset estimation.')
+    xr = _xr(xr)
+    unwrapped, labels = unwrap2d_irls(phase, weight, device=device, conncomp_size=conncomp_size, debug=debug,
+                                      xr=xr, **kwargs)
+    if conncomp:
+        return unwrapped, labels
+    result = {}
+    unlinked = 0
+    for key, ds in unwrapped.items():
+        lds = labels[key]
+        vars_ = {}
+        for var in ds.data_vars:
+            uda = ds[var]
+            u = np.asarray(uda.values)
+            cc = np.asarray(lds[var].values)
+            u3, c3 = (u[None], cc[None]) if u.ndim == 2 else (u, cc)
+            out = np.empty_like(u3)
+            for p in range(u3.shape[0]):
+                out[p], kept = filter_small_components(u3[p], c3[p], conncomp_size)
+                unlinked += kept >= 2
+            out = out[0] if u.ndim == 2 else out
+            nda = xr.DataArray(out, dims=uda.dims, coords=uda.coords)
+            nda.attrs.update(dict(uda.attrs))
+            vars_[var] = nda
+        result[key] = xr.Dataset(vars_, attrs=dict(ds.attrs))
+    if unlinked:
+        warnings.warn(f"unwrap2d: {unlinked} grid(s) keep two or more components; linking them by 2 pi offsets "
+                      "(the reference's integer program) is not part of this build, they are returned unlinked "
+                      "(use conncomp=True to get their labels)", RuntimeWarning, stacklevel=2)
+    return result
+
+
+def unwrap2d_dataset(phase, weight=None, conncomp=False, conncomp_size=1000, conncomp_gap=None,
+                     conncomp_linksize=5, conncomp_linkcount=30, device='auto', debug=False, xr=None, **kwargs):
+    """Stack_unwrap2d.unwrap2d_dataset (:525-643): one merged xr.Dataset in, Dataset(s) out."""
+    xr = _xr(xr)
+    if not isinstance(phase, xr.Dataset):
+        raise TypeError(f"phase must be xr.Dataset, got {type(phase).__name__}")
+    if weight is not None and not isinstance(weight, xr.Dataset):
+        raise TypeError(f"weight must be xr.Dataset, got {type(weight).__name__}")
+    res = unwrap2d({'': phase}, None if weight is None else {'': weight}, conncomp=conncomp,
+                   conncomp_size=conncomp_size, conncomp_gap=conncomp_gap, conncomp_linksize=conncomp_linksize,
+                   conncomp_linkcount=conncomp_linkcount, device=device, debug=debug, xr=xr, **kwargs)
+    if conncomp:
+        return res[0][''], res[1]['']
+    return res['']
