@@ -31,6 +31,7 @@ EXPORTS = (
     "irls2d_plan_override",
     "irls2d_plan_describe",
     "irls2d_plan_reset",
+    "irls2d_set_column_mode",
     "irls2d_release",
     "irls2d_kernel_stats",
     "irls2d_kernel_stats_reset",
@@ -85,6 +86,8 @@ def load():
     L.irls2d_plan_describe.restype = ctypes.c_int
     L.irls2d_plan_reset.argtypes = []
     L.irls2d_plan_reset.restype = ctypes.c_int
+    L.irls2d_set_column_mode.argtypes = [ctypes.c_int]
+    L.irls2d_set_column_mode.restype = ctypes.c_int
     L.irls2d_release.argtypes = [ctypes.c_void_p]
     L.irls2d_release.restype = ctypes.c_int
     L.irls2d_kernel_stats.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), i64p]
@@ -174,3 +177,11 @@ def plan_override(n: int, fft_len: int, radices=(), ept: int = 0):
 
 def plan_reset():
     check(load().irls2d_plan_reset(), "irls2d_plan_reset")
+
+
+COLUMN_AUTO, COLUMN_TRANSPOSE, COLUMN_DIRECT = 0, 1, 2
+
+
+def set_column_mode(mode: int):
+    """0 = automatic, 1 = transpose + column kernel, 2 = column kernel on the row-major field."""
+    check(load().irls2d_set_column_mode(int(mode)), "irls2d_set_column_mode")

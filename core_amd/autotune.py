@@ -220,6 +220,13 @@ def tune(shapes, device="cuda", reps=5, path=DEFAULT_FILE, save=True, verbose=Fa
         for n, p in chosen.items():
             p["ept"] = variants[best_v][n]
             _lib.plan_override(n, p["fft_len"], p["radices"], p["ept"])
+        # column strategy (transpose + column kernel vs column kernel on the row-major field):
+        # measured for the record; the library chooses it per batch by its Bluestein share
+        column_ms = {}
+        for name, mode in (("transpose", _lib.COLUMN_TRANSPOSE), ("direct", _lib.COLUMN_DIRECT),
+                           ("auto", _lib.COLUMN_AUTO)):
+            _lib.set_column_mode(mode)
+            column_ms[name] = round(batch.time(reps), 4)
         after = batch.time(reps)
         if after > 0.99 * base:  # no gain on the real batch: keep the built-in plans
             for n in chosen:
@@ -227,7 +234,7 @@ def tune(shapes, device="cuda", reps=5, path=DEFAULT_FILE, save=True, verbose=Fa
             chosen = {}
             after = batch.time(reps)
         batch.close()
-    result = {"plans": chosen, "before_ms": round(base, 4), "after_ms": round(after, 4)}
+    result = {"plans": chosen, "before_ms": round(base, 4), "after_ms": round(after, 4), "column_ms": column_ms}
     if save:
         _merge(path, _device_key(dev), chosen)
     return result
@@ -282,4 +289,5 @@ if __name__ == "__main__":  # python -m core_amd.autotune [--out FILE]: tune the
         t0 = time.time()
         res = tune(shp, reps=args.reps, verbose=args.verbose, path=args.out)
         print(json.dumps({"set": name, "before_ms": res["before_ms"], "after_ms": res["after_ms"],
-                          "lengths": len(res["plans"]), "s": round(time.time() - t0, 1)}), flush=True)
+                          "lengths": len(res["plans"]), "column_ms": res["column_ms"],
+                          "s": round(time.time() - t0, 1)}), flush=True)

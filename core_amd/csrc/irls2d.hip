@@ -97,6 +97,9 @@ struct PlanOverride {
   int ept;
 };
 static std::map<int, PlanOverride> g_over;
+// Column pass strategy: 0 = automatic, 1 = transpose + column kernel, 2 = column kernel on the
+// row-major field (irls2d_set_column_mode).
+static int g_col_mode = 0;
 
 // Bluestein length: the 5-smooth M >= 2N - 1 (within 30%) with the lowest M * (passes + 1).
 static int64_t bluestein_len(int64_t N) {
@@ -299,6 +302,7 @@ struct Layout {
   int64_t totalS = 0;  // float2 elements of the long-transform scratch arena
   int row_threads = 64, col_threads = 64, row_E = 16, col_E = 16;
   size_t row_lds = 0, col_lds = 0, long_lds = 0;
+  bool col_direct = false;  // column pass straight on the row-major field (no transposes)
   // byte offsets inside the workspace
   size_t o_fields = 0, o_Tt = 0, o_S = 0, o_valid = 0, o_meta = 0, o_desc = 0, o_state = 0, o_geo[NGEO] = {},
          o_plans = 0, o_lplans = 0, o_tab = 0, o_dtab = 0, o_partials = 0, o_partials2 = 0, bytes = 0,
@@ -456,6 +460,20 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
   }
   for (int g = 0; g < 4; ++g) Lo.max_blocks = std::max(Lo.max_blocks, Lo.geo[g].back());
   for (int g = 4; g < NGEO; ++g) Lo.max_blocks2 = std::max(Lo.max_blocks2, Lo.geo[g].back());
+  {
+    // The transposes cost 8 B/px each; reading column pairs straight from the row-major field
+    // (float2 per row, L2-shared between neighbouring blocks) costs more than that when the column
+    // FFT is short and cheap, but hides under it when the column transforms are Bluestein
+    // (measured on MI355X: -16% per PCG iteration for the 48-burst stack, +16% at 8192^2).
+    double blue_px = 0, all_px = 0;
+    for (int i = 0; i < n; ++i) {
+      const double px = (double)H[i] * (double)W[i];
+      all_px += px;
+      if (Lo.desc[i].colplan >= 0 && Lo.plans[Lo.desc[i].colplan].blue) blue_px += px;
+    }
+    const bool auto_direct = blue_px >= 0.5 * all_px;
+    Lo.col_direct = Lo.geo[6].back() == 0 && (g_col_mode == 2 || (g_col_mode == 0 && auto_direct));
+  }
   fft_shape(maxM_row, ept_row, Lo.row_E, Lo.row_threads);
   fft_shape(maxM_col, ept_col, Lo.col_E, Lo.col_threads);
   Lo.row_lds = (size_t)(TW_LDS + lds_seq_len(maxM_row)) * sizeof(float2);
@@ -674,11 +692,17 @@ static void col_pass(const Layout &Lo, const Ctx &c, int guard, hipStream_t s) {
   }
   if (Lo.geo[2].back() == 0) return;
   const dim3 g(Lo.geo[2].back()), b(Lo.col_threads);
-  if (Lo.col_E == 16) LAUNCH("k_col", (k_col<16>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
-  else LAUNCH("k_col", (k_col<24>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
+  if (Lo.col_direct) {
+    if (Lo.col_E == 16) LAUNCH("k_col_direct", (k_col<16, true>), g, b, Lo.col_lds, s, c, c.f.T, guard);
+    else LAUNCH("k_col_direct", (k_col<24, true>), g, b, Lo.col_lds, s, c, c.f.T, guard);
+  } else {
+    if (Lo.col_E == 16) LAUNCH("k_col", (k_col<16, false>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
+    else LAUNCH("k_col", (k_col<24, false>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
+  }
 }
 
 static void transpose(const Layout &Lo, const Ctx &c, int dir, int guard, hipStream_t s) {
+  if (Lo.col_direct) return;
   const float *src = dir == 0 ? c.f.T : c.f.Tt;
   float *dst = dir == 0 ? c.f.Tt : c.f.T;
   LAUNCH("k_transpose", k_transpose, dim3(Lo.geo[3].back()), dim3(256), 0, s, c, src, dst, dir, guard);
@@ -799,7 +823,8 @@ static int set_attrs() {
   const int lds = (TW_LDS + lds_seq_len(MAX_FFT_LEN)) * (int)sizeof(float2);  // largest transform
   const void *fns[] = {(const void *)k_row_dct2<false, 16>, (const void *)k_row_dct2<false, 24>,
                        (const void *)k_row_dct2<true, 16>,  (const void *)k_row_dct2<true, 24>,
-                       (const void *)k_col<16>,             (const void *)k_col<24>,
+                       (const void *)k_col<16, false>,      (const void *)k_col<24, false>,
+                       (const void *)k_col<16, true>,       (const void *)k_col<24, true>,
                        (const void *)k_row_dct3<16>,        (const void *)k_row_dct3<24>};
   for (const void *f : fns) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   const int llds = LONG_LDS * (int)sizeof(float2);
@@ -1038,6 +1063,17 @@ int plan_describe(int64_t N, int64_t *fft_len, int *rad, int *nrad, int *ept) {
   return IRLS2D_OK;
 }
 
+int set_column_mode(int mode) {
+  g_err.clear();
+  if (mode < 0 || mode > 2) return fail(IRLS2D_EINVAL, "column mode must be 0 (auto), 1 (transpose) or 2 (direct)");
+  {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    g_col_mode = mode;
+  }
+  drop_caches();
+  return IRLS2D_OK;
+}
+
 int plan_reset() {
   {
     std::lock_guard<std::mutex> lk(g_plan_mu);
@@ -1108,6 +1144,8 @@ int irls2d_plan_describe(int64_t n, int64_t *fft_len, int *radices, int *nradice
 }
 
 int irls2d_plan_reset(void) { return irls2d::plan_reset(); }
+
+int irls2d_set_column_mode(int mode) { return irls2d::set_column_mode(mode); }
 
 int irls2d_precondition(int n, const int64_t *H, const int64_t *W, const float *r, float *z, void *workspace,
                         size_t workspace_bytes, void *stream) {

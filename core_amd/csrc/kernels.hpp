@@ -519,18 +519,43 @@ __global__ __launch_bounds__(256) void k_transpose(Ctx c, const float *__restric
 
 // Column pass on the transposed field: DCT-II along H, divide by -lambda (float32 lambda from
 // float64 cosines, utils_unwrap2d.py:1284-1289, :1395-1398), zero the DC term, inverse DCT.
-template <int E>
+// DIRECT: the column pair is read from and written back to the row-major field (one float2 per
+// row) instead of the transposed copy; blocks are remapped so that each XCD takes a contiguous
+// range of column pairs, whose rows then share that XCD's L2 lines.
+template <int E, bool DIRECT>
 __global__ __launch_bounds__(MAX_FFT_THREADS) void k_col(Ctx c, float *__restrict__ Tt, int guard) {
   extern __shared__ float2 lds[];
-  const int img = find_img(c.geo.colpair, c.n, blockIdx.x);
+  const int blk = DIRECT ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int img = find_img(c.geo.colpair, c.n, blk);
   const ImgDesc d = c.img[img];
   if (guard_off(&c.st[img], guard)) return;
-  const int ja = 2 * (blockIdx.x - c.geo.colpair[img]);
+  const int ja = 2 * (blk - c.geo.colpair[img]);
   const bool hasb = ja + 1 < d.W;
   const int H = d.H;
-  float *la = Tt + d.offT + (long long)ja * d.PT;
+  float *la = DIRECT ? Tt + d.off + ja : Tt + d.offT + (long long)ja * d.PT;
   const DctCtx D = dct_begin(lds, c.plans + d.colplan, c.tab);
-  dct2_pair_load<E>(D, la, hasb ? la + d.PT : nullptr);
+  if (DIRECT) {
+    constexpr int U = 4;
+    const int T = blockDim.x;
+    for (int m0 = threadIdx.x; m0 < H; m0 += U * T) {
+      float2 v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int m = m0 + u * T;
+        v[u] = make_float2(0.f, 0.f);
+        if (m < H) {
+          const float *p = la + (long long)m * d.P;
+          v[u] = hasb ? *(const float2 *)p : make_float2(p[0], 0.f);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (m0 + u * T < H) dct2_pair_put(D, m0 + u * T, v[u].x, v[u].y);
+    }
+    dct2_pair_run<E>(D);
+  } else {
+    dct2_pair_load<E>(D, la, hasb ? la + d.PT : nullptr);
+  }
   const double *cosH = c.dtab + d.cosH_off;
   const double cwa = 2.0 * c.dtab[d.cosW_off + ja];
   const double cwb = hasb ? 2.0 * c.dtab[d.cosW_off + ja + 1] : 0.0;
@@ -556,8 +581,14 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) void k_col(Ctx c, float *__restric
   for (int m = threadIdx.x; m < H; m += T) {
     float a, b;
     dct3_pair_value(D, m, a, b);
-    la[m] = a;
-    if (hasb) la[d.PT + m] = b;
+    if (DIRECT) {
+      float *p = la + (long long)m * d.P;
+      if (hasb) *(float2 *)p = make_float2(a, b);
+      else p[0] = a;
+    } else {
+      la[m] = a;
+      if (hasb) la[d.PT + m] = b;
+    }
   }
 }
 

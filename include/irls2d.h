@@ -137,6 +137,16 @@ int irls2d_plan_describe(int64_t n, int64_t *fft_len, int *radices, int *nradice
 int irls2d_plan_reset(void);
 
 /*
+ * Column-pass strategy (no reference counterpart: the reference transposes with torch).
+ * 0 = automatic (default): the column DCTs read column pairs straight from the row-major field
+ * when at least half of the batch's pixels have Bluestein column transforms (whose arithmetic
+ * hides the strided access), otherwise the field is transposed, transformed and transposed back;
+ * 1 = always transpose; 2 = always direct (columns longer than 12288 always transpose).  Drops
+ * cached layouts and hipGraphs.  core_amd.autotune measures both for a batch.
+ */
+int irls2d_set_column_mode(int mode);
+
+/*
  * Release cached hipGraphs and plans that refer to `workspace` (NULL = all).  Call before
  * freeing a workspace that was used with IRLS2D_FLAG_GRAPH.
  */

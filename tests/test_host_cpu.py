@@ -194,3 +194,10 @@ def test_autotune_candidates_are_valid(lib):
                 assert lib.plan_describe(n) == (fl, rad, ept)
     finally:
         lib.plan_reset()
+
+
+def test_column_mode_validation(lib):
+    with pytest.raises(ValueError):
+        lib.set_column_mode(3)
+    lib.set_column_mode(lib.COLUMN_DIRECT)
+    lib.set_column_mode(lib.COLUMN_AUTO)
