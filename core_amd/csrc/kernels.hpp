@@ -479,6 +479,10 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) void k_row_dct2(Ctx c, float *in_,
 }
 
 // T (H x W, pitch P) -> Tt (W x H, pitch PT) when dir == 0, Tt -> T when dir == 1.
+// 64 x 64 tiles moved as float4: both pitches are multiples of 64 and offsets multiples of 64, so
+// every float4 of a tile row lies inside the row's pitch; elements in the pitch padding are read
+// and written but never used.  The LDS tile has a 65-float row pitch: the four scalar reads that
+// gather an output float4 then hit distinct banks.
 __global__ __launch_bounds__(256) void k_transpose(Ctx c, const float *__restrict__ src, float *__restrict__ dst,
                                                    int dir, int guard) {
   __shared__ float tile[TT][TT + 1];
@@ -487,32 +491,34 @@ __global__ __launch_bounds__(256) void k_transpose(Ctx c, const float *__restric
   if (guard_off(&c.st[img], guard)) return;
   const int local = blockIdx.x - c.geo.trans[img];
   const int ntx = (d.W + TT - 1) / TT;
+  // tile origin in image coordinates: rows ti.., columns tj..
   const int ti = (local / ntx) * TT, tj = (local % ntx) * TT;
-  const int x = threadIdx.x & 63, y0 = threadIdx.x >> 6;
-  const int H = d.H, W = d.W;
-  if (dir == 0) {
-    const float *s = src + d.off;
-    float *o = dst + d.offT;
-    for (int y = y0; y < TT; y += 4) {
-      const int i = ti + y, j = tj + x;
-      if (i < H && j < W) tile[y][x] = s[(long long)i * d.P + j];
+  // source rows (length `sl` valid of pitch `sp`) and destination rows
+  const int srows = dir == 0 ? d.H : d.W, scols = dir == 0 ? d.W : d.H;
+  const int s0 = dir == 0 ? ti : tj, c0 = dir == 0 ? tj : ti;  // tile origin in source coordinates
+  const long long sp = dir == 0 ? d.P : d.PT, dp = dir == 0 ? d.PT : d.P;
+  const float *s = src + (dir == 0 ? d.off : d.offT);
+  float *o = dst + (dir == 0 ? d.offT : d.off);
+  const int q4 = threadIdx.x & 15, r0 = threadIdx.x >> 4;  // float4 column, row within a 16-row slab
+#pragma unroll
+  for (int k = 0; k < TT / 16; ++k) {
+    const int r = r0 + 16 * k;
+    if (s0 + r < srows && c0 + 4 * q4 < scols) {
+      const float4 v = *(const float4 *)(s + (long long)(s0 + r) * sp + c0 + 4 * q4);
+      tile[r][4 * q4 + 0] = v.x;
+      tile[r][4 * q4 + 1] = v.y;
+      tile[r][4 * q4 + 2] = v.z;
+      tile[r][4 * q4 + 3] = v.w;
     }
-    __syncthreads();
-    for (int y = y0; y < TT; y += 4) {
-      const int j = tj + y, i = ti + x;
-      if (i < H && j < W) o[(long long)j * d.PT + i] = tile[x][y];
-    }
-  } else {
-    const float *s = src + d.offT;
-    float *o = dst + d.off;
-    for (int y = y0; y < TT; y += 4) {
-      const int j = tj + y, i = ti + x;
-      if (i < H && j < W) tile[y][x] = s[(long long)j * d.PT + i];
-    }
-    __syncthreads();
-    for (int y = y0; y < TT; y += 4) {
-      const int i = ti + y, j = tj + x;
-      if (i < H && j < W) o[(long long)i * d.P + j] = tile[x][y];
+  }
+  __syncthreads();
+  // destination row j (= source column c0 + j) takes source rows s0 .. s0 + 63 of that column
+#pragma unroll
+  for (int k = 0; k < TT / 16; ++k) {
+    const int j = r0 + 16 * k;
+    if (c0 + j < scols && s0 + 4 * q4 < srows) {
+      const float4 v = make_float4(tile[4 * q4 + 0][j], tile[4 * q4 + 1][j], tile[4 * q4 + 2][j], tile[4 * q4 + 3][j]);
+      *(float4 *)(o + (long long)(c0 + j) * dp + s0 + 4 * q4) = v;
     }
   }
 }
