@@ -55,7 +55,8 @@ struct FftPlan {
   int npass;   // number of Stockham passes
   int tw_off;  // float2 offset of the split twiddle table: exp(-2 pi i b / L), b < 64, then
                // exp(-2 pi i 64 a / L), a < ceil(L / 64)
-  int pad;
+  int ct_len;  // L when the passes are the default power-of-two schedule that has a compile-time
+               // specialisation (1024 .. 8192), else 0
   PassDesc pass[MAX_PASS];
 };
 

@@ -167,11 +167,9 @@ __device__ __forceinline__ int stockham_k(int j, const PassDesc &pd) {
   return k;
 }
 
-// Multiplies butterfly input t by exp(-2 pi i t k / (Ns R)) (no-op in the first pass).
+// Multiplies butterfly input t by exp(-2 pi i t m1 / L) from the split table.
 template <int R>
-__device__ __forceinline__ void stockham_twiddle(float2 *v, const float2 *tws, const PassDesc &pd, int k) {
-  if (pd.Ns == 1) return;
-  const int m1 = __umul24(k, pd.twmul);
+__device__ __forceinline__ void twiddle_inputs(float2 *v, const float2 *tws, int m1) {
   float2 w[R];
   w[1] = twiddle(tws, m1);
 #pragma unroll
@@ -181,6 +179,13 @@ __device__ __forceinline__ void stockham_twiddle(float2 *v, const float2 *tws, c
   }
 #pragma unroll
   for (int t = 1; t < R; ++t) v[t] = cmul(v[t], w[t]);
+}
+
+// Multiplies butterfly input t by exp(-2 pi i t k / (Ns R)) (no-op in the first pass).
+template <int R>
+__device__ __forceinline__ void stockham_twiddle(float2 *v, const float2 *tws, const PassDesc &pd, int k) {
+  if (pd.Ns == 1) return;
+  twiddle_inputs<R>(v, tws, __umul24(k, pd.twmul));
 }
 
 // One Stockham pass of radix R over a length-L sequence in LDS, Ns = product of earlier radices.
@@ -235,9 +240,65 @@ __device__ __forceinline__ void stockham_pass(float2 *buf, const float2 *tws, co
   __syncthreads();
 }
 
+// Compile-time form of stockham_pass for the default power-of-two schedules (radix-16 passes,
+// then one radix-8/4/2 pass): every stride, mask and twiddle step is a constant, so LDS accesses
+// are one base address plus immediate offsets.  Requires NB % 16 == 0 and NS == 1 or NS % 16 == 0.
+template <int R, int NS, int L, int E>
+__device__ __forceinline__ void stockham_pass_ct(float2 *buf, const float2 *tws) {
+  constexpr int NB = L / R;
+  constexpr int MB = (E + R - 1) / R;
+  constexpr int RS = NB + NB / 16;                 // padded distance of x[j] and x[j + NB]
+  constexpr int WS = NS == 1 ? 1 : NS + NS / 16;   // padded distance of y[b] and y[b + NS]
+  constexpr int TWMUL = L / (NS * R);
+  static_assert(NB % 16 == 0 && (NS == 1 ? 16 % R == 0 : NS % 16 == 0), "no constant strides");
+  const int T = blockDim.x, tid = threadIdx.x;
+  float2 v[MB][R];
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int j = tid + q * T;
+    if (j < NB) {
+      const float2 *p = buf + pidx(j);
+#pragma unroll
+      for (int t = 0; t < R; ++t) v[q][t] = p[t * RS];
+      if (NS > 1) twiddle_inputs<R>(v[q], tws, (j & (NS - 1)) * TWMUL);
+      Dft<R>::run(v[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int j = tid + q * T;
+    if (j < NB) {
+      const int k = NS == 1 ? 0 : (j & (NS - 1));
+      float2 *p = buf + pidx((j - k) * R + k);
+#pragma unroll
+      for (int t = 0; t < R; ++t) p[t * WS] = v[q][t];
+    }
+  }
+  __syncthreads();
+}
+
+template <int L, int E>
+__device__ __forceinline__ void fft_lds_pow2(float2 *buf, const float2 *tws) {
+  constexpr int M = __builtin_ctz(L);
+  constexpr int N16 = M / 4, REM = M % 4;
+  static_assert((1 << M) == L && N16 >= 2 && N16 <= 3, "power-of-two lengths 512 .. 8192");
+  stockham_pass_ct<16, 1, L, E>(buf, tws);
+  stockham_pass_ct<16, 16, L, E>(buf, tws);
+  if constexpr (N16 >= 3) stockham_pass_ct<16, 256, L, E>(buf, tws);
+  if constexpr (REM) stockham_pass_ct<(1 << REM), (N16 >= 3 ? 4096 : 256), L, E>(buf, tws);
+}
+
 // Forward complex FFT of length P->L in LDS (in place).  Requires L <= E * blockDim.x.
 template <int E>
 __device__ __forceinline__ void fft_lds(float2 *buf, const float2 *tws, const FftPlan *P) {
+  switch (P->ct_len) {
+    case 8192: fft_lds_pow2<8192, E>(buf, tws); return;
+    case 4096: fft_lds_pow2<4096, E>(buf, tws); return;
+    case 2048: fft_lds_pow2<2048, E>(buf, tws); return;
+    case 1024: fft_lds_pow2<1024, E>(buf, tws); return;
+    default: break;
+  }
   const int np = P->npass;
   for (int s = 0; s < np; ++s) {
     const PassDesc pd = P->pass[s];

@@ -186,6 +186,8 @@ static std::shared_ptr<HostDct> get_plan(int N) {
   P.fft.tw_off = push(tw);
   const std::vector<int> rad = (ov != g_over.end() && !ov->second.rad.empty()) ? ov->second.rad : radices(M);
   P.fft.npass = (int)rad.size();
+  // compile-time schedule available for the default order of 1024 .. 8192
+  P.fft.ct_len = ((M & (M - 1)) == 0 && M >= 1024 && M <= 8192 && rad == radices(M)) ? M : 0;
   int Ns = 1;
   for (size_t s = 0; s < rad.size(); ++s) {
     PassDesc &pd = P.fft.pass[s];
