@@ -15,7 +15,8 @@ Candidates per length N (plan = fft_len, radix order, elements per thread):
   * direct FFT (N 5-smooth): the power-of-two part as radix-16, radix-8 or radix-4 passes, with
     the radix-3/5 passes after or before it;
   * Bluestein (prime factors > 5): the three 5-smooth lengths M >= 2N - 1 of lowest
-    M * (passes + 1), each with its default order;
+    M * (passes + 1) and the lengths with compile-time schedules in that range, each with its
+    default order;
   * 16 or 24 elements per thread where the FFT length allows both.
 """
 from __future__ import annotations
@@ -64,11 +65,23 @@ def _default_radices(m: int):
     return out
 
 
+def _has_ct(m: int) -> bool:
+    """Lengths with a compile-time pass schedule in the library: 2^a * {1, 3, 5}, 1024 .. 12288."""
+    if m < 1024 or m > MAX_FFT_LEN:
+        return False
+    while m % 2 == 0:
+        m //= 2
+    return m in (1, 3, 5)
+
+
 def bluestein_candidates(n: int, k: int = 3):
+    """The k 5-smooth lengths M >= 2n - 1 of lowest M * (passes + 1) (within 30%), plus every
+    length in that range with a compile-time schedule."""
     lo = 2 * n - 1
     hi = min(MAX_FFT_LEN, lo + lo * 3 // 10 + 8)
     cands = [(m * (len(_default_radices(m)) + 1), m) for m in range(lo, hi + 1) if _smooth(m)]
-    return [m for _, m in sorted(cands)[:k]]
+    best = [m for _, m in sorted(cands)[:k]]
+    return best + [m for m in range(lo, hi + 1) if _has_ct(m) and m not in best]
 
 
 def candidates(n: int):

@@ -278,26 +278,44 @@ __device__ __forceinline__ void stockham_pass_ct(float2 *buf, const float2 *tws)
   __syncthreads();
 }
 
+// Default schedule of L = 2^a * f (f = 1, 3 or 5): radix-16 passes, one radix-8/4/2 pass for the
+// rest of 2^a, then one radix-f pass (the order radices() produces), all with constant strides.
 template <int L, int E>
-__device__ __forceinline__ void fft_lds_pow2(float2 *buf, const float2 *tws) {
-  constexpr int M = __builtin_ctz(L);
-  constexpr int N16 = M / 4, REM = M % 4;
-  static_assert((1 << M) == L && N16 >= 2 && N16 <= 3, "power-of-two lengths 512 .. 8192");
+__device__ __forceinline__ void fft_lds_ct(float2 *buf, const float2 *tws) {
+  constexpr int A = __builtin_ctz(L);
+  constexpr int F = L >> A;
+  constexpr int N16 = A / 4, REM = A % 4;
+  static_assert((F == 1 || F == 3 || F == 5) && N16 >= 2 && N16 <= 3, "lengths 2^a * {1, 3, 5}, 1024 .. 12288");
+  constexpr int NS_REM = N16 == 3 ? 4096 : 256;
   stockham_pass_ct<16, 1, L, E>(buf, tws);
   stockham_pass_ct<16, 16, L, E>(buf, tws);
   if constexpr (N16 >= 3) stockham_pass_ct<16, 256, L, E>(buf, tws);
-  if constexpr (REM) stockham_pass_ct<(1 << REM), (N16 >= 3 ? 4096 : 256), L, E>(buf, tws);
+  if constexpr (REM) stockham_pass_ct<(1 << REM), NS_REM, L, E>(buf, tws);
+  if constexpr (F > 1) stockham_pass_ct<F, (1 << A), L, E>(buf, tws);
 }
 
 // Forward complex FFT of length P->L in LDS (in place).  Requires L <= E * blockDim.x.
 template <int E>
 __device__ __forceinline__ void fft_lds(float2 *buf, const float2 *tws, const FftPlan *P) {
   switch (P->ct_len) {
-    case 8192: fft_lds_pow2<8192, E>(buf, tws); return;
-    case 4096: fft_lds_pow2<4096, E>(buf, tws); return;
-    case 2048: fft_lds_pow2<2048, E>(buf, tws); return;
-    case 1024: fft_lds_pow2<1024, E>(buf, tws); return;
+    case 1024: fft_lds_ct<1024, E>(buf, tws); return;
+    case 1280: fft_lds_ct<1280, E>(buf, tws); return;
+    case 1536: fft_lds_ct<1536, E>(buf, tws); return;
+    case 2048: fft_lds_ct<2048, E>(buf, tws); return;
+    case 2560: fft_lds_ct<2560, E>(buf, tws); return;
+    case 3072: fft_lds_ct<3072, E>(buf, tws); return;
+    case 4096: fft_lds_ct<4096, E>(buf, tws); return;
+    case 5120: fft_lds_ct<5120, E>(buf, tws); return;
+    case 6144: fft_lds_ct<6144, E>(buf, tws); return;
+    case 8192: fft_lds_ct<8192, E>(buf, tws); return;
     default: break;
+  }
+  if constexpr (E >= 24) {
+    switch (P->ct_len) {
+      case 10240: fft_lds_ct<10240, E>(buf, tws); return;
+      case 12288: fft_lds_ct<12288, E>(buf, tws); return;
+      default: break;
+    }
   }
   const int np = P->npass;
   for (int s = 0; s < np; ++s) {

@@ -101,6 +101,14 @@ static std::map<int, PlanOverride> g_over;
 // row-major field (irls2d_set_column_mode).
 static int g_col_mode = 0;
 
+// FFT lengths with a compile-time Stockham schedule (fft_lds_ct): 2^a * {1, 3, 5}, 1024 .. 12288.
+static bool has_ct(int64_t M) {
+  if (M < 1024 || M > MAX_FFT_LEN) return false;
+  int64_t f = M;
+  while (f % 2 == 0) f /= 2;
+  return f == 1 || f == 3 || f == 5;
+}
+
 // Bluestein length: the 5-smooth M >= 2N - 1 (within 30%) with the lowest M * (passes + 1).
 static int64_t bluestein_len(int64_t N) {
   const int64_t lo = 2 * N - 1;
@@ -186,8 +194,7 @@ static std::shared_ptr<HostDct> get_plan(int N) {
   P.fft.tw_off = push(tw);
   const std::vector<int> rad = (ov != g_over.end() && !ov->second.rad.empty()) ? ov->second.rad : radices(M);
   P.fft.npass = (int)rad.size();
-  // compile-time schedule available for the default order of 1024 .. 8192
-  P.fft.ct_len = ((M & (M - 1)) == 0 && M >= 1024 && M <= 8192 && rad == radices(M)) ? M : 0;
+  P.fft.ct_len = (has_ct(M) && rad == radices(M)) ? M : 0;
   int Ns = 1;
   for (size_t s = 0; s < rad.size(); ++s) {
     PassDesc &pd = P.fft.pass[s];
