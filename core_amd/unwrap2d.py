@@ -160,12 +160,33 @@ def pack(arrays, ws, device):
     return host.to(device, non_blocking=True)
 
 
+def memory_groups(shapes, budget_bytes):
+    """Split image shapes (in order) into consecutive groups whose batched workspace plus the
+    phase / weight / output arenas stay within `budget_bytes` (a single image that alone exceeds
+    it still forms its own group).  Returns lists of indices."""
+    groups, cur, cur_bytes = [], [], 0
+    for i, (h, w) in enumerate(shapes):
+        pitch = (int(w) + 63) // 64 * 64
+        need = _lib.workspace_size([(h, w)]) + 3 * int(h) * pitch * 4
+        if cur and cur_bytes + need > budget_bytes:
+            groups.append(cur)
+            cur, cur_bytes = [], 0
+        cur.append(i)
+        cur_bytes += need
+    if cur:
+        groups.append(cur)
+    return groups
+
+
 def irls_unwrap_2d_batch(phases, weights=None, device='auto', max_iter=50, tol=1e-3, cg_max_iter=20,
-                         cg_tol=1e-4, epsilon=1e-2, conncomp_size=30, debug=False, graph=False):
-    """Unwrap a list of 2-D interferograms in one device solve.
+                         cg_tol=1e-4, epsilon=1e-2, conncomp_size=30, debug=False, graph=False,
+                         memory_fraction=0.6):
+    """Unwrap a list of 2-D interferograms, all images of a group in one device solve.
 
     Each image gets exactly the per-image semantics of `irls_unwrap_2d` (crop, solve, epilogue);
-    returns a list of (unwrapped float32, conncomp uint16) tuples in input order.
+    returns a list of (unwrapped float32, conncomp uint16) tuples in input order.  The images are
+    split into consecutive groups whose device memory (workspace and arenas) fits within
+    `memory_fraction` of the free HBM, so stacks of any length can be passed.
     """
     phases = list(phases)
     if weights is None:
@@ -188,11 +209,23 @@ def irls_unwrap_2d_batch(phases, weights=None, device='auto', max_iter=50, tol=1
         jobs.append((k, ph, None if wt is None else np.asarray(wt), nan_mask[r0:r1, c0:c1], box))
     if not jobs:
         return results
+    shapes_all = [(j[4][1] - j[4][0], j[4][3] - j[4][2]) for j in jobs]
+    free, _total = torch.cuda.mem_get_info(dev)
+    groups = memory_groups(shapes_all, int(free * memory_fraction))
+    for g in groups:
+        # one group: keep its workspace cached for the next call; several: free each before the next
+        _solve_group([jobs[i] for i in g], results, dev, max_iter, tol, cg_max_iter, cg_tol, epsilon,
+                     conncomp_size, debug, graph, cache=len(groups) == 1)
+    return results
+
+
+def _solve_group(jobs, results, dev, max_iter, tol, cg_max_iter, cg_tol, epsilon, conncomp_size, debug, graph,
+                 cache=True):
     t0 = time.time()
     shapes = [(j[4][1] - j[4][0], j[4][3] - j[4][2]) for j in jobs]
     has_w = [j[2] is not None for j in jobs]
     with _gpu_lock, torch.cuda.device(dev):
-        ws = Workspace.get(shapes, dev)
+        ws = Workspace.get(shapes, dev) if cache else Workspace(shapes, dev)
         ph_t = pack([j[1][j[4][0]:j[4][1], j[4][2]:j[4][3]] for j in jobs], ws, dev)
         w_t = None
         if any(has_w):
@@ -212,6 +245,8 @@ def irls_unwrap_2d_batch(phases, weights=None, device='auto', max_iter=50, tol=1
         info_h = info.cpu().numpy().copy()
         torch.cuda.synchronize(dev)
         t2 = time.time()
+        if not cache:
+            del ph_t, w_t, u_t, info
     uh = u_host.numpy()
     for idx, ((k, ph, wt, nm, box), (h, w), inf) in enumerate(zip(jobs, shapes, info_h)):
         u = image_view(uh, ws, idx).copy()
@@ -223,11 +258,12 @@ def irls_unwrap_2d_batch(phases, weights=None, device='auto', max_iter=50, tol=1
         results[k] = pp.uncrop(ep[0], ep[1], box, ph.shape)
         if debug:
             print(f'  image {k}: {h}x{w} irls_iters={inf[1]} cg_iters={inf[2]} status={inf[0]} valid={inf[3]}')
+    if not cache:
+        ws.release()  # its device memory returns to torch's allocator with the object
     if debug:
         t3 = time.time()
         print(f'TIMING irls_unwrap_2d_batch ({len(jobs)} images) on {dev}: upload {(t1 - t0) * 1e3:.1f} ms, '
               f'solve+download {(t2 - t1) * 1e3:.1f} ms, epilogue {(t3 - t2) * 1e3:.1f} ms')
-    return results
 
 
 def irls_unwrap_2d(phase, weight=None, device='auto', max_iter=50, tol=1e-3,

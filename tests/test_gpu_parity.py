@@ -191,3 +191,14 @@ def test_column_modes_agree(hip, H, W, seed):
             assert np.array_equal(ub, up, equal_nan=True) and np.array_equal(cb, cp)
     finally:
         _lib.set_column_mode(_lib.COLUMN_AUTO)
+
+
+def test_batch_split_by_memory_equals_one_group(hip):
+    """A stack split into several device solves (tiny memory budget) gives the same results."""
+    from core_amd import irls_unwrap_2d_batch
+    cases = [_case(50, 60, 70 + i, i == 1, 0.05 if i == 2 else 0.0) for i in range(4)]
+    a = irls_unwrap_2d_batch([c[0] for c in cases], [c[1] for c in cases], device='cuda', **FIXED)
+    b = irls_unwrap_2d_batch([c[0] for c in cases], [c[1] for c in cases], device='cuda', memory_fraction=1e-9,
+                             **FIXED)
+    for (ua, ca), (ub, cb) in zip(a, b):
+        assert np.array_equal(ua, ub, equal_nan=True) and np.array_equal(ca, cb)

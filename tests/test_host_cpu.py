@@ -201,3 +201,15 @@ def test_column_mode_validation(lib):
         lib.set_column_mode(3)
     lib.set_column_mode(lib.COLUMN_DIRECT)
     lib.set_column_mode(lib.COLUMN_AUTO)
+
+
+def test_memory_groups_split_in_order(lib):
+    from core_amd.unwrap2d import memory_groups
+    shapes = [(100, 200), (300, 50), (64, 64), (500, 500), (10, 10)]
+    one = [lib.workspace_size([s]) + 3 * s[0] * ((s[1] + 63) // 64 * 64) * 4 for s in shapes]
+    assert memory_groups(shapes, 10 ** 12) == [[0, 1, 2, 3, 4]]
+    g = memory_groups(shapes, one[3] - 1)      # image 3 alone exceeds the budget: its own group
+    assert [i for grp in g for i in grp] == list(range(5))
+    assert [3] in g
+    for grp in g:
+        assert len(grp) == 1 or sum(one[i] for i in grp) <= one[3] - 1
