@@ -32,6 +32,13 @@ EXPORTS = (
     "irls2d_plan_describe",
     "irls2d_plan_reset",
     "irls2d_set_column_mode",
+    "irls2d_ep_blocks",
+    "irls2d_ep_scan",
+    "irls2d_ep_hist",
+    "irls2d_ep_label",
+    "irls2d_ep_number",
+    "irls2d_ep_sums",
+    "irls2d_ep_apply",
     "irls2d_release",
     "irls2d_kernel_stats",
     "irls2d_kernel_stats_reset",

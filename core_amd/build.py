@@ -13,29 +13,47 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "irls2d.hip")
+CSRC = os.path.join(HERE, "csrc")
+# translation units, compiled in parallel: the host code and small kernels, and the transform
+# kernel instantiations (the slow ones to compile) in three units of their own
+SOURCES = ["irls2d.hip", "fft_e16.hip", "fft_e24.hip", "long_passes.hip"]
+SRC = os.path.join(CSRC, "irls2d.hip")
 OUT = os.path.join(HERE, "libirls2d.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("IRLS2D_ARCH", "gfx950")
+FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    # no packed-FP32 instructions (v_pk_*_f32) in the device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics",
+    "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+]
 
 
 def command(out: str = OUT, extra=()):
-    return [
-        HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-        # no packed-FP32 instructions (v_pk_*_f32) in the device code
-        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-munsafe-fp-atomics",
-        "-I", os.path.join(ROOT, "include"), "-I", os.path.join(HERE, "csrc"),
-        *extra, SRC, "-o", out,
-    ]
+    """Single-invocation form (all sources at once), kept for reference builds."""
+    return [HIPCC, *FLAGS, "-shared", *extra, *[os.path.join(CSRC, f) for f in SOURCES], "-o", out]
 
 
 def build(verbose: bool = True) -> str:
+    objdir = os.path.join(HERE, "_obj")
+    os.makedirs(objdir, exist_ok=True)
+    procs, objs = [], []
+    for f in SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(f)[0] + ".o")
+        cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, f), "-o", obj]
+        if verbose:
+            print(" ".join(shlex.quote(c) for c in cmd), flush=True)
+        procs.append((f, subprocess.Popen(cmd)))
+        objs.append(obj)
+    failed = [f for f, p in procs if p.wait() != 0]
+    if failed:
+        raise RuntimeError(f"hipcc failed on {failed}")
     tmp = OUT + ".tmp"
-    cmd = command(tmp)
+    link = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:
-        print(" ".join(shlex.quote(c) for c in cmd), flush=True)
-    subprocess.run(cmd, check=True)
+        print(" ".join(shlex.quote(c) for c in link), flush=True)
+    subprocess.run(link, check=True)
     os.replace(tmp, OUT)
     return OUT
 

@@ -17,8 +17,15 @@
 #include <vector>
 
 #include "irls2d.h"
-#include "kernels.hpp"
-#include "long_dct.hpp"
+#include "epilogue.hpp"
+#include "instances.hpp"
+
+namespace irls2d {
+// defined in fft_e16.hip, fft_e24.hip and long_passes.hip
+IRLS2D_FFT_KERNELS(16, extern)
+IRLS2D_FFT_KERNELS(24, extern)
+IRLS2D_LONG_KERNELS(extern)
+}  // namespace irls2d
 
 namespace irls2d {
 
@@ -1093,6 +1100,83 @@ int plan_reset() {
   return IRLS2D_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Device epilogue steps (one image, pitched view; the host takes the median and size order)
+// ---------------------------------------------------------------------------------------------
+
+static int ep_check(int64_t H, int64_t W, int64_t P) {
+  g_err.clear();
+  if (H < 1 || W < 1 || P < W) return fail(IRLS2D_EINVAL, "bad image shape or pitch");
+  if (H * P >= ((int64_t)1 << 31)) return fail(IRLS2D_EUNSUPPORTED, "image too large for the device epilogue");
+  return IRLS2D_OK;
+}
+
+static dim3 ep_grid_stride(int64_t n) {
+  return dim3((unsigned)std::min<int64_t>((n + EP_THREADS - 1) / EP_THREADS, 256 * 64));
+}
+
+int64_t ep_blocks(int64_t H, int64_t P) { return (H * P + EP_BLOCK_PX - 1) / EP_BLOCK_PX; }
+
+int ep_scan(int64_t H, int64_t W, int64_t P, const float *psi, const float *u, int32_t *stats, hipStream_t s) {
+  int rc = ep_check(H, W, P);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_ep_scan, ep_grid_stride(H * P), dim3(EP_THREADS), 0, s, (int)H, (int)W, (int)P, psi, u, stats);
+  HIP_TRY(hipGetLastError());
+  return IRLS2D_OK;
+}
+
+int ep_hist(int64_t H, int64_t W, int64_t P, const float *psi, const float *u, int kmin, int nb, int32_t *hist,
+            hipStream_t s) {
+  int rc = ep_check(H, W, P);
+  if (rc) return rc;
+  if (nb < 1) return fail(IRLS2D_EINVAL, "empty histogram");
+  hipLaunchKernelGGL(k_ep_hist, ep_grid_stride(H * P), dim3(EP_THREADS), 0, s, (int)H, (int)W, (int)P, psi, u, kmin,
+                     nb, hist);
+  HIP_TRY(hipGetLastError());
+  return IRLS2D_OK;
+}
+
+int ep_label(int64_t H, int64_t W, int64_t P, const float *psi, float *u, float shift, int32_t *L, int32_t *counts,
+             hipStream_t s) {
+  int rc = ep_check(H, W, P);
+  if (rc) return rc;
+  const dim3 g = ep_grid_stride(H * P), b(EP_THREADS);
+  hipLaunchKernelGGL(k_ep_shift, g, b, 0, s, (int)H, (int)W, (int)P, psi, u, shift, L);
+  hipLaunchKernelGGL(k_ccl_merge, g, b, 0, s, (int)H, (int)W, (int)P, L);
+  hipLaunchKernelGGL(k_ccl_flat, dim3((unsigned)ep_blocks(H, P)), b, 0, s, (int)H, (int)P, L, counts);
+  HIP_TRY(hipGetLastError());
+  return IRLS2D_OK;
+}
+
+int ep_number(int64_t H, int64_t P, const int32_t *L, const int32_t *offsets, int32_t *R, hipStream_t s) {
+  int rc = ep_check(H, 1, P);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_ccl_number, dim3((unsigned)ep_blocks(H, P)), dim3(EP_THREADS), 0, s, (int)H, (int)P, L,
+                     offsets, R);
+  HIP_TRY(hipGetLastError());
+  return IRLS2D_OK;
+}
+
+int ep_sums(int64_t H, int64_t W, int64_t P, const float *psi, const float *u, const int32_t *L, const int32_t *R,
+            double *sums, uint64_t *cnt, hipStream_t s) {
+  int rc = ep_check(H, W, P);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_ep_sums, dim3((unsigned)ep_blocks(H, P)), dim3(EP_THREADS), 0, s, (int)H, (int)W, (int)P, psi,
+                     u, L, R, sums, (unsigned long long *)cnt);
+  HIP_TRY(hipGetLastError());
+  return IRLS2D_OK;
+}
+
+int ep_apply(int64_t H, int64_t W, int64_t P, float *u, const int32_t *L, const int32_t *R, const double *adj,
+             const uint16_t *relabel, uint16_t *cc, hipStream_t s) {
+  int rc = ep_check(H, W, P);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_ep_apply, ep_grid_stride(H * P), dim3(EP_THREADS), 0, s, (int)H, (int)W, (int)P, u, L, R, adj,
+                     (const unsigned short *)relabel, (unsigned short *)cc);
+  HIP_TRY(hipGetLastError());
+  return IRLS2D_OK;
+}
+
 int release(void *workspace) {
   std::lock_guard<std::mutex> lk(g_mu);
   for (auto it = g_graphs.begin(); it != g_graphs.end();) {
@@ -1143,6 +1227,37 @@ int irls2d_solve(int n, const int64_t *H, const int64_t *W, const float *phase, 
 }
 
 int irls2d_release(void *workspace) { return irls2d::release(workspace); }
+
+int64_t irls2d_ep_blocks(int64_t H, int64_t P) { return irls2d::ep_blocks(H, P); }
+
+int irls2d_ep_scan(int64_t H, int64_t W, int64_t P, const float *phase, const float *u, int32_t *stats, void *stream) {
+  return irls2d::ep_scan(H, W, P, phase, u, stats, (hipStream_t)stream);
+}
+
+int irls2d_ep_hist(int64_t H, int64_t W, int64_t P, const float *phase, const float *u, int32_t kmin, int32_t nbins,
+                   int32_t *hist, void *stream) {
+  return irls2d::ep_hist(H, W, P, phase, u, kmin, nbins, hist, (hipStream_t)stream);
+}
+
+int irls2d_ep_label(int64_t H, int64_t W, int64_t P, const float *phase, float *u, float shift, int32_t *parent,
+                    int32_t *block_roots, void *stream) {
+  return irls2d::ep_label(H, W, P, phase, u, shift, parent, block_roots, (hipStream_t)stream);
+}
+
+int irls2d_ep_number(int64_t H, int64_t P, const int32_t *parent, const int32_t *block_offsets, int32_t *root_label,
+                     void *stream) {
+  return irls2d::ep_number(H, P, parent, block_offsets, root_label, (hipStream_t)stream);
+}
+
+int irls2d_ep_sums(int64_t H, int64_t W, int64_t P, const float *phase, const float *u, const int32_t *parent,
+                   const int32_t *root_label, double *sums, uint64_t *counts, void *stream) {
+  return irls2d::ep_sums(H, W, P, phase, u, parent, root_label, sums, counts, (hipStream_t)stream);
+}
+
+int irls2d_ep_apply(int64_t H, int64_t W, int64_t P, float *u, const int32_t *parent, const int32_t *root_label,
+                    const double *adjust, const uint16_t *relabel, uint16_t *conncomp, void *stream) {
+  return irls2d::ep_apply(H, W, P, u, parent, root_label, adjust, relabel, conncomp, (hipStream_t)stream);
+}
 
 int irls2d_plan_override(int64_t n, int64_t fft_len, const int *radices, int nradices, int ept) {
   return irls2d::plan_override(n, fft_len, radices, nradices, ept);

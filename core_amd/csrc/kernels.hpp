@@ -90,6 +90,7 @@ __device__ __forceinline__ float4 apply_A4(int i, int c0, int H, int W, float4 x
   return make_float4(a[0], a[1], a[2], a[3]);
 }
 
+#ifndef IRLS2D_TRANSFORMS_ONLY  // the transform-only translation units need none of these
 // ---------------------------------------------------------------------------------------------
 // Set-up
 // ---------------------------------------------------------------------------------------------
@@ -393,6 +394,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_cg_pap(Ctx c, const float *__r
   block_partial<1>(acc, c.partials, t.blk);
 }
 
+#endif  // IRLS2D_TRANSFORMS_ONLY
 // ---------------------------------------------------------------------------------------------
 // Preconditioner z = IDCT2(-DCT2(r) / lambda), DC zeroed (utils_unwrap2d.py:1390-1403), as
 // row DCT-II -> transpose -> column DCT-II * (-1/lambda) * column DCT-III -> transpose ->
@@ -478,6 +480,7 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) void k_row_dct2(Ctx c, float *in_,
   if (FUSE_UPDATE) block_partial<2>(acc, c.partials);
 }
 
+#ifndef IRLS2D_TRANSFORMS_ONLY
 // T (H x W, pitch P) -> Tt (W x H, pitch PT) when dir == 0, Tt -> T when dir == 1.
 // 64 x 64 tiles moved as float4: both pitches are multiples of 64 and offsets multiples of 64, so
 // every float4 of a tile row lies inside the row's pitch; elements in the pitch padding are read
@@ -523,6 +526,7 @@ __global__ __launch_bounds__(256) void k_transpose(Ctx c, const float *__restric
   }
 }
 
+#endif  // IRLS2D_TRANSFORMS_ONLY
 // Column pass on the transposed field: DCT-II along H, divide by -lambda (float32 lambda from
 // float64 cosines, utils_unwrap2d.py:1284-1289, :1395-1398), zero the DC term, inverse DCT.
 // DIRECT: the column pair is read from and written back to the row-major field (one float2 per
@@ -645,6 +649,7 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) void k_row_dct3(Ctx c, const float
   if (dot) block_partial<1>(acc, c.partials);
 }
 
+#ifndef IRLS2D_TRANSFORMS_ONLY
 // ---------------------------------------------------------------------------------------------
 // End of an IRLS step (utils_unwrap2d.py:1599-1626)
 // ---------------------------------------------------------------------------------------------
@@ -870,4 +875,5 @@ __global__ void k_info(Ctx c, int32_t *info) {
   }
 }
 
+#endif  // IRLS2D_TRANSFORMS_ONLY
 }  // namespace irls2d

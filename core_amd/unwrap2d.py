@@ -18,6 +18,7 @@ import torch
 
 from . import _lib
 from . import postprocess as pp
+from .epilogue_gpu import epilogue_device
 
 __all__ = ["irls_unwrap_2d", "irls_unwrap_2d_batch", "solve_device", "precondition_device", "resolve_device",
            "Workspace"]
@@ -180,14 +181,19 @@ def memory_groups(shapes, budget_bytes):
 
 def irls_unwrap_2d_batch(phases, weights=None, device='auto', max_iter=50, tol=1e-3, cg_max_iter=20,
                          cg_tol=1e-4, epsilon=1e-2, conncomp_size=30, debug=False, graph=False,
-                         memory_fraction=0.6):
+                         memory_fraction=0.6, epilogue='device'):
     """Unwrap a list of 2-D interferograms, all images of a group in one device solve.
 
     Each image gets exactly the per-image semantics of `irls_unwrap_2d` (crop, solve, epilogue);
     returns a list of (unwrapped float32, conncomp uint16) tuples in input order.  The images are
     split into consecutive groups whose device memory (workspace and arenas) fits within
-    `memory_fraction` of the free HBM, so stacks of any length can be passed.
+    `memory_fraction` of the free HBM, so stacks of any length can be passed.  epilogue='device'
+    (default) runs the post-processing of utils_unwrap2d.py:1659-1746 in the library's kernels,
+    'host' in numpy/scipy (core_amd.postprocess); both give the same conncomp labels and the same
+    phase to float32 rounding of the per-component sin/cos sums.
     """
+    if epilogue not in ('device', 'host'):
+        raise ValueError("epilogue must be 'device' or 'host'")
     phases = list(phases)
     if weights is None:
         weights = [None] * len(phases)
@@ -215,12 +221,12 @@ def irls_unwrap_2d_batch(phases, weights=None, device='auto', max_iter=50, tol=1
     for g in groups:
         # one group: keep its workspace cached for the next call; several: free each before the next
         _solve_group([jobs[i] for i in g], results, dev, max_iter, tol, cg_max_iter, cg_tol, epsilon,
-                     conncomp_size, debug, graph, cache=len(groups) == 1)
+                     conncomp_size, debug, graph, cache=len(groups) == 1, epilogue=epilogue)
     return results
 
 
 def _solve_group(jobs, results, dev, max_iter, tol, cg_max_iter, cg_tol, epsilon, conncomp_size, debug, graph,
-                 cache=True):
+                 cache=True, epilogue='device'):
     t0 = time.time()
     shapes = [(j[4][1] - j[4][0], j[4][3] - j[4][2]) for j in jobs]
     has_w = [j[2] is not None for j in jobs]
@@ -240,18 +246,25 @@ def _solve_group(jobs, results, dev, max_iter, tol, cg_max_iter, cg_tol, epsilon
         t1 = time.time()
         u_t, info = solve_device(ph_t, w_t, shapes, max_iter, tol, cg_max_iter, cg_tol, epsilon,
                                  graph=graph, workspace=ws)
-        u_host = torch.empty(ws.total, dtype=torch.float32, pin_memory=True)
-        u_host.copy_(u_t, non_blocking=True)
         info_h = info.cpu().numpy().copy()
+        if epilogue == 'device':
+            # per-pixel epilogue work on the device; only the images come back to the host
+            eps = [epilogue_device(ph_t, u_t, ws.offsets[idx], ws.pitches[idx], h, w, conncomp_size)
+                   for idx, (h, w) in enumerate(shapes)]
+        else:
+            u_host = torch.empty(ws.total, dtype=torch.float32, pin_memory=True)
+            u_host.copy_(u_t, non_blocking=True)
         torch.cuda.synchronize(dev)
         t2 = time.time()
         if not cache:
             del ph_t, w_t, u_t, info
-    uh = u_host.numpy()
-    for idx, ((k, ph, wt, nm, box), (h, w), inf) in enumerate(zip(jobs, shapes, info_h)):
-        u = image_view(uh, ws, idx).copy()
-        r0, r1, c0, c1 = box
-        ep = pp.epilogue(u, ph[r0:r1, c0:c1], nm, conncomp_size)
+    if epilogue != 'device':
+        uh = u_host.numpy()
+        eps = []
+        for idx, (k, ph, wt, nm, box) in enumerate(jobs):
+            r0, r1, c0, c1 = box
+            eps.append(pp.epilogue(image_view(uh, ws, idx).copy(), ph[r0:r1, c0:c1], nm, conncomp_size))
+    for (k, ph, wt, nm, box), (h, w), inf, ep in zip(jobs, shapes, info_h, eps):
         if ep is None:
             results[k] = (np.full(ph.shape, np.nan, np.float32), np.zeros(ph.shape, np.uint16))
             continue

@@ -147,6 +147,42 @@ int irls2d_plan_reset(void);
 int irls2d_set_column_mode(int mode);
 
 /*
+ * Device epilogue of one solved image (replaces the host part of utils_unwrap2d.py:1659-1746: the
+ * non-finite check, the 2 pi median shift, 4-connected labelling, the per-component circular-mean
+ * alignment of :24-75 and the relabel by size).  The image is an H x W view with row pitch P
+ * (elements) of the phase arena and of the solver output `u` (updated in place); the other arrays
+ * are device scratch owned by the caller.  The host runs the steps in order and takes the two
+ * decisions with numpy semantics between them (core_amd/epilogue_gpu.py):
+ *   irls2d_ep_scan   stats[4] (int32, preset {0, INT32_MAX, INT32_MIN, 0}) += {non-finite u on
+ *                    valid pixels, min k, max k, valid pixels}, k = rint((u - phase) / 2 pi)
+ *   irls2d_ep_hist   hist[k - kmin] += 1 (nbins = kmax - kmin + 1, zeroed)     -> host: median
+ *   irls2d_ep_label  u -= shift on valid pixels, NaN elsewhere; parent[H * P]: 4-connected union
+ *                    find, every valid pixel pointing at its component's raster-first pixel,
+ *                    -1 elsewhere; block_roots[irls2d_ep_blocks(H, P)] = roots per block
+ *   irls2d_ep_number root_label[root] = raster-order component number (1-based) from the
+ *                    exclusive prefix sums of block_roots (block_offsets)
+ *   irls2d_ep_sums   sums[4 c + 0..3] += sin phase, cos phase, sin u, cos u (float64), counts[c]
+ *                    += 1 over component c's pixels (arrays of n_components + 1, zeroed)
+ *                    -> host: adjustment per component and size relabel
+ *   irls2d_ep_apply  u = float(double(u) + adjust[c]), conncomp = relabel[c] (uint16, pitch P),
+ *                    0 on invalid pixels
+ * No reference counterpart as an interface (the reference does these steps with numpy, cv2 and
+ * numba on the host).  Images with H * P >= 2^31 are rejected (IRLS2D_EUNSUPPORTED).
+ */
+int64_t irls2d_ep_blocks(int64_t H, int64_t P);
+int irls2d_ep_scan(int64_t H, int64_t W, int64_t P, const float *phase, const float *u, int32_t *stats, void *stream);
+int irls2d_ep_hist(int64_t H, int64_t W, int64_t P, const float *phase, const float *u, int32_t kmin, int32_t nbins,
+                   int32_t *hist, void *stream);
+int irls2d_ep_label(int64_t H, int64_t W, int64_t P, const float *phase, float *u, float shift, int32_t *parent,
+                    int32_t *block_roots, void *stream);
+int irls2d_ep_number(int64_t H, int64_t P, const int32_t *parent, const int32_t *block_offsets, int32_t *root_label,
+                     void *stream);
+int irls2d_ep_sums(int64_t H, int64_t W, int64_t P, const float *phase, const float *u, const int32_t *parent,
+                   const int32_t *root_label, double *sums, uint64_t *counts, void *stream);
+int irls2d_ep_apply(int64_t H, int64_t W, int64_t P, float *u, const int32_t *parent, const int32_t *root_label,
+                    const double *adjust, const uint16_t *relabel, uint16_t *conncomp, void *stream);
+
+/*
  * Release cached hipGraphs and plans that refer to `workspace` (NULL = all).  Call before
  * freeing a workspace that was used with IRLS2D_FLAG_GRAPH.
  */
