@@ -32,6 +32,7 @@ EXPORTS = (
     "irls2d_plan_describe",
     "irls2d_plan_reset",
     "irls2d_set_column_mode",
+    "irls2d_describe_columns",
     "irls2d_ep_blocks",
     "irls2d_ep_scan",
     "irls2d_ep_hist",
@@ -95,6 +96,8 @@ def load():
     L.irls2d_plan_reset.restype = ctypes.c_int
     L.irls2d_set_column_mode.argtypes = [ctypes.c_int]
     L.irls2d_set_column_mode.restype = ctypes.c_int
+    L.irls2d_describe_columns.argtypes = [ctypes.c_int, i64p, i64p, ctypes.POINTER(ctypes.c_int32)]
+    L.irls2d_describe_columns.restype = ctypes.c_int
     L.irls2d_release.argtypes = [ctypes.c_void_p]
     L.irls2d_release.restype = ctypes.c_int
     L.irls2d_kernel_stats.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), i64p]
@@ -186,9 +189,21 @@ def plan_reset():
     check(load().irls2d_plan_reset(), "irls2d_plan_reset")
 
 
-COLUMN_AUTO, COLUMN_TRANSPOSE, COLUMN_DIRECT = 0, 1, 2
+COLUMN_AUTO, COLUMN_TRANSPOSE, COLUMN_DIRECT, COLUMN_SLAB = 0, 1, 2, 3
+
+
+COLUMN_STRATEGIES = ("transpose", "direct", "long", "slab", "slab_register")
+
+
+def describe_columns(shapes):
+    """Column strategy of each image of a batch of (H, W) shapes (a name from COLUMN_STRATEGIES)."""
+    n, H, W = shapes_arrays(shapes)
+    out = (ctypes.c_int32 * n)()
+    check(load().irls2d_describe_columns(n, H, W, out), "irls2d_describe_columns")
+    return [COLUMN_STRATEGIES[v] for v in out]
 
 
 def set_column_mode(mode: int):
-    """0 = automatic, 1 = transpose + column kernel, 2 = column kernel on the row-major field."""
+    """0 = automatic, 1 = transpose + column kernel, 2 = column kernel on the row-major field,
+    3 = four-step slab passes on the row-major field wherever the column length allows them."""
     check(load().irls2d_set_column_mode(int(mode)), "irls2d_set_column_mode")

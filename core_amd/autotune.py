@@ -233,11 +233,11 @@ def tune(shapes, device="cuda", reps=5, path=DEFAULT_FILE, save=True, verbose=Fa
         for n, p in chosen.items():
             p["ept"] = variants[best_v][n]
             _lib.plan_override(n, p["fft_len"], p["radices"], p["ept"])
-        # column strategy (transpose + column kernel vs column kernel on the row-major field):
-        # measured for the record; the library chooses it per batch by its Bluestein share
+        # column strategy (transpose + column kernel, column kernel on the row-major field, slab
+        # passes): measured for the record; the library chooses it per image
         column_ms = {}
         for name, mode in (("transpose", _lib.COLUMN_TRANSPOSE), ("direct", _lib.COLUMN_DIRECT),
-                           ("auto", _lib.COLUMN_AUTO)):
+                           ("slab", _lib.COLUMN_SLAB), ("auto", _lib.COLUMN_AUTO)):
             _lib.set_column_mode(mode)
             column_ms[name] = round(batch.time(reps), 4)
         after = batch.time(reps)

@@ -38,6 +38,13 @@ constexpr int LONG_MAX_BATCH = 32;
 // LDS of a long pass in float2: twiddles of the factor, then factor x (B + 1) values.
 constexpr int LONG_LDS = TW_LDS + LONG_BATCH_ELEMS + LONG_MAX_FACTOR;
 
+// Column pass on the row-major field (no transposes): slabs of SLAB_CP column pairs, the column
+// length H = N1 N2 transformed as a four-step FFT in three passes, each a batch of short FFTs in
+// LDS (slots x factor <= SLAB_E * threads, at most SLAB_MAX_THREADS threads).
+constexpr int SLAB_CP = 32;
+constexpr int SLAB_E = 16;
+constexpr int SLAB_MAX_THREADS = 512;
+
 // One Stockham pass, precomputed on the host so the device does no integer division.
 struct PassDesc {
   int R;        // radix
@@ -91,6 +98,18 @@ struct LongPlan {
   int pad;
 };
 
+// Column DCT pair of length N = N1 N2 on slabs of the row-major field (k_slab1/2/3).
+struct SlabPlan {
+  int N, N1, N2;
+  int f1, f2;    // DctPlan indices whose FftPlans transform lengths N1 and N2
+  int tw_off;    // split twiddle table of length N: exp(-2 pi i b / N), b < 64, then exp(-2 pi i 64 a / N)
+  int mk_off;    // N entries exp(-i pi k / 2N)
+  int lb1;       // log2 of the slots of a K1 / K3 block: 2^(lb1 - 5) values of n2 (or m2) x SLAB_CP pairs
+  int nb1;       // K1 / K3 blocks per slab
+  int nb2;       // K2 blocks per slab: one per pair {k1, N1 - k1}, k1 = 0 .. N1 / 2
+  float invN;
+};
+
 struct ImgDesc {
   long long off;   // element offset of the image in every field arena (multiple of 64)
   long long offT;  // element offset of the image in the transposed arena (multiple of 64)
@@ -104,6 +123,8 @@ struct ImgDesc {
   int collong;     // LongPlan index for length H when H > MAX_FFT_LEN, else -1
   int cosH_off;    // double offset of cos(pi i / H), i < H
   int cosW_off;    // double offset of cos(pi j / W), j < W
+  int colslab;     // SlabPlan index when the column pass runs on slabs (offT is then its scratch), else -1
+  int nslab;       // slabs of the image: ceil(W / (2 SLAB_CP))
 };
 
 // Per-image solver state.  Written by k_state_init and k_finalize (one block per image), read by
@@ -131,21 +152,23 @@ struct Fields {
   float *u, *u_prev, *u_best;   // solution (the caller's output arena), copies
   float *wxi, *wyi, *b;         // IRLS edge weights, right-hand side
   float *r, *z, *p0, *p1, *Ap;  // PCG vectors (p ping-pong)
-  float *T, *Tt;                // DCT row-pass output and its transpose
+  float *T, *Tt;                // DCT row-pass output and its transpose (or the slab scratch)
   unsigned char *valid;
 };
 
 struct Geo {  // block start of each image per launch geometry, [n + 1] each
   const int *tile, *rowpair, *colpair, *trans;
   const int *rowL1, *rowL2, *colL1, *colL2;  // long-line passes (images with W or H > MAX_FFT_LEN)
+  const int *slab1, *slab2;                  // slab column passes: K1 / K3, and K2
 };
-constexpr int NGEO = 8;
+constexpr int NGEO = 10;
 
 struct Ctx {
   const ImgDesc *img;
   ImgState *st;
   const DctPlan *plans;
   const LongPlan *lplans;
+  const SlabPlan *splans;
   const float2 *tab;
   const double *dtab;
   double *partials;
@@ -167,6 +190,11 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
   const int x = b % NUM_XCD, i = b / NUM_XCD;
   return x * q + min(x, r) + i;
 }
+
+// Row of the row-pass output field T that holds Makhoul position n of a slab-pass image: the
+// positions N2 n1 + n2 sit at row n2 N1 + n1, so that the strided subsequences the four-step
+// passes transform are runs of consecutive rows.
+__device__ __forceinline__ int slab_trow(int n, const SlabPlan &L) { return (n % L.N2) * L.N1 + n / L.N2; }
 
 __device__ __forceinline__ int find_img(const int *start, int n, int blk) {
   int lo = 0, hi = n;

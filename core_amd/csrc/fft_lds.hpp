@@ -34,6 +34,10 @@ __device__ __forceinline__ int pidx(int i) { return i + (i >> 4); }
 
 template <int R> struct Dft;
 
+template <> struct Dft<1> {
+  static __device__ __forceinline__ void run(float2 *) {}
+};
+
 template <> struct Dft<2> {
   static __device__ __forceinline__ void run(float2 *v) {
     float2 a = v[0], b = v[1];

@@ -5,6 +5,7 @@
 
 #include "kernels.hpp"
 #include "long_dct.hpp"
+#include "slab_dct.hpp"
 
 #define IRLS2D_FFT_KERNELS(E, X)                                                                  \
   X template __global__ void k_row_dct2<false, E>(Ctx, float *, float *, const float *, int);     \
@@ -20,3 +21,19 @@
   X template __global__ void k_long_p1<LONG_DCT3_SCALE, LONG_E>(Ctx, int, float *, const float *, int); \
   X template __global__ void k_long_p2<LONG_DCT2, LONG_E>(Ctx, int, float *, int, int);           \
   X template __global__ void k_long_p2<LONG_DCT3, LONG_E>(Ctx, int, float *, int, int);
+
+#define IRLS2D_SLAB_KERNELS(X)                                                                    \
+  X template __global__ void k_slab1<SLAB_E>(Ctx, const float *, int);                            \
+  X template __global__ void k_slab2<SLAB_E>(Ctx, int);                                           \
+  X template __global__ void k_slab3<SLAB_E>(Ctx, float *, int);                                 \
+  X template __global__ void k_slab1_f<16>(Ctx, const float *, int);                              \
+  X template __global__ void k_slab1_f<32>(Ctx, const float *, int);                              \
+  X template __global__ void k_slab1_f<64>(Ctx, const float *, int);                              \
+  X template __global__ void k_slab1_f<128>(Ctx, const float *, int);                             \
+  X template __global__ void k_slab3_f<16>(Ctx, float *, int);                                    \
+  X template __global__ void k_slab3_f<32>(Ctx, float *, int);                                    \
+  X template __global__ void k_slab3_f<64>(Ctx, float *, int);                                    \
+  X template __global__ void k_slab3_f<128>(Ctx, float *, int);                                   \
+  X template __global__ void k_slab2_f<16>(Ctx, int);                                             \
+  X template __global__ void k_slab2_f<32>(Ctx, int);                                             \
+  X template __global__ void k_slab2_f<64>(Ctx, int);

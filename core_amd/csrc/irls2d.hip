@@ -25,6 +25,7 @@ namespace irls2d {
 IRLS2D_FFT_KERNELS(16, extern)
 IRLS2D_FFT_KERNELS(24, extern)
 IRLS2D_LONG_KERNELS(extern)
+IRLS2D_SLAB_KERNELS(extern)
 }  // namespace irls2d
 
 namespace irls2d {
@@ -105,8 +106,10 @@ struct PlanOverride {
 };
 static std::map<int, PlanOverride> g_over;
 // Column pass strategy: 0 = automatic, 1 = transpose + column kernel, 2 = column kernel on the
-// row-major field (irls2d_set_column_mode).
+// row-major field, 3 = four-step slab passes on the row-major field (irls2d_set_column_mode).
 static int g_col_mode = 0;
+// Automatic mode: slab passes for 5-smooth column lengths from this length on.
+static int g_slab_min = 256;
 
 // FFT lengths with a compile-time Stockham schedule (fft_lds_ct): 2^a * {1, 3, 5}, 1024 .. 12288.
 static bool has_ct(int64_t M) {
@@ -291,6 +294,88 @@ static std::shared_ptr<HostLong> get_long_plan(int N) {
   return h;
 }
 
+// Split of a 5-smooth column length for the slab passes: N = N1 N2 such that a K2 block (2 SLAB_CP
+// slots of length N2) and a K1 block (at least SLAB_CP slots of length N1) fit SLAB_E values per
+// thread; the split with the smallest larger block, then the larger N2.
+static bool slab_split(int64_t N, int &N1, int &N2) {
+  if (N < 4 || N > MAX_FFT_LEN || !smooth235(N)) return false;
+  const int64_t cap = (int64_t)SLAB_E * SLAB_MAX_THREADS;
+  int64_t best = -1, bestN2 = 0;
+  for (int64_t b = 2; b <= N / 2; ++b) {
+    if (N % b) continue;
+    const int64_t a = N / b;
+    const int64_t foot = std::max<int64_t>(2 * SLAB_CP * b, SLAB_CP * a);
+    if (foot > cap) continue;
+    if (best < 0 || foot < best || (foot == best && b > bestN2)) {
+      best = foot;
+      bestN2 = b;
+    }
+  }
+  if (best < 0) return false;
+  N2 = (int)bestN2;
+  N1 = (int)(N / bestN2);
+  return true;
+}
+
+// Factors with register-stage kernels (k_slab1_f / k_slab2_f / k_slab3_f).
+static bool slab_fast(int N1, int N2) {
+  return (N1 == 16 || N1 == 32 || N1 == 64 || N1 == 128) && (N2 == 16 || N2 == 32 || N2 == 64);
+}
+
+struct HostSlab {
+  SlabPlan plan;  // tw_off / mk_off relative to `tab`; f1, f2 set per layout
+  std::vector<float2> tab;
+  int threads;    // workgroup size the plan's blocks need
+};
+static std::map<int, std::shared_ptr<HostSlab>> g_splans;
+
+static std::shared_ptr<HostSlab> get_slab_plan(int N) {
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  auto it = g_splans.find(N);
+  if (it != g_splans.end()) return it->second;
+  int N1 = 0, N2 = 0;
+  std::shared_ptr<HostSlab> h;
+  if (slab_split(N, N1, N2)) {
+    h = std::make_shared<HostSlab>();
+    SlabPlan &P = h->plan;
+    std::memset(&P, 0, sizeof(P));
+    P.N = N;
+    P.N1 = N1;
+    P.N2 = N2;
+    P.invN = (float)(1.0 / (double)N);
+    P.tw_off = 0;
+    for (int b = 0; b < TW_LO; ++b) {
+      const double ang = -2.0 * M_PI * (double)b / (double)N;
+      h->tab.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+    }
+    for (int a = 0; a < (N + 63) / 64; ++a) {
+      const double ang = -2.0 * M_PI * (double)(64 * (int64_t)a) / (double)N;
+      h->tab.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+    }
+    P.mk_off = (int)h->tab.size();
+    for (int k = 0; k < N; ++k) {
+      const double ang = -M_PI * (double)k / (2.0 * (double)N);
+      h->tab.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
+    }
+    const int foot = std::max(2 * SLAB_CP * N2, SLAB_CP * N1);
+    h->threads = std::max(64, ((foot + SLAB_E - 1) / SLAB_E + 63) / 64 * 64);
+    int lg = 0;
+    if (slab_fast(N1, N2)) {
+      // the register-stage kernels: 8 / (N1 / 16) values of n2 per K1 / K3 block of 256 threads
+      h->threads = std::max(h->threads, 256);
+      while ((1 << lg) < 128 / N1) ++lg;
+    } else {
+      // K1 / K3: as many n2 values per block as fill those threads, at most N2
+      while ((2 << lg) <= N2 && (int64_t)SLAB_CP * (2 << lg) * N1 <= (int64_t)SLAB_E * h->threads) ++lg;
+    }
+    P.lb1 = 5 + lg;
+    P.nb1 = (N2 + (1 << lg) - 1) >> lg;
+    P.nb2 = N1 / 2 + 1;
+  }
+  g_splans[N] = h;
+  return h;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Batch layout
 // ---------------------------------------------------------------------------------------------
@@ -309,19 +394,22 @@ struct Layout {
   int64_t fstride = 0;  // elements between consecutive workspace fields
   // metadata (host images of the device region)
   std::vector<ImgDesc> desc;
-  std::vector<int> geo[NGEO];  // tile, rowpair, colpair, trans, rowL1, rowL2, colL1, colL2
+  std::vector<int> geo[NGEO];  // tile, rowpair, colpair, trans, rowL1, rowL2, colL1, colL2, slab1, slab2
   std::vector<DctPlan> plans;
   std::vector<LongPlan> lplans;
+  std::vector<SlabPlan> splans;
   std::vector<float2> tab;
   std::vector<double> dtab;
   int max_blocks = 0, max_blocks2 = 0;
   int64_t totalS = 0;  // float2 elements of the long-transform scratch arena
   int row_threads = 64, col_threads = 64, row_E = 16, col_E = 16;
-  size_t row_lds = 0, col_lds = 0, long_lds = 0;
+  size_t row_lds = 0, col_lds = 0, long_lds = 0, slab_lds = 0;
+  int slab_threads = 64;
+  int slab_f1 = 0, slab_f2 = 0;  // N1, N2 of the register-stage kernels when every slab image uses them
   bool col_direct = false;  // column pass straight on the row-major field (no transposes)
   // byte offsets inside the workspace
   size_t o_fields = 0, o_Tt = 0, o_S = 0, o_valid = 0, o_meta = 0, o_desc = 0, o_state = 0, o_geo[NGEO] = {},
-         o_plans = 0, o_lplans = 0, o_tab = 0, o_dtab = 0, o_partials = 0, o_partials2 = 0, bytes = 0,
+         o_plans = 0, o_lplans = 0, o_splans = 0, o_tab = 0, o_dtab = 0, o_partials = 0, o_partials2 = 0, bytes = 0,
          meta_bytes = 0;
   uint64_t sig = 0;
 };
@@ -369,7 +457,6 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
   Lo.offT.resize(n);
   Lo.total = arena_layout(n, H, W, Lo.off.data(), Lo.pitch.data());
   if (Lo.total < 0) return fail(IRLS2D_EINVAL, "every image needs H >= 1 and W >= 1");
-  Lo.totalT = arena_layout(n, W, H, Lo.offT.data(), nullptr);  // transposed images
   for (int i = 0; i < n; ++i) {
     if (H[i] > (int64_t)1 << 30 || W[i] > (int64_t)1 << 30) return fail(IRLS2D_EINVAL, "dimension too large");
     if (!supported(H[i]) || !supported(W[i]))
@@ -414,6 +501,33 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     Lo.long_lds = std::max(Lo.long_lds, std::max(l1, l2));
     return idx;
   };
+  // slab column plan of length N, or -1 when N has no slab split or its factors' plans are not
+  // plain FFTs of their own length (a Bluestein override)
+  std::map<int, int> slab_idx;
+  auto add_slab = [&](int N) -> int {
+    auto it = slab_idx.find(N);
+    if (it != slab_idx.end()) return it->second;
+    auto h = get_slab_plan(N);
+    int idx = -1;
+    if (h) {
+      SlabPlan P = h->plan;
+      P.f1 = add_plan(P.N1);
+      P.f2 = add_plan(P.N2);
+      if (!Lo.plans[P.f1].blue && !Lo.plans[P.f2].blue) {
+        const int base = (int)Lo.tab.size();
+        Lo.tab.insert(Lo.tab.end(), h->tab.begin(), h->tab.end());
+        P.tw_off += base;
+        P.mk_off += base;
+        Lo.splans.push_back(P);
+        idx = (int)Lo.splans.size() - 1;
+        const int e1 = P.N1 * ((1 << P.lb1) + 1), e2 = P.N2 * (2 * SLAB_CP + 1);
+        Lo.slab_lds = std::max(Lo.slab_lds, (size_t)(TW_LDS + std::max(e1, e2)) * sizeof(float2));
+        Lo.slab_threads = std::max(Lo.slab_threads, h->threads);
+      }
+    }
+    slab_idx[N] = idx;
+    return idx;
+  };
   auto add_cos = [&](int N) -> int {
     auto it = cos_off.find(N);
     if (it != cos_off.end()) return it->second;
@@ -428,7 +542,6 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     const int h = (int)H[i], w = (int)W[i];
     ImgDesc d;
     d.off = Lo.off[i];
-    d.offT = Lo.offT[i];
     d.H = h;
     d.W = w;
     d.P = (int)Lo.pitch[i];
@@ -440,6 +553,14 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     d.collong = lh ? add_long(h) : -1;
     d.cosH_off = add_cos(h);
     d.cosW_off = add_cos(w);
+    const bool want_slab = !lh && !lw && (g_col_mode == 3 || (g_col_mode == 0 && h >= g_slab_min));
+    d.colslab = want_slab ? add_slab(h) : -1;
+    d.nslab = (w + 2 * SLAB_CP - 1) / (2 * SLAB_CP);
+    const bool slab = d.colslab >= 0;
+    // transposed copy (W x PT), or the slab scratch (nslab x H x SLAB_CP float2 = H x P floats)
+    Lo.offT[i] = Lo.totalT;
+    d.offT = Lo.totalT;
+    Lo.totalT += slab ? (int64_t)h * d.P : (int64_t)w * d.PT;
     int64_t sz = 0;
     int64_t lr1 = 0, lr2 = 0, lc1 = 0, lc2 = 0;
     if (lw) {
@@ -461,13 +582,20 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
       maxM_row = std::max(maxM_row, Lo.plans[d.rowplan].M);
       ept_row = std::max(ept_row, Lo.plans[d.rowplan].ept);
     }
-    if (!lh) {
+    if (!lh && !slab) {
       maxM_col = std::max(maxM_col, Lo.plans[d.colplan].M);
       ept_col = std::max(ept_col, Lo.plans[d.colplan].ept);
     }
     const int64_t tiles = ((int64_t)(h + TILE_H - 1) / TILE_H) * ((w + TILE_W - 1) / TILE_W);
-    const int64_t ttiles = ((int64_t)(h + TT - 1) / TT) * ((w + TT - 1) / TT);
-    const int64_t geo_add[NGEO] = {tiles, lw ? 0 : (h + 1) / 2, lh ? 0 : (w + 1) / 2, ttiles, lr1, lr2, lc1, lc2};
+    const int64_t ttiles = slab ? 0 : ((int64_t)(h + TT - 1) / TT) * ((w + TT - 1) / TT);
+    int64_t ls1 = 0, ls2 = 0;
+    if (slab) {
+      const SlabPlan &P = Lo.splans[d.colslab];
+      ls1 = (int64_t)P.nb1 * d.nslab;
+      ls2 = (int64_t)P.nb2 * d.nslab;
+    }
+    const int64_t geo_add[NGEO] = {tiles, lw ? 0 : (h + 1) / 2, (lh || slab) ? 0 : (w + 1) / 2, ttiles,
+                                   lr1, lr2, lc1, lc2, ls1, ls2};
     for (int g = 0; g < NGEO; ++g) {
       const int64_t v = (int64_t)Lo.geo[g].back() + geo_add[g];
       if (v > INT32_MAX) return fail(IRLS2D_EUNSUPPORTED, "too many blocks in one launch");
@@ -475,7 +603,21 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     }
   }
   for (int g = 0; g < 4; ++g) Lo.max_blocks = std::max(Lo.max_blocks, Lo.geo[g].back());
-  for (int g = 4; g < NGEO; ++g) Lo.max_blocks2 = std::max(Lo.max_blocks2, Lo.geo[g].back());
+  for (int g = 4; g < 8; ++g) Lo.max_blocks2 = std::max(Lo.max_blocks2, Lo.geo[g].back());
+  {
+    // register-stage slab kernels when every slab image has the same power-of-two split
+    int f1 = -1, f2 = -1;
+    for (const SlabPlan &P : Lo.splans) {
+      if (f1 < 0) { f1 = P.N1; f2 = P.N2; }
+      else if (P.N1 != f1 || P.N2 != f2) f1 = 0;
+    }
+    if (f1 > 0 && slab_fast(f1, f2)) {
+      Lo.slab_f1 = f1;
+      Lo.slab_f2 = f2;
+      // K2: exchange buffer, then tables of 2 N2 float2 twiddles and 2 N2 doubles
+      Lo.slab_lds = (size_t)(TW_LDS + std::max(f1 * SLAB_CP * (128 / f1), f2 * 2 * SLAB_CP + 4 * f2)) * sizeof(float2);
+    }
+  }
   {
     // The transposes cost 8 B/px each; reading column pairs straight from the row-major field
     // (float2 per row, L2-shared between neighbouring blocks) costs more than that when the column
@@ -483,12 +625,13 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     // (measured on MI355X: -16% per PCG iteration for the 48-burst stack, +16% at 8192^2).
     double blue_px = 0, all_px = 0;
     for (int i = 0; i < n; ++i) {
+      if (Lo.desc[i].colslab >= 0 || Lo.desc[i].colplan < 0) continue;  // no column kernel
       const double px = (double)H[i] * (double)W[i];
       all_px += px;
       if (Lo.desc[i].colplan >= 0 && Lo.plans[Lo.desc[i].colplan].blue) blue_px += px;
     }
-    const bool auto_direct = blue_px >= 0.5 * all_px;
-    Lo.col_direct = Lo.geo[6].back() == 0 && (g_col_mode == 2 || (g_col_mode == 0 && auto_direct));
+    const bool auto_direct = all_px > 0 && blue_px >= 0.5 * all_px;
+    Lo.col_direct = Lo.geo[6].back() == 0 && (g_col_mode == 2 || ((g_col_mode == 0 || g_col_mode == 3) && auto_direct));
   }
   fft_shape(maxM_row, ept_row, Lo.row_E, Lo.row_threads);
   fft_shape(maxM_col, ept_col, Lo.col_E, Lo.col_threads);
@@ -522,6 +665,8 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
   o = align_up(o + Lo.plans.size() * sizeof(DctPlan), 256);
   Lo.o_lplans = o;
   o = align_up(o + Lo.lplans.size() * sizeof(LongPlan), 256);
+  Lo.o_splans = o;
+  o = align_up(o + Lo.splans.size() * sizeof(SlabPlan), 256);
   Lo.o_tab = o;
   o = align_up(o + Lo.tab.size() * sizeof(float2), 256);
   Lo.o_dtab = o;
@@ -540,6 +685,7 @@ static std::vector<unsigned char> pack_meta(const Layout &Lo) {
   for (int g = 0; g < NGEO; ++g) put(Lo.o_geo[g], Lo.geo[g].data(), Lo.geo[g].size() * sizeof(int));
   put(Lo.o_plans, Lo.plans.data(), Lo.plans.size() * sizeof(DctPlan));
   put(Lo.o_lplans, Lo.lplans.data(), Lo.lplans.size() * sizeof(LongPlan));
+  put(Lo.o_splans, Lo.splans.data(), Lo.splans.size() * sizeof(SlabPlan));
   put(Lo.o_tab, Lo.tab.data(), Lo.tab.size() * sizeof(float2));
   put(Lo.o_dtab, Lo.dtab.data(), Lo.dtab.size() * sizeof(double));
   return buf;
@@ -601,6 +747,7 @@ static Ctx make_ctx(const Layout &Lo, unsigned char *ws, float *u, const Params 
   c.st = (ImgState *)(ws + Lo.o_state);
   c.plans = (const DctPlan *)(ws + Lo.o_plans);
   c.lplans = (const LongPlan *)(ws + Lo.o_lplans);
+  c.splans = (const SlabPlan *)(ws + Lo.o_splans);
   c.partials2 = (double *)(ws + Lo.o_partials2);
   c.S = (float2 *)(ws + Lo.o_S);
   c.tab = (const float2 *)(ws + Lo.o_tab);
@@ -614,6 +761,8 @@ static Ctx make_ctx(const Layout &Lo, unsigned char *ws, float *u, const Params 
   c.geo.rowL2 = (const int *)(ws + Lo.o_geo[5]);
   c.geo.colL1 = (const int *)(ws + Lo.o_geo[6]);
   c.geo.colL2 = (const int *)(ws + Lo.o_geo[7]);
+  c.geo.slab1 = (const int *)(ws + Lo.o_geo[8]);
+  c.geo.slab2 = (const int *)(ws + Lo.o_geo[9]);
   float *F = (float *)(ws + Lo.o_fields);
   float *fld[NFIELD];
   for (int k = 0; k < NFIELD; ++k) fld[k] = F + (size_t)k * Lo.fstride;
@@ -700,6 +849,36 @@ static void row_dct3(const Layout &Lo, const Ctx &c, const float *src, float *ds
 }
 
 static void col_pass(const Layout &Lo, const Ctx &c, int guard, hipStream_t s) {
+  if (Lo.geo[8].back() > 0) {
+    const dim3 g1(Lo.geo[8].back()), g2(Lo.geo[9].back());
+    const size_t l = Lo.slab_lds;
+    const float *T = c.f.T;
+    if (Lo.slab_f1) {
+      const dim3 b1(256), b2(4 * Lo.slab_f2);
+      switch (Lo.slab_f1) {
+        case 16: LAUNCH("k_slab1_f", k_slab1_f<16>, g1, b1, l, s, c, T, guard); break;
+        case 32: LAUNCH("k_slab1_f", k_slab1_f<32>, g1, b1, l, s, c, T, guard); break;
+        case 64: LAUNCH("k_slab1_f", k_slab1_f<64>, g1, b1, l, s, c, T, guard); break;
+        default: LAUNCH("k_slab1_f", k_slab1_f<128>, g1, b1, l, s, c, T, guard); break;
+      }
+      switch (Lo.slab_f2) {
+        case 16: LAUNCH("k_slab2_f", k_slab2_f<16>, g2, b2, l, s, c, guard); break;
+        case 32: LAUNCH("k_slab2_f", k_slab2_f<32>, g2, b2, l, s, c, guard); break;
+        default: LAUNCH("k_slab2_f", k_slab2_f<64>, g2, b2, l, s, c, guard); break;
+      }
+      switch (Lo.slab_f1) {
+        case 16: LAUNCH("k_slab3_f", k_slab3_f<16>, g1, b1, l, s, c, c.f.T, guard); break;
+        case 32: LAUNCH("k_slab3_f", k_slab3_f<32>, g1, b1, l, s, c, c.f.T, guard); break;
+        case 64: LAUNCH("k_slab3_f", k_slab3_f<64>, g1, b1, l, s, c, c.f.T, guard); break;
+        default: LAUNCH("k_slab3_f", k_slab3_f<128>, g1, b1, l, s, c, c.f.T, guard); break;
+      }
+    } else {
+      const dim3 b(Lo.slab_threads);
+      LAUNCH("k_slab1", (k_slab1<SLAB_E>), g1, b, l, s, c, T, guard);
+      LAUNCH("k_slab2", (k_slab2<SLAB_E>), g2, b, l, s, c, guard);
+      LAUNCH("k_slab3", (k_slab3<SLAB_E>), g1, b, l, s, c, c.f.T, guard);
+    }
+  }
   if (Lo.geo[6].back() > 0) {
     long_pass1(Lo, c, 1, LONG_DCT2, c.f.Tt, nullptr, guard, s);
     long_pass2(Lo, c, 1, LONG_DCT2, c.f.Tt, 0, guard, s);
@@ -718,7 +897,7 @@ static void col_pass(const Layout &Lo, const Ctx &c, int guard, hipStream_t s) {
 }
 
 static void transpose(const Layout &Lo, const Ctx &c, int dir, int guard, hipStream_t s) {
-  if (Lo.col_direct) return;
+  if (Lo.col_direct || Lo.geo[3].back() == 0) return;
   const float *src = dir == 0 ? c.f.T : c.f.Tt;
   float *dst = dir == 0 ? c.f.Tt : c.f.T;
   LAUNCH("k_transpose", k_transpose, dim3(Lo.geo[3].back()), dim3(256), 0, s, c, src, dst, dir, guard);
@@ -848,6 +1027,9 @@ static int set_attrs() {
                         (const void *)k_long_p1<LONG_DCT3, LONG_E>, (const void *)k_long_p1<LONG_DCT3_SCALE, LONG_E>,
                         (const void *)k_long_p2<LONG_DCT2, LONG_E>, (const void *)k_long_p2<LONG_DCT3, LONG_E>};
   for (const void *f : lfns) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, llds));
+  const int slds = (TW_LDS + SLAB_E * SLAB_MAX_THREADS + 2 * SLAB_MAX_THREADS) * (int)sizeof(float2);
+  const void *sfns[] = {(const void *)k_slab1<SLAB_E>, (const void *)k_slab2<SLAB_E>, (const void *)k_slab3<SLAB_E>};
+  for (const void *f : sfns) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, slds));
   if (dev >= 0 && dev < 64) g_attr_done[dev] = true;
   return IRLS2D_OK;
 }
@@ -1081,7 +1263,8 @@ int plan_describe(int64_t N, int64_t *fft_len, int *rad, int *nrad, int *ept) {
 
 int set_column_mode(int mode) {
   g_err.clear();
-  if (mode < 0 || mode > 2) return fail(IRLS2D_EINVAL, "column mode must be 0 (auto), 1 (transpose) or 2 (direct)");
+  if (mode < 0 || mode > 3)
+    return fail(IRLS2D_EINVAL, "column mode must be 0 (auto), 1 (transpose), 2 (direct) or 3 (slab)");
   {
     std::lock_guard<std::mutex> lk(g_plan_mu);
     g_col_mode = mode;
@@ -1270,6 +1453,20 @@ int irls2d_plan_describe(int64_t n, int64_t *fft_len, int *radices, int *nradice
 int irls2d_plan_reset(void) { return irls2d::plan_reset(); }
 
 int irls2d_set_column_mode(int mode) { return irls2d::set_column_mode(mode); }
+
+int irls2d_describe_columns(int n, const int64_t *H, const int64_t *W, int32_t *strategy) {
+  if (!strategy) return irls2d::fail(IRLS2D_EINVAL, "null strategy pointer");
+  std::shared_ptr<irls2d::Layout> Lo;
+  int rc = irls2d::get_layout(n, H, W, Lo);
+  if (rc) return rc;
+  for (int i = 0; i < n; ++i) {
+    const irls2d::ImgDesc &d = Lo->desc[i];
+    if (d.colslab >= 0) strategy[i] = Lo->slab_f1 ? IRLS2D_COLUMNS_SLAB_REGISTER : IRLS2D_COLUMNS_SLAB;
+    else if (d.collong >= 0) strategy[i] = IRLS2D_COLUMNS_LONG;
+    else strategy[i] = Lo->col_direct ? IRLS2D_COLUMNS_DIRECT : IRLS2D_COLUMNS_TRANSPOSE;
+  }
+  return IRLS2D_OK;
+}
 
 int irls2d_precondition(int n, const int64_t *H, const int64_t *W, const float *r, float *z, void *workspace,
                         size_t workspace_bytes, void *stream) {

@@ -402,6 +402,20 @@ __global__ __launch_bounds__(TILE_THREADS) void k_cg_pap(Ctx c, const float *__r
 // (utils_unwrap2d.py:1444-1458) to the two rows it transforms.
 // ---------------------------------------------------------------------------------------------
 
+// Element offsets in T of the row-pass output of image rows ra and ra + 1: the rows themselves,
+// or for an image whose column pass runs on slabs, the rows of their Makhoul positions in the
+// slab order (slab_trow).
+__device__ __forceinline__ void t_rows(const Ctx &c, const ImgDesc &d, int ra, long long &ta, long long &tb) {
+  if (d.colslab < 0) {
+    ta = d.off + (long long)ra * d.P;
+    tb = ta + d.P;
+    return;
+  }
+  const SlabPlan &L = c.splans[d.colslab];
+  ta = d.off + (long long)slab_trow(makhoul_pos(ra, d.H), L) * d.P;
+  tb = ra + 1 < d.H ? d.off + (long long)slab_trow(makhoul_pos(ra + 1, d.H), L) * d.P : ta;
+}
+
 template <bool FUSE_UPDATE, int E>
 __global__ __launch_bounds__(MAX_FFT_THREADS) void k_row_dct2(Ctx c, float *in_, float *__restrict__ out,
                                                                const float *__restrict__ p, int guard) {
@@ -414,6 +428,8 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) void k_row_dct2(Ctx c, float *in_,
   const bool hasb = ra + 1 < d.H;
   const int W = d.W;
   const long long oa = d.off + (long long)ra * d.P;
+  long long ta, tb;
+  t_rows(c, d, ra, ta, tb);
   const DctCtx D = dct_begin(lds, c.plans + d.rowplan, c.tab);
   double acc[2] = {0.0, 0.0};  // r.r, non-finite x
   if (FUSE_UPDATE) {
@@ -470,11 +486,11 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) void k_row_dct2(Ctx c, float *in_,
     const int nk = k ? W - k : 0;
     float Xak, Xbk, Xan, Xbn;
     dct2_coef_pair(D, k, nk, Xak, Xbk, Xan, Xbn);
-    out[oa + k] = Xak;
-    if (hasb) out[oa + d.P + k] = Xbk;
+    out[ta + k] = Xak;
+    if (hasb) out[tb + k] = Xbk;
     if (nk != k) {
-      out[oa + nk] = Xan;
-      if (hasb) out[oa + d.P + nk] = Xbn;
+      out[ta + nk] = Xan;
+      if (hasb) out[tb + nk] = Xbn;
     }
   }
   if (FUSE_UPDATE) block_partial<2>(acc, c.partials);
@@ -614,11 +630,13 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) void k_row_dct3(Ctx c, const float
   const bool hasb = ra + 1 < d.H;
   const int W = d.W, T = blockDim.x;
   const long long oa = d.off + (long long)ra * d.P;
+  long long ta, tb;
+  t_rows(c, d, ra, ta, tb);
   const DctCtx D = dct_begin(lds, c.plans + d.rowplan, c.tab);
   for (int k = threadIdx.x; k <= W / 2; k += T) {
     const int nk = k ? W - k : 0;
-    const float Xak = T_[oa + k], Xan = T_[oa + nk];
-    const float Xbk = hasb ? T_[oa + d.P + k] : 0.f, Xbn = hasb ? T_[oa + d.P + nk] : 0.f;
+    const float Xak = T_[ta + k], Xan = T_[ta + nk];
+    const float Xbk = hasb ? T_[tb + k] : 0.f, Xbn = hasb ? T_[tb + nk] : 0.f;
     dct3_put_pair(D, k, nk, Xak, Xbk, Xan, Xbn);
   }
   dct3_run<E>(D);

@@ -1,0 +1,488 @@
+// Column pass of the preconditioner straight on the row-major field, without transposes (gfx950).
+//
+// Columns (ja, ja + 1) form one complex Makhoul sequence of length N = H (v[n] = x[2n] for 2n < N,
+// v[N - 1 - n] = x[2n + 1]), as in the LDS-resident column transform; an image's columns are taken
+// in slabs of SLAB_CP such pairs, so that every row access is a 256-byte run.  The column DCT-II,
+// the division by -lambda and the DCT-III are two forward FFTs of length N = N1 N2, written as
+// four-step FFTs and split over three global passes through a scratch copy S:
+//   K1 (k_slab1): n = N2 n1 + n2.  For each n2, the length-N1 DFT over n1 of the packed input,
+//       times exp(-2 pi i n2 k1 / N), stored at S row k1 N2 + n2.
+//   K2 (k_slab2): for each k1, the length-N2 DFT over n2 gives the spectrum Z at k = k1 + N1 k2.
+//       A block holds k1 and N1 - k1, so Z_k and Z_{N-k} are both at hand: DCT-II coefficients,
+//       division by -lambda, DCT-III pre-step c_k.  Then the first step of the second FFT, with
+//       k = k1 + N1 k2 and m = m2 + N2 m1: the length-N2 DFT over k2, times exp(-2 pi i k1 m2 / N),
+//       stored back at S row k1 N2 + m2.
+//   K3 (k_slab3): for each m2, the length-N1 DFT over k1 gives the DCT-III sequence at
+//       n = m2 + N2 m1.
+// The row passes keep T of such an image in slab order: Makhoul position n = N2 n1 + n2 at row
+// n2 N1 + n1 (slab_trow), so K1 reads and K3 writes runs of consecutive rows.  S is slab-major
+// ([slab][row][pair], float2), so K2 reads and writes N2 consecutive rows as one contiguous run.
+// Two kernel families: for power-of-two factors (N1 = 16 .. 128, N2 = 16 .. 64) each short FFT
+// of length 16 R is two register stages with one LDS exchange between them (reg_stage1/2, all
+// strides and radices compile-time); for the other factors the short FFTs run batched in LDS,
+// interleaved over the slots of the block (batch_fft of the long-line transform).
+#pragma once
+
+#include "long_dct.hpp"
+
+namespace irls2d {
+
+struct SlabBlk {
+  const ImgDesc *d;
+  const SlabPlan *L;
+  int img, grp, slab;
+  float2 *S;  // scratch of this slab
+};
+
+__device__ __forceinline__ bool slab_block(const Ctx &c, const int *geo, int guard, SlabBlk &b) {
+  b.img = find_img(geo, c.n, blockIdx.x);
+  b.d = c.img + b.img;
+  if (guard_off(&c.st[b.img], guard)) return false;
+  b.L = c.splans + b.d->colslab;
+  const int local = blockIdx.x - geo[b.img];
+  const int ns = b.d->nslab;
+  b.grp = local / ns;
+  b.slab = local - b.grp * ns;
+  b.S = (float2 *)(c.f.Tt + b.d->offT) + (long long)b.slab * b.L->N * SLAB_CP;
+  return true;
+}
+
+constexpr int SLAB_U = 4;  // global loads in flight per thread before their LDS stores
+
+// K1: field T (row DCT-II output) -> S.
+template <int E>
+__global__ __launch_bounds__(SLAB_MAX_THREADS) void k_slab1(Ctx c, const float *__restrict__ src, int guard) {
+  extern __shared__ float2 lds[];
+  SlabBlk b;
+  if (!slab_block(c, c.geo.slab1, guard, b)) return;
+  const SlabPlan &L = *b.L;
+  const int N1 = L.N1, N2 = L.N2, lb = L.lb1, B = 1 << lb, BS = B + 1;
+  const int W = b.d->W;
+  const long long P = b.d->P;
+  const FftPlan *F = &c.plans[L.f1].fft;
+  float2 *tws = lds, *buf = lds + TW_LDS;
+  load_tws(tws, c.tab, F);
+  const float *base = src + b.d->off;
+  const int T = blockDim.x, tot = N1 << lb;
+  const int n2_0 = b.grp << (lb - 5);
+  const int jslab = 2 * b.slab * SLAB_CP;
+  for (int e0 = threadIdx.x; e0 < tot; e0 += SLAB_U * T) {
+    float2 v[SLAB_U];
+#pragma unroll
+    for (int u = 0; u < SLAB_U; ++u) {
+      const int e = e0 + u * T;
+      v[u] = make_float2(0.f, 0.f);
+      if (e < tot) {
+        const int s = e & (B - 1), n1 = e >> lb;
+        const int n2 = n2_0 + (s >> 5), ja = jslab + 2 * (s & (SLAB_CP - 1));
+        if (n2 < N2 && ja < W) {
+          const float *p = base + (long long)(n2 * N1 + n1) * P + ja;  // slab_trow(N2 n1 + n2)
+          v[u] = ja + 1 < W ? *(const float2 *)p : make_float2(p[0], 0.f);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SLAB_U; ++u) {
+      const int e = e0 + u * T;
+      if (e < tot) buf[(e >> lb) * BS + (e & (B - 1))] = v[u];
+    }
+  }
+  __syncthreads();
+  batch_fft<E>(buf, BS, lb, tws, F);
+  const float2 *tw = c.tab + L.tw_off;
+  for (int e = threadIdx.x; e < tot; e += T) {
+    const int s = e & (B - 1), k1 = e >> lb;
+    const int n2 = n2_0 + (s >> 5), cp = s & (SLAB_CP - 1);
+    if (n2 >= N2) continue;
+    float2 v = buf[k1 * BS + s];
+    if (k1 && n2) v = cmul(v, long_twiddle(tw, k1 * n2));
+    b.S[(k1 * N2 + n2) * SLAB_CP + cp] = v;
+  }
+}
+
+// K2: S -> S in place (the rows of k1 and N1 - k1).
+template <int E>
+__global__ __launch_bounds__(SLAB_MAX_THREADS) void k_slab2(Ctx c, int guard) {
+  extern __shared__ float2 lds[];
+  SlabBlk b;
+  if (!slab_block(c, c.geo.slab2, guard, b)) return;
+  const SlabPlan &L = *b.L;
+  const int N = L.N, N1 = L.N1, N2 = L.N2;
+  constexpr int lb = 6, B = 2 * SLAB_CP, BS = B + 1;  // slots: set (k1a, then k1b) x pair
+  const int k1a = b.grp, k1b = b.grp ? N1 - b.grp : 0;
+  const bool two = k1b != k1a;
+  const int W = b.d->W;
+  const FftPlan *F = &c.plans[L.f2].fft;
+  float2 *tws = lds, *buf = lds + TW_LDS;
+  load_tws(tws, c.tab, F);
+  const int T = blockDim.x, tot = N2 << lb;
+  const float2 *Sa = b.S + (long long)k1a * N2 * SLAB_CP;
+  const float2 *Sb = b.S + (long long)k1b * N2 * SLAB_CP;
+  for (int e0 = threadIdx.x; e0 < tot; e0 += SLAB_U * T) {
+    float2 v[SLAB_U];
+#pragma unroll
+    for (int u = 0; u < SLAB_U; ++u) {
+      const int e = e0 + u * T;
+      v[u] = make_float2(0.f, 0.f);
+      if (e < tot) {
+        const int s = e & (B - 1), n2 = e >> lb, cp = s & (SLAB_CP - 1);
+        if (s < SLAB_CP) v[u] = Sa[n2 * SLAB_CP + cp];
+        else if (two) v[u] = Sb[n2 * SLAB_CP + cp];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SLAB_U; ++u) {
+      const int e = e0 + u * T;
+      if (e < tot) buf[(e >> lb) * BS + (e & (B - 1))] = v[u];
+    }
+  }
+  __syncthreads();
+  batch_fft<E>(buf, BS, lb, tws, F);
+  // Pairs (k, N - k): with two sets, (k1a, k2) and (k1b, N2 - 1 - k2); in the single set of
+  // k1 = 0, (k2, -k2 mod N2); in that of k1 = N1 / 2, (k2, N2 - 1 - k2).
+  const int npairs = two ? N2 : (k1a == 0 ? N2 / 2 + 1 : (N2 + 1) / 2);
+  const float2 *mk = c.tab + L.mk_off;
+  const double *cosH = c.dtab + b.d->cosH_off;
+  for (int e = threadIdx.x; e < (npairs << 5); e += T) {
+    const int cp = e & (SLAB_CP - 1), k2 = e >> 5;
+    const int k2p = (two || k1a) ? N2 - 1 - k2 : (k2 ? N2 - k2 : 0);
+    const int sp = two ? SLAB_CP + cp : cp;
+    const int k = k1a + N1 * k2;
+    const int nk = k ? N - k : 0;
+    const int ja = 2 * (b.slab * SLAB_CP + cp);
+    float Xak = 0.f, Xbk = 0.f, Xan = 0.f, Xbn = 0.f;
+    const float2 wk = mk[k], wn = mk[nk];
+    if (ja < W) {
+      dct2_coef_w(buf[k2 * BS + cp], buf[k2p * BS + sp], wk, wn, Xak, Xbk, Xan, Xbn);
+      // X / -lambda, lambda = 2cos(pi k/H) + 2cos(pi j/W) - 4 rounded to float from float64
+      // (utils_unwrap2d.py:1284-1289, :1395-1398); the DC coefficient becomes 0
+      const bool hasb = ja + 1 < W;
+      const double cwa = 2.0 * c.dtab[b.d->cosW_off + ja];
+      const double cwb = hasb ? 2.0 * c.dtab[b.d->cosW_off + ja + 1] : 0.0;
+      const double ck = 2.0 * cosH[k], cn = 2.0 * cosH[nk];
+      Xak = (k == 0 && ja == 0) ? 0.f : -(Xak * __builtin_amdgcn_rcpf((float)(ck + cwa - 4.0)));
+      Xbk = hasb ? -(Xbk * __builtin_amdgcn_rcpf((float)(ck + cwb - 4.0))) : 0.f;
+      Xan = (nk == 0 && ja == 0) ? 0.f : -(Xan * __builtin_amdgcn_rcpf((float)(cn + cwa - 4.0)));
+      Xbn = hasb ? -(Xbn * __builtin_amdgcn_rcpf((float)(cn + cwb - 4.0))) : 0.f;
+    }
+    buf[k2 * BS + cp] = dct3_pre_w(wk, Xak, Xbk, k ? Xan : 0.f, k ? Xbn : 0.f);
+    if (nk != k) buf[k2p * BS + sp] = dct3_pre_w(wn, Xan, Xbn, Xak, Xbk);
+  }
+  __syncthreads();
+  batch_fft<E>(buf, BS, lb, tws, F);
+  const float2 *tw = c.tab + L.tw_off;
+  float2 *Da = b.S + (long long)k1a * N2 * SLAB_CP;
+  float2 *Db = b.S + (long long)k1b * N2 * SLAB_CP;
+  for (int e = threadIdx.x; e < tot; e += T) {
+    const int s = e & (B - 1), m2 = e >> lb, cp = s & (SLAB_CP - 1);
+    const bool setb = s >= SLAB_CP;
+    if (setb && !two) continue;
+    const int k1 = setb ? k1b : k1a;
+    float2 v = buf[m2 * BS + s];
+    if (k1 && m2) v = cmul(v, long_twiddle(tw, k1 * m2));
+    (setb ? Db : Da)[m2 * SLAB_CP + cp] = v;
+  }
+}
+
+// K3: S -> field (the column pass output, read by the row DCT-III).
+template <int E>
+__global__ __launch_bounds__(SLAB_MAX_THREADS) void k_slab3(Ctx c, float *__restrict__ dst, int guard) {
+  extern __shared__ float2 lds[];
+  SlabBlk b;
+  if (!slab_block(c, c.geo.slab1, guard, b)) return;
+  const SlabPlan &L = *b.L;
+  const int N1 = L.N1, N2 = L.N2, lb = L.lb1, B = 1 << lb, BS = B + 1;
+  const int W = b.d->W;
+  const long long P = b.d->P;
+  const FftPlan *F = &c.plans[L.f1].fft;
+  float2 *tws = lds, *buf = lds + TW_LDS;
+  load_tws(tws, c.tab, F);
+  const int T = blockDim.x, tot = N1 << lb;
+  const int m2_0 = b.grp << (lb - 5);
+  for (int e0 = threadIdx.x; e0 < tot; e0 += SLAB_U * T) {
+    float2 v[SLAB_U];
+#pragma unroll
+    for (int u = 0; u < SLAB_U; ++u) {
+      const int e = e0 + u * T;
+      v[u] = make_float2(0.f, 0.f);
+      if (e < tot) {
+        const int s = e & (B - 1), k1 = e >> lb;
+        const int m2 = m2_0 + (s >> 5);
+        if (m2 < N2) v[u] = b.S[(k1 * N2 + m2) * SLAB_CP + (s & (SLAB_CP - 1))];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SLAB_U; ++u) {
+      const int e = e0 + u * T;
+      if (e < tot) buf[(e >> lb) * BS + (e & (B - 1))] = v[u];
+    }
+  }
+  __syncthreads();
+  batch_fft<E>(buf, BS, lb, tws, F);
+  float *base = dst + b.d->off;
+  const int jslab = 2 * b.slab * SLAB_CP;
+  const float invN = L.invN;
+  for (int e = threadIdx.x; e < tot; e += T) {
+    const int s = e & (B - 1), m1 = e >> lb;
+    const int m2 = m2_0 + (s >> 5), ja = jslab + 2 * (s & (SLAB_CP - 1));
+    if (m2 >= N2 || ja >= W) continue;
+    const float2 y = buf[m1 * BS + s];
+    float *p = base + (long long)(m2 * N1 + m1) * P + ja;  // slab_trow(m2 + N2 m1)
+    if (ja + 1 < W) *(float2 *)p = make_float2(y.x * invN, -y.y * invN);
+    else p[0] = y.x * invN;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Power-of-two factors: two-stage register FFTs
+// ---------------------------------------------------------------------------------------------
+
+// Split twiddle table of length N (exp(-2 pi i b / N), b < 64, then exp(-2 pi i 64 a / N)) into LDS.
+__device__ __forceinline__ void load_tws_len(float2 *tws, const float2 *__restrict__ tab, int N) {
+  const int n = TW_LO + ((N + 63) >> 6);
+  for (int i = threadIdx.x; i < n; i += blockDim.x) tws[i] = tab[i];
+}
+
+// Length L = 16 R FFT, element n = t + R j of a sequence held by thread t < R of its slot s as
+// v[j].  Stage 1: DFT-16 over j, times exp(-2 pi i t k1 / L) (from the length-N table, index
+// step tstep = N / L), to LDS at ((k1 R + t) SB + s).  Needs a barrier before stage 2.
+template <int L>
+__device__ __forceinline__ void reg_stage1(float2 (&v)[16], int t, const float2 *tws, int tstep, float2 *buf,
+                                           int SB, int s) {
+  constexpr int R = L / 16;
+  Dft<16>::run(v);
+  if (R == 1) return;
+  if (t) twiddle_inputs<16>(v, tws, t * tstep);
+#pragma unroll
+  for (int k1 = 0; k1 < 16; ++k1) buf[(k1 * R + t) * SB + s] = v[k1];
+}
+
+// Stage 2: thread q < R of slot s: DFT-R over t of the stage-1 values of k1 = q Q + i (Q = 16 / R);
+// o[i R + k2] = X[q Q + i + 16 k2].
+template <int L>
+__device__ __forceinline__ void reg_stage2(float2 (&o)[16], const float2 (&v)[16], int q, const float2 *buf,
+                                           int SB, int s) {
+  constexpr int R = L / 16, Q = 16 / R;
+  if (R == 1) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) o[i] = v[i];
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    float2 w[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) w[t] = buf[((q * Q + i) * R + t) * SB + s];
+    Dft<R>::run(w);
+#pragma unroll
+    for (int k2 = 0; k2 < R; ++k2) o[i * R + k2] = w[k2];
+  }
+}
+
+// K1 for N1 = 16 R: 256 threads, G = 8 / R values of n2 x SLAB_CP pairs per block.
+template <int N1>
+__global__ __launch_bounds__(256) void k_slab1_f(Ctx c, const float *__restrict__ src, int guard) {
+  constexpr int R = N1 / 16, Q = 16 / R, G = 8 / R, SB = SLAB_CP * G;
+  extern __shared__ float2 lds[];
+  SlabBlk b;
+  if (!slab_block(c, c.geo.slab1, guard, b)) return;
+  const SlabPlan &L = *b.L;
+  const int N = L.N, N2 = L.N2, W = b.d->W;
+  const long long P = b.d->P;
+  float2 *tws = lds, *buf = lds + TW_LDS;
+  load_tws_len(tws, c.tab + L.tw_off, N);
+  const int s = threadIdx.x & (SB - 1), t = threadIdx.x / SB;
+  const int cp = s & (SLAB_CP - 1);
+  const int n2 = b.grp * G + (s >> 5);
+  const int ja = 2 * (b.slab * SLAB_CP + cp);
+  float2 v[16];
+  if (n2 < N2 && ja < W) {
+    const float *p = src + b.d->off + (long long)(n2 * N1 + t) * P + ja;  // slab_trow(N2 n1 + n2), n1 = t + R j
+    const bool two = ja + 1 < W;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const float *q = p + (long long)(R * j) * P;
+      v[j] = two ? *(const float2 *)q : make_float2(q[0], 0.f);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  reg_stage1<N1>(v, t, tws, N / N1, buf, SB, s);
+  __syncthreads();
+  float2 o[16];
+  reg_stage2<N1>(o, v, t, buf, SB, s);
+  if (n2 >= N2) return;
+  float2 *S = b.S + n2 * SLAB_CP + cp;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+#pragma unroll
+    for (int k2 = 0; k2 < R; ++k2) {
+      const int k1 = t * Q + i + 16 * k2;
+      float2 y = o[i * R + k2];
+      if (k1 && n2) y = cmul(y, twiddle(tws, k1 * n2));
+      S[(long long)k1 * N2 * SLAB_CP] = y;
+    }
+  }
+}
+
+// K3 for N1 = 16 R.
+template <int N1>
+__global__ __launch_bounds__(256) void k_slab3_f(Ctx c, float *__restrict__ dst, int guard) {
+  constexpr int R = N1 / 16, Q = 16 / R, G = 8 / R, SB = SLAB_CP * G;
+  extern __shared__ float2 lds[];
+  SlabBlk b;
+  if (!slab_block(c, c.geo.slab1, guard, b)) return;
+  const SlabPlan &L = *b.L;
+  const int N = L.N, N2 = L.N2, W = b.d->W;
+  const long long P = b.d->P;
+  float2 *tws = lds, *buf = lds + TW_LDS;
+  load_tws_len(tws, c.tab + L.tw_off, N);
+  const int s = threadIdx.x & (SB - 1), t = threadIdx.x / SB;
+  const int cp = s & (SLAB_CP - 1);
+  const int m2 = b.grp * G + (s >> 5);
+  const int ja = 2 * (b.slab * SLAB_CP + cp);
+  float2 v[16];
+  if (m2 < N2) {
+    const float2 *S = b.S + (long long)(t * N2 + m2) * SLAB_CP + cp;  // k1 = t + R j
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = S[(long long)(R * j) * N2 * SLAB_CP];
+  } else {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  reg_stage1<N1>(v, t, tws, N / N1, buf, SB, s);
+  __syncthreads();
+  float2 o[16];
+  reg_stage2<N1>(o, v, t, buf, SB, s);
+  if (m2 >= N2 || ja >= W) return;
+  const float invN = L.invN;
+  float *p = dst + b.d->off + (long long)(m2 * N1) * P + ja;  // slab_trow(m2 + N2 m1) = m2 N1 + m1
+  const bool two = ja + 1 < W;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+#pragma unroll
+    for (int k2 = 0; k2 < R; ++k2) {
+      const int m1 = t * Q + i + 16 * k2;
+      const float2 y = o[i * R + k2];
+      float *q = p + (long long)m1 * P;
+      if (two) *(float2 *)q = make_float2(y.x * invN, -y.y * invN);
+      else q[0] = y.x * invN;
+    }
+  }
+}
+
+// K2 for N2 = 16 R: 64 R threads, slots = set (k1a, k1b) x SLAB_CP pairs.
+template <int N2>
+__global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
+  constexpr int R = N2 / 16, Q = 16 / R, SB = 2 * SLAB_CP;
+  extern __shared__ float2 lds[];
+  SlabBlk b;
+  if (!slab_block(c, c.geo.slab2, guard, b)) return;
+  const SlabPlan &L = *b.L;
+  const int N = L.N, N1 = L.N1, W = b.d->W;
+  float2 *tws = lds, *buf = lds + TW_LDS;
+  load_tws_len(tws, c.tab + L.tw_off, N);
+  const int s = threadIdx.x & (SB - 1), t = threadIdx.x / SB;
+  const int cp = s & (SLAB_CP - 1), sig = s >> 5;
+  const int k1a = b.grp, k1b = b.grp ? N1 - b.grp : 0;
+  const bool two = k1a != k1b, live = sig == 0 || two;
+  const int k1 = sig ? k1b : k1a;
+  float2 *Srow = b.S + (long long)k1 * N2 * SLAB_CP + cp;
+  float2 v[16];
+  if (live) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = Srow[(t + R * j) * SLAB_CP];
+  } else {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = make_float2(0.f, 0.f);
+  }
+  // per-block tables at k = k1 + N1 kk of both sets: exp(-i pi k / 2N) and 2 cos(pi k / H)
+  float2 *mkT = buf + N2 * SB;
+  double *chT = (double *)(mkT + 2 * N2);
+  {
+    const float2 *mk = c.tab + L.mk_off;
+    const double *cosH = c.dtab + b.d->cosH_off;
+    for (int e = threadIdx.x; e < 2 * N2; e += blockDim.x) {
+      const int k = (e < N2 ? k1a : k1b) + N1 * (e & (N2 - 1));
+      mkT[e] = mk[k];
+      chT[e] = 2.0 * cosH[k];
+    }
+  }
+  __syncthreads();
+  const int tstep = N / N2;
+  reg_stage1<N2>(v, t, tws, tstep, buf, SB, s);
+  __syncthreads();
+  float2 o[16];
+  reg_stage2<N2>(o, v, t, buf, SB, s);
+  // spectrum Z at k = k1 + N1 kk, kk = t Q + i + 16 k2, in natural order for the partner reads
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < Q; ++i)
+#pragma unroll
+    for (int k2 = 0; k2 < R; ++k2) buf[(t * Q + i + 16 * k2) * SB + s] = o[i * R + k2];
+  __syncthreads();
+  // DCT-II coefficients at k and N - k (partner: the other set at N2 - 1 - kk; in the single set
+  // of k1 = 0 at -kk mod N2, of k1 = N1 / 2 at N2 - 1 - kk), / -lambda, DCT-III pre-step at k
+  const int ja = 2 * (b.slab * SLAB_CP + cp);
+  const bool hasa = live && ja < W, hasb = hasa && ja + 1 < W;
+  const double cwa = hasa ? 2.0 * c.dtab[b.d->cosW_off + ja] : 0.0;
+  const double cwb = hasb ? 2.0 * c.dtab[b.d->cosW_off + ja + 1] : 0.0;
+  const int sp = two ? (s ^ SLAB_CP) : s;
+  const int ts = sig ? N2 : 0, tp = (sp >> 5) ? N2 : 0;  // table rows of the own and partner set
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+#pragma unroll
+    for (int k2 = 0; k2 < R; ++k2) {
+      const int kk = t * Q + i + 16 * k2;
+      const int kp = (two || k1a) ? N2 - 1 - kk : (kk ? N2 - kk : 0);
+      const int k = k1 + N1 * kk;
+      const int nk = k ? N - k : 0;  // = partner set's k1 + N1 kp
+      float2 cz = make_float2(0.f, 0.f);
+      if (hasa) {
+        float Xak, Xbk, Xan, Xbn;
+        const float2 wk = mkT[ts + kk];
+        dct2_coef_w(o[i * R + k2], buf[kp * SB + sp], wk, mkT[tp + kp], Xak, Xbk, Xan, Xbn);
+        const double ck = chT[ts + kk], cn = chT[tp + kp];
+        Xak = (k == 0 && ja == 0) ? 0.f : -(Xak * __builtin_amdgcn_rcpf((float)(ck + cwa - 4.0)));
+        Xbk = hasb ? -(Xbk * __builtin_amdgcn_rcpf((float)(ck + cwb - 4.0))) : 0.f;
+        Xan = (nk == 0 && ja == 0) ? 0.f : -(Xan * __builtin_amdgcn_rcpf((float)(cn + cwa - 4.0)));
+        Xbn = hasb ? -(Xbn * __builtin_amdgcn_rcpf((float)(cn + cwb - 4.0))) : 0.f;
+        cz = dct3_pre_w(wk, Xak, Xbk, k ? Xan : 0.f, k ? Xbn : 0.f);
+      }
+      o[i * R + k2] = cz;
+    }
+  }
+  // second FFT over kk = a + 16 b (a = t Q + i): DFT-R over b, times exp(-2 pi i a m' / N2),
+  // exchange, DFT-16 over a: X[m' + R m''] with m' = t
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    float2 w[R];
+#pragma unroll
+    for (int bb = 0; bb < R; ++bb) w[bb] = o[i * R + bb];
+    Dft<R>::run(w);
+    const int a = t * Q + i;
+#pragma unroll
+    for (int m = 0; m < R; ++m) {
+      const float2 y = (a && m) ? cmul(w[m], twiddle(tws, a * m * tstep)) : w[m];
+      buf[(a * R + m) * SB + s] = y;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 16; ++a) v[a] = buf[(a * R + t) * SB + s];
+  Dft<16>::run(v);
+  if (!live) return;
+#pragma unroll
+  for (int mm = 0; mm < 16; ++mm) {
+    const int m = t + R * mm;
+    float2 y = v[mm];
+    if (k1 && m) y = cmul(y, twiddle(tws, k1 * m));
+    Srow[m * SLAB_CP] = y;
+  }
+}
+
+}  // namespace irls2d

@@ -14,6 +14,9 @@ goes through ONE batched device solve (`irls_unwrap_2d_batch`), with the same pe
   rioxarray is present, the input's spatial chunks when it was dask-backed).  xarray is an
   optional dependency: the functions import it on use (or take the module as `xr`).  Unlike the
   reference they compute eagerly: each burst variable is one device call over all its pairs.
+* `unwrap_chunked` / `unwrap2d_chunk`: the per-chunk unwrapping with overlap margins of
+  Stack_unwrap2d.py:992-1136 (dask map_overlap, boundary 'none'), with every tile of every pair
+  in one batched device solve.
 * `filter_small_components`: the first step of the reference's component linking
   (Stack_unwrap2d.py:343-375), which NaNs out labelled components below max(conncomp_size, 4).
   The integer-program linking of the remaining components (:377-397, utils_unwrap2d.py:195-444)
@@ -29,7 +32,8 @@ import numpy as np
 
 from .unwrap2d import irls_unwrap_2d_batch
 
-__all__ = ["unwrap_stack", "filter_small_components", "unwrap2d_irls", "unwrap2d", "unwrap2d_dataset"]
+__all__ = ["unwrap_stack", "filter_small_components", "unwrap2d_irls", "unwrap2d", "unwrap2d_dataset",
+           "chunk_windows", "unwrap_chunked", "unwrap2d_chunk"]
 
 
 def unwrap_stack(phase, weight=None, device='auto', max_iter=50, tol=1e-2, cg_max_iter=10, cg_tol=1e-3,
@@ -210,3 +214,111 @@ def unwrap2d_dataset(phase, weight=None, conncomp=False, conncomp_size=1000, con
     if conncomp:
         return res[0][''], res[1]['']
     return res['']
+
+
+def _depth(ov, chunk):
+    """Stack_unwrap2d.py:1084-1086: fractions of the first chunk or pixels, at least 1."""
+    if ov is None:
+        ov = 0
+    return max(1, int(ov * chunk) if isinstance(ov, float) else int(ov))
+
+
+def _chunk_bounds(n, c):
+    """Regular chunks of size c along a dimension of length n (the last one shorter)."""
+    edges = list(range(0, n, c)) + [n]
+    return list(zip(edges[:-1], edges[1:]))
+
+
+def chunk_windows(H, W, chunks, overlap=None):
+    """Tiles of a dask map_overlap with boundary 'none' (Stack_unwrap2d.py:1081-1111).
+
+    chunks = (cy, cx) (the spatial chunk sizes, as dask's first chunk of each axis); overlap as
+    the reference's argument (None, a pixel int, a chunk fraction float, or a per-axis tuple).
+    Returns [(core window, extended window)] with windows as (y0, y1, x0, x1): a chunk grows by
+    the depth towards every neighbour chunk and not beyond the array; the unwrapped extended
+    tile is trimmed back to the core."""
+    cy, cx = int(chunks[0]), int(chunks[1])
+    ov_y, ov_x = overlap if isinstance(overlap, tuple) else (overlap, overlap)
+    dy, dx = _depth(ov_y, cy), _depth(ov_x, cx)
+    out = []
+    for y0, y1 in _chunk_bounds(H, cy):
+        for x0, x1 in _chunk_bounds(W, cx):
+            ey0 = y0 - dy if y0 > 0 else y0
+            ey1 = y1 + dy if y1 < H else y1
+            ex0 = x0 - dx if x0 > 0 else x0
+            ex1 = x1 + dx if x1 < W else x1
+            out.append(((y0, y1, x0, x1), (max(0, ey0), min(H, ey1), max(0, ex0), min(W, ex1))))
+    return out
+
+
+def unwrap_chunked(phase, weight=None, chunks=None, overlap=None, device='auto', max_iter=50, tol=1e-2,
+                   cg_max_iter=10, cg_tol=1e-3, epsilon=1e-2, conncomp_size=30, debug=False):
+    """Per-chunk unwrapping with overlap of a (pair, y, x) or (y, x) array.
+
+    Same tiles and per-tile calls as Stack_unwrap2d.unwrap2d_chunk (each extended tile of each
+    pair unwrapped independently, all-NaN tiles passed through, the margins trimmed), with all
+    tiles of all pairs solved in one batched device call.  chunks = (cy, cx); None = one chunk.
+    Returns the unwrapped float32 array (no conncomp, as the reference)."""
+    ph = np.asarray(phase)
+    if ph.ndim not in (2, 3):
+        raise ValueError(f"Expected 2D or 3D phase, got shape {ph.shape}")
+    squeeze = ph.ndim == 2
+    ph3 = ph[None] if squeeze else ph
+    w3 = None if weight is None else (np.asarray(weight)[None] if squeeze else np.asarray(weight))
+    H, W = ph3.shape[1:]
+    wins = chunk_windows(H, W, chunks if chunks is not None else (H, W), overlap)
+    tiles, wt, where = [], [], []
+    for p in range(ph3.shape[0]):
+        for core, ext in wins:
+            ey0, ey1, ex0, ex1 = ext
+            tiles.append(ph3[p, ey0:ey1, ex0:ex1])
+            wt.append(None if w3 is None else w3[p, ey0:ey1, ex0:ex1])
+            where.append((p, core, ext))
+    if debug:
+        print(f'unwrap_chunked: {len(tiles)} tiles of {len(wins)} windows per pair in one batched solve')
+    res = irls_unwrap_2d_batch(tiles, None if w3 is None else wt, device=device, max_iter=max_iter, tol=tol,
+                               cg_max_iter=cg_max_iter, cg_tol=cg_tol, epsilon=epsilon,
+                               conncomp_size=conncomp_size, debug=debug)
+    out = np.empty(ph3.shape, np.float32)
+    for (p, core, ext), (u, _) in zip(where, res):
+        y0, y1, x0, x1 = core
+        ey0, _, ex0, _ = ext
+        out[p, y0:y1, x0:x1] = u[y0 - ey0:y1 - ey0, x0 - ex0:x1 - ex0]
+    return out[0] if squeeze else out
+
+
+def unwrap2d_chunk(phase, weight=None, overlap=None, device='auto', semaphore=8, debug=False, chunks=None,
+                   xr=None, **kwargs):
+    """Stack_unwrap2d.unwrap2d_chunk (:992-1136) over a burst mapping {key: xr.Dataset}.
+
+    The spatial chunks are the DataArray's dask chunks when it has them, else `chunks`
+    ((cy, cx); None = one chunk).  kwargs: max_iter (50), tol (1e-2), cg_max_iter (10), cg_tol
+    (1e-3), epsilon (1e-2), conncomp_size (30), the reference's defaults (:1063-1069).  Returns
+    {key: Dataset} of unwrapped float32 phase (units 'radians', attrs and CRS kept)."""
+    xr = _xr(xr)
+    params = dict(max_iter=kwargs.get('max_iter', 50), tol=kwargs.get('tol', 1e-2),
+                  cg_max_iter=kwargs.get('cg_max_iter', 10), cg_tol=kwargs.get('cg_tol', 1e-3),
+                  epsilon=kwargs.get('epsilon', 1e-2), conncomp_size=kwargs.get('conncomp_size', 30))
+    result = {}
+    for key in phase.keys():
+        ds = phase[key]
+        wds = weight[key] if weight is not None and key in weight else None
+        data_vars = [v for v in ds.data_vars if 'y' in ds[v].dims and 'x' in ds[v].dims]
+        uvars = {}
+        for var in data_vars:
+            pda = ds[var]
+            wda = wds[var] if wds is not None else None
+            dch = getattr(getattr(pda, 'data', None), 'chunks', None)
+            ch = (dch[-2][0], dch[-1][0]) if isinstance(dch, tuple) and len(dch) >= 2 else chunks
+            arr = np.asarray(pda.values, dtype=np.float32)
+            warr = None if wda is None else np.asarray(wda.values, dtype=np.float32)
+            unw = unwrap_chunked(arr, warr, chunks=ch, overlap=overlap, device=device, debug=debug, **params)
+            uda = xr.DataArray(unw, dims=pda.dims, coords=pda.coords)
+            uda.attrs['units'] = 'radians'
+            uvars[var] = uda
+        result[key] = xr.Dataset(uvars, attrs=dict(ds.attrs))
+        rio = getattr(ds, 'rio', None)
+        crs = getattr(rio, 'crs', None) if rio is not None else None
+        if crs is not None:
+            result[key].rio.write_crs(crs, inplace=True)
+    return result

@@ -138,13 +138,28 @@ int irls2d_plan_reset(void);
 
 /*
  * Column-pass strategy (no reference counterpart: the reference transposes with torch).
- * 0 = automatic (default): the column DCTs read column pairs straight from the row-major field
- * when at least half of the batch's pixels have Bluestein column transforms (whose arithmetic
- * hides the strided access), otherwise the field is transposed, transformed and transposed back;
- * 1 = always transpose; 2 = always direct (columns longer than 12288 always transpose).  Drops
- * cached layouts and hipGraphs.  core_amd.autotune measures both for a batch.
+ * 0 = automatic (default): images whose column length is 2^a 3^b 5^c between 256 and 12288 run
+ * the three four-step slab passes on the row-major field (no transposes); for the others the
+ * column DCTs read column pairs straight from the row-major field when at least half of their
+ * pixels have Bluestein column transforms (whose arithmetic hides the strided access), otherwise
+ * the field is transposed, transformed and transposed back;
+ * 1 = always transpose; 2 = always direct; 3 = slab passes for every image whose column length
+ * allows them (2^a 3^b 5^c, 4 .. 12288), the automatic choice for the rest.  Columns longer than
+ * 12288 always transpose.  Drops cached layouts and hipGraphs.  core_amd.autotune measures the
+ * strategies for a batch.
  */
 int irls2d_set_column_mode(int mode);
+
+/*
+ * Column strategy each image of a batch of shapes gets under the current mode and plans (no
+ * reference counterpart): strategy[i] is one of the IRLS2D_COLUMNS_* values below.
+ */
+#define IRLS2D_COLUMNS_TRANSPOSE 0      /* transpose, LDS-resident column DCTs, transpose back */
+#define IRLS2D_COLUMNS_DIRECT 1         /* LDS-resident column DCTs on the row-major field */
+#define IRLS2D_COLUMNS_LONG 2           /* transpose, two-pass four-step DCTs (H > 12288) */
+#define IRLS2D_COLUMNS_SLAB 3           /* three four-step slab passes, batched LDS FFTs */
+#define IRLS2D_COLUMNS_SLAB_REGISTER 4  /* three four-step slab passes, register-stage FFTs */
+int irls2d_describe_columns(int n, const int64_t *H, const int64_t *W, int32_t *strategy);
 
 /*
  * Device epilogue of one solved image (replaces the host part of utils_unwrap2d.py:1659-1746: the

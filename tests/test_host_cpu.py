@@ -196,9 +196,37 @@ def test_autotune_candidates_are_valid(lib):
         lib.plan_reset()
 
 
+def test_column_strategies(lib):
+    """Which column pass each image gets: slab passes for 2^a 3^b 5^c heights from 256 on (the
+    register-stage kernels when every slab image has the same power-of-two split), the column
+    kernel on the row-major field for Bluestein heights, the four-step long path above 12288."""
+    d = lib.describe_columns
+    try:
+        lib.set_column_mode(lib.COLUMN_AUTO)
+        assert d([(8192, 8192)]) == ["slab_register"]
+        assert d([(8192, 70), (8192, 3)]) == ["slab_register"] * 2
+        assert d([(256, 256)]) == ["slab_register"]
+        assert d([(8192, 70), (1000, 7)]) == ["slab", "slab"]     # two splits: batched LDS FFTs
+        assert d([(96, 130)]) == ["transpose"]                     # below the slab threshold
+        assert d([(127, 211)]) == ["direct"]                       # Bluestein columns
+        assert d([(12800, 8)]) == ["long"]
+        assert d([(1000, 12500)]) == ["transpose"]                 # long rows keep the row-major T
+        assert d([(8192, 70), (127, 211), (12800, 8)]) == ["slab_register", "transpose", "long"]
+        lib.set_column_mode(lib.COLUMN_SLAB)
+        assert d([(64, 48), (96, 130)]) == ["slab", "slab"]
+        assert d([(127, 211)]) == ["direct"]                       # no slab split: automatic choice
+        lib.set_column_mode(lib.COLUMN_TRANSPOSE)
+        assert d([(8192, 8192)]) == ["transpose"]
+    finally:
+        lib.set_column_mode(lib.COLUMN_AUTO)
+
+
 def test_column_mode_validation(lib):
     with pytest.raises(ValueError):
-        lib.set_column_mode(3)
+        lib.set_column_mode(4)
+    with pytest.raises(ValueError):
+        lib.set_column_mode(-1)
+    lib.set_column_mode(lib.COLUMN_SLAB)
     lib.set_column_mode(lib.COLUMN_DIRECT)
     lib.set_column_mode(lib.COLUMN_AUTO)
 

@@ -163,3 +163,42 @@ def test_unwrap2d_errors_and_modes(stub_solver):
         stack.unwrap2d_dataset({"not": "a dataset"}, xr=FakeXr)
     res = stack.unwrap2d_dataset(ph, co, conncomp=True, conncomp_size=4, xr=FakeXr)
     assert isinstance(res, tuple) and res[0]["VV"].dims == ("pair", "y", "x")
+
+
+def test_chunk_windows_follow_map_overlap_boundary_none():
+    # 10 x 9 grid, chunks (4, 4): rows [0,4) [4,8) [8,10), cols [0,4) [4,8) [8,9)
+    wins = stack.chunk_windows(10, 9, (4, 4), overlap=2)
+    assert len(wins) == 9
+    core, ext = wins[0]
+    assert core == (0, 4, 0, 4) and ext == (0, 6, 0, 6)        # no margin towards the array edge
+    core, ext = wins[4]
+    assert core == (4, 8, 4, 8) and ext == (2, 10, 2, 9)        # clipped to the array
+    # every pixel belongs to exactly one core window
+    cover = np.zeros((10, 9), int)
+    for (y0, y1, x0, x1), _ in wins:
+        cover[y0:y1, x0:x1] += 1
+    assert (cover == 1).all()
+    # overlap=None still overlaps by 1 pixel; float overlap is a fraction of the chunk
+    assert stack.chunk_windows(10, 9, (4, 4), None)[4][1] == (3, 9, 3, 9)
+    assert stack.chunk_windows(10, 9, (4, 4), 0.5)[4][1] == (2, 10, 2, 9)
+    assert stack.chunk_windows(10, 9, (4, 4), (1, 3))[4][1] == (3, 9, 1, 9)
+
+
+def test_unwrap_chunked_assembles_trimmed_tiles(monkeypatch):
+    """With the solver replaced by a stub that tags each tile, every output pixel comes from the
+    core of its own chunk's extended tile at the right offset."""
+    calls = []
+
+    def fake_batch(tiles, weights=None, **kw):
+        calls.append(kw)
+        return [(t + 1000.0 * i, np.zeros(t.shape, np.uint16)) for i, t in enumerate(tiles)]
+
+    monkeypatch.setattr(stack, "irls_unwrap_2d_batch", fake_batch)
+    ph = np.arange(2 * 10 * 9, dtype=np.float32).reshape(2, 10, 9)
+    out = stack.unwrap_chunked(ph, chunks=(4, 4), overlap=2)
+    wins = stack.chunk_windows(10, 9, (4, 4), 2)
+    for p in range(2):
+        for t, (core, _) in enumerate(wins):
+            y0, y1, x0, x1 = core
+            np.testing.assert_array_equal(out[p, y0:y1, x0:x1], ph[p, y0:y1, x0:x1] + 1000.0 * (p * len(wins) + t))
+    assert len(calls) == 1 and (calls[0]["max_iter"], calls[0]["tol"], calls[0]["cg_max_iter"]) == (50, 1e-2, 10)
