@@ -41,6 +41,14 @@ ALG_BPX = {
     "k_col": 8,               # Tt -> Tt
     "k_row_dct3": 12,         # T, r -> z
     "k_transpose": 8,         # T -> Tt / Tt -> T
+    # column pass on slabs of the row-major field: T (4 B/px real) <-> S (one complex per pixel
+    # pair, 4 B/px)
+    "k_slab1_f": 8,           # T -> S
+    "k_slab2_f": 8,           # S -> S
+    "k_slab3_f": 8,           # S -> T
+    "k_slab1": 8,
+    "k_slab2": 8,
+    "k_slab3": 8,
 }
 # The same for the two-pass transforms of sides above 12288 (the 40000 x 40000 workload); the
 # scratch line holds one complex value per pixel pair, i.e. 4 B per pixel each way.
