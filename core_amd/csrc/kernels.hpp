@@ -719,10 +719,24 @@ __global__ __launch_bounds__(256) void k_finalize(Ctx c, int kind, int it) {
   if (kind == FIN_IRLS_UPDATE || kind == FIN_UPDATE) nv = 2;
   if (kind == FIN_IRLS_CHECK) nv = 3;
   double acc[MAX_NV] = {0.0, 0.0, 0.0};
-  for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+  // partials b0 + tid, + 256, + 512, ... summed in that order; loaded FIN_U at a time so that
+  // their L2 latencies overlap
+  constexpr int FIN_U = 16;
+  for (int b = b0 + threadIdx.x; b < b1; b += FIN_U * blockDim.x) {
+    double v[FIN_U][MAX_NV];
 #pragma unroll
-    for (int q = 0; q < MAX_NV; ++q)
-      if (q < nv) acc[q] += parts[(long long)b * MAX_NV + q];
+    for (int u = 0; u < FIN_U; ++u) {
+      const int bb = b + u * blockDim.x;
+#pragma unroll
+      for (int q = 0; q < MAX_NV; ++q) v[u][q] = (bb < b1 && q < nv) ? parts[(long long)bb * MAX_NV + q] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < FIN_U; ++u) {
+      if (b + u * (int)blockDim.x >= b1) break;
+#pragma unroll
+      for (int q = 0; q < MAX_NV; ++q)
+        if (q < nv) acc[q] += v[u][q];
+    }
   }
   block_sum<MAX_NV>(acc);
   if (threadIdx.x != 0) return;
