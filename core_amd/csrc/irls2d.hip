@@ -557,7 +557,7 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     d.colslab = want_slab ? add_slab(h) : -1;
     d.nslab = (w + 2 * SLAB_CP - 1) / (2 * SLAB_CP);
     const bool slab = d.colslab >= 0;
-    // transposed copy (W x PT), or the slab scratch (nslab x H x SLAB_CP float2 = H x P floats)
+    // transposed copy (W x PT), or the slab scratch (H rows of nslab x SLAB_CP float2 = P floats)
     Lo.offT[i] = Lo.totalT;
     d.offT = Lo.totalT;
     Lo.totalT += slab ? (int64_t)h * d.P : (int64_t)w * d.PT;

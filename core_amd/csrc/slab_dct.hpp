@@ -15,8 +15,9 @@
 //   K3 (k_slab3): for each m2, the length-N1 DFT over k1 gives the DCT-III sequence at
 //       n = m2 + N2 m1.
 // The row passes keep T of such an image in slab order: Makhoul position n = N2 n1 + n2 at row
-// n2 N1 + n1 (slab_trow), so K1 reads and K3 writes runs of consecutive rows.  S is slab-major
-// ([slab][row][pair], float2), so K2 reads and writes N2 consecutive rows as one contiguous run.
+// n2 N1 + n1 (slab_trow), so K1 reads and K3 writes runs of consecutive rows.  S is row-major
+// with the pitch of T (row q holds the pairs of every slab), so the blocks of neighbouring slabs,
+// which run together, write and read whole rows between them.
 // Two kernel families: for power-of-two factors (N1 = 16 .. 128, N2 = 16 .. 64) each short FFT
 // of length 16 R is two register stages with one LDS exchange between them (reg_stage1/2, all
 // strides and radices compile-time); for the other factors the short FFTs run batched in LDS,
@@ -31,7 +32,8 @@ struct SlabBlk {
   const ImgDesc *d;
   const SlabPlan *L;
   int img, grp, slab;
-  float2 *S;  // scratch of this slab
+  float2 *S;     // scratch row 0 of this slab
+  long long ss;  // float2 between consecutive scratch rows (nslab x SLAB_CP: pitch P floats)
 };
 
 __device__ __forceinline__ bool slab_block(const Ctx &c, const int *geo, int guard, SlabBlk &b) {
@@ -43,7 +45,8 @@ __device__ __forceinline__ bool slab_block(const Ctx &c, const int *geo, int gua
   const int ns = b.d->nslab;
   b.grp = local / ns;
   b.slab = local - b.grp * ns;
-  b.S = (float2 *)(c.f.Tt + b.d->offT) + (long long)b.slab * b.L->N * SLAB_CP;
+  b.S = (float2 *)(c.f.Tt + b.d->offT) + (long long)b.slab * SLAB_CP;
+  b.ss = (long long)ns * SLAB_CP;
   return true;
 }
 
@@ -96,7 +99,7 @@ __global__ __launch_bounds__(SLAB_MAX_THREADS) void k_slab1(Ctx c, const float *
     if (n2 >= N2) continue;
     float2 v = buf[k1 * BS + s];
     if (k1 && n2) v = cmul(v, long_twiddle(tw, k1 * n2));
-    b.S[(k1 * N2 + n2) * SLAB_CP + cp] = v;
+    b.S[(k1 * N2 + n2) * b.ss + cp] = v;
   }
 }
 
@@ -116,8 +119,8 @@ __global__ __launch_bounds__(SLAB_MAX_THREADS) void k_slab2(Ctx c, int guard) {
   float2 *tws = lds, *buf = lds + TW_LDS;
   load_tws(tws, c.tab, F);
   const int T = blockDim.x, tot = N2 << lb;
-  const float2 *Sa = b.S + (long long)k1a * N2 * SLAB_CP;
-  const float2 *Sb = b.S + (long long)k1b * N2 * SLAB_CP;
+  const float2 *Sa = b.S + (long long)k1a * N2 * b.ss;
+  const float2 *Sb = b.S + (long long)k1b * N2 * b.ss;
   for (int e0 = threadIdx.x; e0 < tot; e0 += SLAB_U * T) {
     float2 v[SLAB_U];
 #pragma unroll
@@ -126,8 +129,8 @@ __global__ __launch_bounds__(SLAB_MAX_THREADS) void k_slab2(Ctx c, int guard) {
       v[u] = make_float2(0.f, 0.f);
       if (e < tot) {
         const int s = e & (B - 1), n2 = e >> lb, cp = s & (SLAB_CP - 1);
-        if (s < SLAB_CP) v[u] = Sa[n2 * SLAB_CP + cp];
-        else if (two) v[u] = Sb[n2 * SLAB_CP + cp];
+        if (s < SLAB_CP) v[u] = Sa[n2 * b.ss + cp];
+        else if (two) v[u] = Sb[n2 * b.ss + cp];
       }
     }
 #pragma unroll
@@ -171,8 +174,8 @@ __global__ __launch_bounds__(SLAB_MAX_THREADS) void k_slab2(Ctx c, int guard) {
   __syncthreads();
   batch_fft<E>(buf, BS, lb, tws, F);
   const float2 *tw = c.tab + L.tw_off;
-  float2 *Da = b.S + (long long)k1a * N2 * SLAB_CP;
-  float2 *Db = b.S + (long long)k1b * N2 * SLAB_CP;
+  float2 *Da = b.S + (long long)k1a * N2 * b.ss;
+  float2 *Db = b.S + (long long)k1b * N2 * b.ss;
   for (int e = threadIdx.x; e < tot; e += T) {
     const int s = e & (B - 1), m2 = e >> lb, cp = s & (SLAB_CP - 1);
     const bool setb = s >= SLAB_CP;
@@ -180,7 +183,7 @@ __global__ __launch_bounds__(SLAB_MAX_THREADS) void k_slab2(Ctx c, int guard) {
     const int k1 = setb ? k1b : k1a;
     float2 v = buf[m2 * BS + s];
     if (k1 && m2) v = cmul(v, long_twiddle(tw, k1 * m2));
-    (setb ? Db : Da)[m2 * SLAB_CP + cp] = v;
+    (setb ? Db : Da)[m2 * b.ss + cp] = v;
   }
 }
 
@@ -208,7 +211,7 @@ __global__ __launch_bounds__(SLAB_MAX_THREADS) void k_slab3(Ctx c, float *__rest
       if (e < tot) {
         const int s = e & (B - 1), k1 = e >> lb;
         const int m2 = m2_0 + (s >> 5);
-        if (m2 < N2) v[u] = b.S[(k1 * N2 + m2) * SLAB_CP + (s & (SLAB_CP - 1))];
+        if (m2 < N2) v[u] = b.S[(k1 * N2 + m2) * b.ss + (s & (SLAB_CP - 1))];
       }
     }
 #pragma unroll
@@ -314,7 +317,7 @@ __global__ __launch_bounds__(256) void k_slab1_f(Ctx c, const float *__restrict_
   float2 o[16];
   reg_stage2<N1>(o, v, t, buf, SB, s);
   if (n2 >= N2) return;
-  float2 *S = b.S + n2 * SLAB_CP + cp;
+  float2 *S = b.S + n2 * b.ss + cp;
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
 #pragma unroll
@@ -322,7 +325,7 @@ __global__ __launch_bounds__(256) void k_slab1_f(Ctx c, const float *__restrict_
       const int k1 = t * Q + i + 16 * k2;
       float2 y = o[i * R + k2];
       if (k1 && n2) y = cmul(y, twiddle(tws, k1 * n2));
-      S[(long long)k1 * N2 * SLAB_CP] = y;
+      S[(long long)k1 * N2 * b.ss] = y;
     }
   }
 }
@@ -345,9 +348,9 @@ __global__ __launch_bounds__(256) void k_slab3_f(Ctx c, float *__restrict__ dst,
   const int ja = 2 * (b.slab * SLAB_CP + cp);
   float2 v[16];
   if (m2 < N2) {
-    const float2 *S = b.S + (long long)(t * N2 + m2) * SLAB_CP + cp;  // k1 = t + R j
+    const float2 *S = b.S + (long long)(t * N2 + m2) * b.ss + cp;  // k1 = t + R j
 #pragma unroll
-    for (int j = 0; j < 16; ++j) v[j] = S[(long long)(R * j) * N2 * SLAB_CP];
+    for (int j = 0; j < 16; ++j) v[j] = S[(long long)(R * j) * N2 * b.ss];
   } else {
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = make_float2(0.f, 0.f);
@@ -390,11 +393,11 @@ __global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
   const int k1a = b.grp, k1b = b.grp ? N1 - b.grp : 0;
   const bool two = k1a != k1b, live = sig == 0 || two;
   const int k1 = sig ? k1b : k1a;
-  float2 *Srow = b.S + (long long)k1 * N2 * SLAB_CP + cp;
+  float2 *Srow = b.S + (long long)k1 * N2 * b.ss + cp;
   float2 v[16];
   if (live) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) v[j] = Srow[(t + R * j) * SLAB_CP];
+    for (int j = 0; j < 16; ++j) v[j] = Srow[(t + R * j) * b.ss];
   } else {
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = make_float2(0.f, 0.f);
@@ -481,7 +484,7 @@ __global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
     const int m = t + R * mm;
     float2 y = v[mm];
     if (k1 && m) y = cmul(y, twiddle(tws, k1 * m));
-    Srow[m * SLAB_CP] = y;
+    Srow[m * b.ss] = y;
   }
 }
 
