@@ -152,6 +152,63 @@ template <> struct Dft<16> {
   }
 };
 
+// exp(-2 pi i m / 64) for a compile-time m (first-quadrant cosines, folded by symmetry).
+__device__ __forceinline__ float2 w64(int m) {
+  constexpr float C[17] = {1.00000000000000000000f, 0.99518472667219692873f, 0.98078528040323043058f,
+                           0.95694033573220882438f, 0.92387953251128673848f, 0.88192126434835504956f,
+                           0.83146961230254523567f, 0.77301045336273699338f, 0.70710678118654757274f,
+                           0.63439328416364548779f, 0.55557023301960228867f, 0.47139673682599780857f,
+                           0.38268343236508983729f, 0.29028467725446233105f, 0.19509032201612833135f,
+                           0.09801714032956077016f, 0.0f};
+  m &= 63;
+  const float c = m <= 16 ? C[m] : m <= 32 ? -C[32 - m] : m <= 48 ? -C[m - 32] : C[64 - m];
+  const float s = m <= 16 ? C[16 - m] : m <= 32 ? C[m - 16] : m <= 48 ? -C[48 - m] : -C[m - 48];
+  return make_float2(c, -s);
+}
+
+// 32-point DFT: two 16-point DFTs of the even and odd samples, radix-2 combine.
+template <> struct Dft<32> {
+  static __device__ __forceinline__ void run(float2 *v) {
+    float2 e[16], o[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      e[i] = v[2 * i];
+      o[i] = v[2 * i + 1];
+    }
+    Dft<16>::run(e);
+    Dft<16>::run(o);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const float2 t = k == 0 ? o[0] : (k == 8 ? mul_negi(o[8]) : cmul(o[k], w64(2 * k)));
+      v[k] = cadd(e[k], t);
+      v[k + 16] = csub(e[k], t);
+    }
+  }
+};
+
+// 64-point DFT: four 16-point DFTs of x[4m + q], twiddles exp(-2 pi i q k1 / 64), radix-4 combine:
+// X[k1 + 16 k2].
+template <> struct Dft<64> {
+  static __device__ __forceinline__ void run(float2 *v) {
+    float2 e[4][16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+      for (int m = 0; m < 16; ++m) e[q][m] = v[4 * m + q];
+      Dft<16>::run(e[q]);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) {
+      float2 a[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a[q] = (q == 0 || k1 == 0) ? e[q][k1] : cmul(e[q][k1], w64(q * k1));
+      Dft<4>::run(a);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 16 * k2] = a[k2];
+    }
+  }
+};
+
 // Twiddle exp(-2 pi i m / L) from the split LDS table (lo: 64 entries, hi: ceil(L/64) entries).
 __device__ __forceinline__ float2 twiddle(const float2 *tws, int m) {
   return cmul(tws[TW_LO + (m >> 6)], tws[m & 63]);

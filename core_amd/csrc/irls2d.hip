@@ -18,6 +18,7 @@
 
 #include "irls2d.h"
 #include "epilogue.hpp"
+#include "goldstein.hpp"
 #include "instances.hpp"
 
 namespace irls2d {
@@ -26,6 +27,19 @@ IRLS2D_FFT_KERNELS(16, extern)
 IRLS2D_FFT_KERNELS(24, extern)
 IRLS2D_LONG_KERNELS(extern)
 IRLS2D_SLAB_KERNELS(extern)
+// defined in goldstein.hip
+#define IRLS2D_GOLD_EXTERN(PY, PX)                                 \
+  extern template __global__ void k_gold_patch<PY, PX>(GoldArgs); \
+  extern template __global__ void k_gold_fold<PY, PX>(GoldArgs);
+IRLS2D_GOLD_EXTERN(16, 16)
+IRLS2D_GOLD_EXTERN(16, 32)
+IRLS2D_GOLD_EXTERN(16, 64)
+IRLS2D_GOLD_EXTERN(32, 16)
+IRLS2D_GOLD_EXTERN(32, 32)
+IRLS2D_GOLD_EXTERN(32, 64)
+IRLS2D_GOLD_EXTERN(64, 16)
+IRLS2D_GOLD_EXTERN(64, 32)
+IRLS2D_GOLD_EXTERN(64, 64)
 }  // namespace irls2d
 
 namespace irls2d {
@@ -1284,6 +1298,104 @@ int plan_reset() {
 }
 
 // ---------------------------------------------------------------------------------------------
+// Goldstein filter (utils_goldstein.py): patch filter + overlap-add
+// ---------------------------------------------------------------------------------------------
+
+static bool gold_psize_ok(int p) { return p == 16 || p == 32 || p == 64; }
+
+// Patch grid per axis: floor((n - s) / s) + 1 patches for mode 1 (goldstein_numpy,
+// utils_goldstein.py:53-57, :85-86), ceil(...) + 1 for mode 0 (goldstein_pytorch, :224-230).
+static int64_t gold_npatch(int64_t n, int p, int mode) {
+  const int64_t s = p / 2;
+  return (mode == 1 ? (n - s) / s : (n - s + s - 1) / s) + 1;
+}
+
+static int gold_check(int64_t ny, int64_t nx, int py, int px) {
+  g_err.clear();
+  if (!gold_psize_ok(py) || !gold_psize_ok(px))
+    return fail(IRLS2D_EUNSUPPORTED, "goldstein patch sizes must be 16, 32 or 64");
+  if (ny < py / 2 || nx < px / 2) return fail(IRLS2D_EINVAL, "goldstein: image smaller than half a patch");
+  if (ny * nx >= ((int64_t)1 << 40)) return fail(IRLS2D_EINVAL, "goldstein: image too large");
+  return IRLS2D_OK;
+}
+
+static size_t gold_ws_bytes(int64_t ny, int64_t nx, int py, int px) {
+  const int64_t np = gold_npatch(ny, py, 0) * gold_npatch(nx, px, 0);  // the larger grid
+  return align_up((size_t)np * py * px * sizeof(float2), 256) + align_up((size_t)np * sizeof(int), 256);
+}
+
+int goldstein_workspace_size(int64_t ny, int64_t nx, int py, int px, size_t *bytes) {
+  int rc = gold_check(ny, nx, py, px);
+  if (rc) return rc;
+  if (!bytes) return fail(IRLS2D_EINVAL, "null bytes pointer");
+  *bytes = gold_ws_bytes(ny, nx, py, px);
+  return IRLS2D_OK;
+}
+
+template <int PY, int PX>
+static int gold_launch(const GoldArgs &g, hipStream_t s) {
+  constexpr int PM = PY > PX ? PY : PX;
+  constexpr int NP = GOLD_THREADS / PM;
+  const size_t lds = (size_t)NP * PY * (PX + 1) * sizeof(float2);
+  HIP_TRY(hipFuncSetAttribute((const void *)k_gold_patch<PY, PX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds));
+  const long long np = (long long)g.npy * g.npx;
+  LAUNCH("k_gold_patch", (k_gold_patch<PY, PX>), dim3((unsigned)((np + NP - 1) / NP)), dim3(GOLD_THREADS), lds, s, g);
+  const long long n = g.ny * g.nx;
+  const unsigned nb = (unsigned)std::min<long long>((n + GOLD_THREADS - 1) / GOLD_THREADS, 1 << 20);
+  LAUNCH("k_gold_fold", (k_gold_fold<PY, PX>), dim3(nb), dim3(GOLD_THREADS), 0, s, g);
+  HIP_TRY(hipGetLastError());
+  return IRLS2D_OK;
+}
+
+int goldstein(int64_t ny, int64_t nx, const void *phase, const float *corr, void *out, int py, int px,
+              double threshold, int mode, void *workspace, size_t ws_bytes, void *stream) {
+  int rc = gold_check(ny, nx, py, px);
+  if (rc) return rc;
+  if (!phase || !corr || !out || !workspace) return fail(IRLS2D_EINVAL, "null pointer argument");
+  if (mode != 0 && mode != 1) return fail(IRLS2D_EINVAL, "goldstein mode must be 0 (count) or 1 (weight)");
+  if (!(threshold >= 0.0) || threshold > 1.0) return fail(IRLS2D_EINVAL, "goldstein threshold must be in [0, 1]");
+  if (ws_bytes < gold_ws_bytes(ny, nx, py, px)) return fail(IRLS2D_EWORKSPACE, "goldstein workspace too small");
+  GoldArgs g;
+  g.phase = (const float2 *)phase;
+  g.corr = corr;
+  g.out = (float2 *)out;
+  const int64_t npy = gold_npatch(ny, py, mode), npx = gold_npatch(nx, px, mode);
+  g.patches = (float2 *)workspace;
+  g.pvalid = (int *)((unsigned char *)workspace +
+                     align_up((size_t)gold_npatch(ny, py, 0) * gold_npatch(nx, px, 0) * py * px * sizeof(float2), 256));
+  g.ny = ny;
+  g.nx = nx;
+  g.npy = (int)npy;
+  g.npx = (int)npx;
+  g.min_valid = (int)((double)py * (double)px * threshold);  // Python int(): truncation
+  g.mode = mode;
+  double ws = 0.0;
+  for (int r = 0; r < py; ++r)
+    for (int c = 0; c < px; ++c) {
+      auto w1 = [](int i, int p) {
+        const int h = p / 2, j = i < h ? i : p - 1 - i;
+        return 1.0 - std::fabs((double)j - (double)(h - 1)) / (double)(h - 1);
+      };
+      ws += (double)(float)(w1(r, py) * w1(c, px));
+    }
+  g.wsum = ws;
+  hipStream_t s = (hipStream_t)stream;
+#define IRLS2D_GOLD_CASE(PY, PX) \
+  if (py == PY && px == PX) return gold_launch<PY, PX>(g, s);
+  IRLS2D_GOLD_CASE(16, 16)
+  IRLS2D_GOLD_CASE(16, 32)
+  IRLS2D_GOLD_CASE(16, 64)
+  IRLS2D_GOLD_CASE(32, 16)
+  IRLS2D_GOLD_CASE(32, 32)
+  IRLS2D_GOLD_CASE(32, 64)
+  IRLS2D_GOLD_CASE(64, 16)
+  IRLS2D_GOLD_CASE(64, 32)
+  IRLS2D_GOLD_CASE(64, 64)
+  return fail(IRLS2D_EUNSUPPORTED, "goldstein patch size");
+}
+
+// ---------------------------------------------------------------------------------------------
 // Device epilogue steps (one image, pitched view; the host takes the median and size order)
 // ---------------------------------------------------------------------------------------------
 
@@ -1453,6 +1565,17 @@ int irls2d_plan_describe(int64_t n, int64_t *fft_len, int *radices, int *nradice
 int irls2d_plan_reset(void) { return irls2d::plan_reset(); }
 
 int irls2d_set_column_mode(int mode) { return irls2d::set_column_mode(mode); }
+
+int irls2d_goldstein_workspace_size(int64_t ny, int64_t nx, int psize_y, int psize_x, size_t *bytes) {
+  return irls2d::goldstein_workspace_size(ny, nx, psize_y, psize_x, bytes);
+}
+
+int irls2d_goldstein(int64_t ny, int64_t nx, const void *phase, const float *corr, void *out, int psize_y,
+                     int psize_x, double threshold, int mode, void *workspace, size_t workspace_bytes,
+                     void *stream) {
+  return irls2d::goldstein(ny, nx, phase, corr, out, psize_y, psize_x, threshold, mode, workspace, workspace_bytes,
+                           stream);
+}
 
 int irls2d_describe_columns(int n, const int64_t *H, const int64_t *W, int32_t *strategy) {
   if (!strategy) return irls2d::fail(IRLS2D_EINVAL, "null strategy pointer");

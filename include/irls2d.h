@@ -151,6 +151,23 @@ int irls2d_plan_reset(void);
 int irls2d_set_column_mode(int mode);
 
 /*
+ * Goldstein adaptive phase filter of one ny x nx complex64 image (interleaved re, im; rows of nx)
+ * with its float32 correlation, into `out` (complex64, same layout; may not alias `phase`).
+ * Replaces insardev/insardev/utils_goldstein.py: goldstein_pytorch (:157-330, the reference's GPU
+ * path, mode 0: overlapping patches averaged over the number of valid patches) and goldstein_numpy
+ * (:12-154, its CPU path, mode 1: weighted by the triangle window sums), as dispatched by
+ * Batch.py:1969-2035.  Patch sizes 16, 32 or 64 per axis (steps of half a patch); a patch is valid
+ * with at least int(psize_y psize_x threshold) non-NaN pixels; NaN phase/corr read as 0; output
+ * NaN where the input is NaN or no valid patch (mode 0) / no window weight (mode 1) covers it.
+ * Errors: IRLS2D_EUNSUPPORTED for other patch sizes, IRLS2D_EINVAL for an image smaller than half
+ * a patch or bad arguments, IRLS2D_EWORKSPACE below irls2d_goldstein_workspace_size.
+ */
+int irls2d_goldstein_workspace_size(int64_t ny, int64_t nx, int psize_y, int psize_x, size_t *bytes);
+int irls2d_goldstein(int64_t ny, int64_t nx, const void *phase, const float *corr, void *out, int psize_y,
+                     int psize_x, double threshold, int mode, void *workspace, size_t workspace_bytes,
+                     void *stream);
+
+/*
  * Column strategy each image of a batch of shapes gets under the current mode and plans (no
  * reference counterpart): strategy[i] is one of the IRLS2D_COLUMNS_* values below.
  */
