@@ -15,6 +15,8 @@ Also reported:
   batched       48 uneven SBAS-burst interferograms (H ~ U{1000..1500}, W ~ U{4000..6000},
                 seeds 1000 + i) solved in one batched graph launch vs 48 sequential plain calls.
   adjoint       16 x 2048 x 2048 forward solve and implicit adjoint (dL/dphase, dL/dweight).
+  goldstein     the upstream Goldstein filter on an 8192 x 8192 complex64 interferogram (psize 32),
+                with its float32 numpy restatement timed on a 2048 x 2048 crop (--no-goldstein).
   max_scale     (--max-scale / --only-max-scale) 40000 x 40000 solve with two-pass transforms,
                 per-kernel GB/s, and cycle-correctness against the generator's noiseless phase.
   cpu_baseline  the float32 torch-CPU restatement of the reference solver (oracle.irls_torch32)
@@ -290,6 +292,50 @@ def max_scale(args, dev):
     }
 
 
+def goldstein_bench(args, dev, with_cpu=True):
+    """Goldstein filter (the unwrapper's upstream step, utils_goldstein.py) on an 8192 x 8192
+    complex64 interferogram: GPU time per call (inputs in HBM), the GB/s of its algorithmic bytes
+    (phase 8 + corr 4 + out 8 B/px), and the float32 numpy restatement of
+    the reference's CPU path on a 2048 x 2048 crop (one thread, as its scipy.fft workers=1)."""
+    import numpy as np
+    import torch
+    from core_amd.goldstein import goldstein_device
+    n = 8192
+    g = torch.Generator(device=dev).manual_seed(5)
+    ph = torch.polar(0.5 + torch.rand(n, n, device=dev, generator=g), torch.rand(n, n, device=dev, generator=g) * 6.2832)
+    cr = torch.rand(n, n, device=dev, generator=g)
+    out = goldstein_device(ph, cr, psize=32)
+    ms, _ = time_steps(lambda: goldstein_device(ph, cr, psize=32, out=out), 10, 2)
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    e[0].record()
+    goldstein_device(ph, cr, psize=32, out=out)
+    e[1].record()
+    torch.cuda.synchronize()
+    res = {
+        "workload": "8192x8192 complex64 (random phase, amplitude 0.5-1.5), corr U(0,1), psize 32, threshold 0.5, "
+                    "valid-patch-count normalisation (the reference's GPU path)",
+        "ms": round(ms, 3), "mpx_s": round(n * n / ms / 1e3, 1),
+        "alg_bytes_per_px": 20,
+        "alg_gbs": round(20 * n * n / (ms / 1e3) / 1e9, 1),
+        "single_call_ms": round(e[0].elapsed_time(e[1]), 3),
+    }
+    del ph, cr, out
+    torch.cuda.empty_cache()
+    if with_cpu:
+        import time
+        from oracle import goldstein as G
+        rng = np.random.default_rng(5)
+        m = 2048
+        z = ((0.5 + rng.uniform(0, 1, (m, m))) * np.exp(1j * rng.uniform(0, 6.2832, (m, m)))).astype(np.complex64)
+        c = rng.uniform(0, 1, (m, m)).astype(np.float32)
+        t0 = time.time()
+        G.filter(z, c, 32, 32, 0.5, "count", np.float32)
+        dt = time.time() - t0
+        res["cpu_baseline"] = {"value": round(m * m / dt / 1e6, 3), "unit": "Mpx/s", "cores": 1, "kind": "port",
+                               "sample": f"2048x2048 crop, float32 numpy restatement (batched FFT2), {dt:.2f} s"}
+    return res
+
+
 def cpu_baseline(psi, coh, rows=2048, cols=2048):
     """Reference CPU path (float32 torch restatement) on the top-left rows x cols of the workload."""
     import numpy as np
@@ -322,6 +368,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-goldstein", action="store_true")
     ap.add_argument("--max-scale", action="store_true",
                     help="also solve the 40000 x 40000 workload (about 4 minutes, ~140 GB of HBM)")
     ap.add_argument("--only-max-scale", action="store_true", help="run only the 40000 x 40000 workload")
@@ -375,6 +422,8 @@ def main():
         result["batched"] = batched_stack(args, dev)
     if not args.no_adjoint:
         result["adjoint"] = adjoint_batch(args, dev)
+    if not args.no_goldstein:
+        result["goldstein"] = goldstein_bench(args, dev, with_cpu=not args.no_cpu)
     if args.max_scale:
         result["max_scale"] = max_scale(args, dev)
     if not args.no_cpu:

@@ -69,18 +69,28 @@ def filter(phase, corr, psize_y=32, psize_x=32, threshold=0.5, mode="count", dty
     valid = np.zeros((Hp, Wp), bool)
     valid[:ny, :nx] = ~nan_mask
     min_valid = int(psize_y * psize_x * threshold)
+    # all patches at once (as the reference does): statistics, FFT2, spectral weight, inverse
+    from numpy.lib.stride_tricks import sliding_window_view
+    sy, sx = psize_y // 2, psize_x // 2
+    win = (psize_y, psize_x)
+    P = sliding_window_view(ph, win)[::sy, ::sx].reshape(-1, psize_y, psize_x)
+    C = sliding_window_view(cr, win)[::sy, ::sx].reshape(-1, psize_y, psize_x)
+    V = sliding_window_view(valid, win)[::sy, ::sx].reshape(-1, psize_y, psize_x)
+    pvalid = V.sum(axis=(1, 2)) >= min_valid
+    alpha = 1.0 - np.einsum('ij,kij->k', w.astype(np.float64), C.astype(np.float64)) / wsum
+    F = np.fft.fft2(P, axes=(1, 2))
+    F = F * np.power(np.maximum(np.abs(F), 1e-10), alpha[:, None, None])
+    Q = (np.fft.ifft2(F, axes=(1, 2)).astype(cdt) * w)
     out = np.zeros((Hp, Wp), cdt)
     cnt = np.zeros((Hp, Wp), np.float64)
-    for y0 in ys:
+    k = 0
+    for y0 in ys:  # overlap-add in raster order, as the reference
         for x0 in xs:
-            sl = (slice(y0, y0 + psize_y), slice(x0, x0 + psize_x))
-            if int(valid[sl].sum()) < min_valid:
-                continue
-            alpha = 1.0 - float((w.astype(np.float64) * cr[sl]).sum()) / wsum
-            F = np.fft.fft2(ph[sl])
-            F = F * np.power(np.maximum(np.abs(F), 1e-10), alpha)
-            out[sl] += np.fft.ifft2(F).astype(cdt) * w
-            cnt[sl] += w if mode == "weight" else 1.0
+            if pvalid[k]:
+                sl = (slice(y0, y0 + psize_y), slice(x0, x0 + psize_x))
+                out[sl] += Q[k]
+                cnt[sl] += w if mode == "weight" else 1.0
+            k += 1
     insufficient = cnt < 1e-6
     res = out / (np.maximum(cnt, 1e-10) if mode == "weight" else np.maximum(cnt, 1.0))
     res = res[:ny, :nx].astype(np.complex64 if dtype == np.float32 else np.complex128)
