@@ -33,6 +33,8 @@ EXPORTS = (
     "irls2d_plan_reset",
     "irls2d_set_column_mode",
     "irls2d_describe_columns",
+    "irls2d_gaussian_workspace_size",
+    "irls2d_gaussian",
     "irls2d_goldstein_workspace_size",
     "irls2d_goldstein",
     "irls2d_ep_blocks",
@@ -100,6 +102,13 @@ def load():
     L.irls2d_set_column_mode.restype = ctypes.c_int
     L.irls2d_describe_columns.argtypes = [ctypes.c_int, i64p, i64p, ctypes.POINTER(ctypes.c_int32)]
     L.irls2d_describe_columns.restype = ctypes.c_int
+    L.irls2d_gaussian_workspace_size.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_double,
+                                                 ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_size_t)]
+    L.irls2d_gaussian_workspace_size.restype = ctypes.c_int
+    L.irls2d_gaussian.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    L.irls2d_gaussian.restype = ctypes.c_int
     L.irls2d_goldstein_workspace_size.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                                   ctypes.POINTER(ctypes.c_size_t)]
     L.irls2d_goldstein_workspace_size.restype = ctypes.c_int

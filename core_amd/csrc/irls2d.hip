@@ -18,6 +18,7 @@
 
 #include "irls2d.h"
 #include "epilogue.hpp"
+#include "gaussian.hpp"
 #include "goldstein.hpp"
 #include "instances.hpp"
 
@@ -1396,6 +1397,77 @@ int goldstein(int64_t ny, int64_t nx, const void *phase, const float *corr, void
 }
 
 // ---------------------------------------------------------------------------------------------
+// NaN-aware Gaussian filter (utils_gaussian.py nanconvolve2d_gaussian_pytorch)
+// ---------------------------------------------------------------------------------------------
+
+static int gauss_check(int64_t ny, int64_t nx, int channels, double sy, double sx, double truncate) {
+  g_err.clear();
+  if (ny < 1 || nx < 1 || ny * nx >= ((int64_t)1 << 40)) return fail(IRLS2D_EINVAL, "gaussian: bad image shape");
+  if (channels != 1 && channels != 2) return fail(IRLS2D_EINVAL, "gaussian: channels must be 1 (real) or 2 (complex)");
+  if (!(sy > 0) || !(sx > 0) || !(truncate > 0) || !std::isfinite(sy * truncate) || !std::isfinite(sx * truncate))
+    return fail(IRLS2D_EINVAL, "gaussian: sigma and truncate must be positive and finite");
+  if (sy * truncate > (1 << 20) || sx * truncate > (1 << 20)) return fail(IRLS2D_EINVAL, "gaussian: kernel too long");
+  return IRLS2D_OK;
+}
+
+// kernel radius: size 2 ceil(s truncate) + 1 (utils_gaussian.py:51-52)
+static int gauss_radius(double s, double truncate) { return (int)std::ceil(s * truncate); }
+
+static size_t gauss_ws_bytes(int64_t ny, int64_t nx, int channels, int ry, int rx) {
+  return align_up((size_t)ny * nx * 2 * channels * sizeof(float), 256) +
+         align_up((size_t)(2 * ry + 1) * sizeof(float), 256) + align_up((size_t)(2 * rx + 1) * sizeof(float), 256);
+}
+
+int gaussian_workspace_size(int64_t ny, int64_t nx, int channels, double sy, double sx, double truncate,
+                            size_t *bytes) {
+  int rc = gauss_check(ny, nx, channels, sy, sx, truncate);
+  if (rc) return rc;
+  if (!bytes) return fail(IRLS2D_EINVAL, "null bytes pointer");
+  *bytes = gauss_ws_bytes(ny, nx, channels, gauss_radius(sy, truncate), gauss_radius(sx, truncate));
+  return IRLS2D_OK;
+}
+
+int gaussian(int64_t ny, int64_t nx, int channels, const float *data, const float *weight, float *out, double sy,
+             double sx, double truncate, double threshold, void *workspace, size_t ws_bytes, void *stream) {
+  int rc = gauss_check(ny, nx, channels, sy, sx, truncate);
+  if (rc) return rc;
+  if (!data || !out || !workspace) return fail(IRLS2D_EINVAL, "null pointer argument");
+  const int ry = gauss_radius(sy, truncate), rx = gauss_radius(sx, truncate);
+  if (ws_bytes < gauss_ws_bytes(ny, nx, channels, ry, rx)) return fail(IRLS2D_EWORKSPACE, "gaussian workspace too small");
+  GaussArgs g;
+  g.data = data;
+  g.weight = weight;
+  g.out = out;
+  unsigned char *w = (unsigned char *)workspace;
+  g.tmp = (float *)w;
+  w += align_up((size_t)ny * nx * 2 * channels * sizeof(float), 256);
+  g.ky = (float *)w;
+  w += align_up((size_t)(2 * ry + 1) * sizeof(float), 256);
+  g.kx = (float *)w;
+  g.ny = ny;
+  g.nx = nx;
+  g.ry = ry;
+  g.rx = rx;
+  g.sy = (float)sy;
+  g.sx = (float)sx;
+  // the comparison of the float32 denominator with the Python threshold happens in float32
+  g.lim = weight ? 1e-10f : (float)threshold;
+  hipStream_t s = (hipStream_t)stream;
+  LAUNCH("k_gauss_taps", k_gauss_taps, dim3(1), dim3(GAUSS_THREADS), 0, s, g);
+  const long long n = ny * nx;
+  const dim3 grid((unsigned)std::min<long long>((n + GAUSS_THREADS - 1) / GAUSS_THREADS, 1 << 20));
+  if (channels == 1) {
+    LAUNCH("k_gauss_v", k_gauss_v<1>, grid, dim3(GAUSS_THREADS), 0, s, g);
+    LAUNCH("k_gauss_h", k_gauss_h<1>, grid, dim3(GAUSS_THREADS), 0, s, g);
+  } else {
+    LAUNCH("k_gauss_v", k_gauss_v<2>, grid, dim3(GAUSS_THREADS), 0, s, g);
+    LAUNCH("k_gauss_h", k_gauss_h<2>, grid, dim3(GAUSS_THREADS), 0, s, g);
+  }
+  HIP_TRY(hipGetLastError());
+  return IRLS2D_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
 // Device epilogue steps (one image, pitched view; the host takes the median and size order)
 // ---------------------------------------------------------------------------------------------
 
@@ -1565,6 +1637,18 @@ int irls2d_plan_describe(int64_t n, int64_t *fft_len, int *radices, int *nradice
 int irls2d_plan_reset(void) { return irls2d::plan_reset(); }
 
 int irls2d_set_column_mode(int mode) { return irls2d::set_column_mode(mode); }
+
+int irls2d_gaussian_workspace_size(int64_t ny, int64_t nx, int channels, double sigma_y, double sigma_x,
+                                   double truncate, size_t *bytes) {
+  return irls2d::gaussian_workspace_size(ny, nx, channels, sigma_y, sigma_x, truncate, bytes);
+}
+
+int irls2d_gaussian(int64_t ny, int64_t nx, int channels, const float *data, const float *weight, float *out,
+                    double sigma_y, double sigma_x, double truncate, double threshold, void *workspace,
+                    size_t workspace_bytes, void *stream) {
+  return irls2d::gaussian(ny, nx, channels, data, weight, out, sigma_y, sigma_x, truncate, threshold, workspace,
+                          workspace_bytes, stream);
+}
 
 int irls2d_goldstein_workspace_size(int64_t ny, int64_t nx, int psize_y, int psize_x, size_t *bytes) {
   return irls2d::goldstein_workspace_size(ny, nx, psize_y, psize_x, bytes);

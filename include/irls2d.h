@@ -151,6 +151,21 @@ int irls2d_plan_reset(void);
 int irls2d_set_column_mode(int mode);
 
 /*
+ * NaN-aware separable Gaussian filter of one ny x nx image, real (channels 1) or complex
+ * (channels 2, interleaved re, im; each part filtered on its own), into `out` (same layout; may
+ * not alias `data`).  Replaces insardev/insardev/utils_gaussian.py nanconvolve2d_gaussian_pytorch
+ * (:14-116, the reference's GPU path): 1-D kernels of size 2 ceil(sigma truncate) + 1, replicate
+ * borders, y then x; NaN data and NaN weights read as weight 0; `weight` (ny x nx float32) may be
+ * null; out = numerator / (denominator + 1e-10) where the denominator exceeds `threshold` (no
+ * weight) or 1e-10 (with weight), NaN elsewhere.
+ */
+int irls2d_gaussian_workspace_size(int64_t ny, int64_t nx, int channels, double sigma_y, double sigma_x,
+                                   double truncate, size_t *bytes);
+int irls2d_gaussian(int64_t ny, int64_t nx, int channels, const float *data, const float *weight, float *out,
+                    double sigma_y, double sigma_x, double truncate, double threshold, void *workspace,
+                    size_t workspace_bytes, void *stream);
+
+/*
  * Goldstein adaptive phase filter of one ny x nx complex64 image (interleaved re, im; rows of nx)
  * with its float32 correlation, into `out` (complex64, same layout; may not alias `phase`).
  * Replaces insardev/insardev/utils_goldstein.py: goldstein_pytorch (:157-330, the reference's GPU
