@@ -15,8 +15,9 @@ Also reported:
   batched       48 uneven SBAS-burst interferograms (H ~ U{1000..1500}, W ~ U{4000..6000},
                 seeds 1000 + i) solved in one batched graph launch vs 48 sequential plain calls.
   adjoint       16 x 2048 x 2048 forward solve and implicit adjoint (dL/dphase, dL/dweight).
-  goldstein     the upstream Goldstein filter on an 8192 x 8192 complex64 interferogram (psize 32),
-                with its float32 numpy restatement timed on a 2048 x 2048 crop (--no-goldstein).
+  filters       the upstream Goldstein (psize 32) and Gaussian (sigma 2) filters on an 8192 x 8192
+                complex64 interferogram, with their numpy restatements timed on a 2048 x 2048
+                crop (--no-filters).
   max_scale     (--max-scale / --only-max-scale) 40000 x 40000 solve with two-pass transforms,
                 per-kernel GB/s, and cycle-correctness against the generator's noiseless phase.
   cpu_baseline  the float32 torch-CPU restatement of the reference solver (oracle.irls_torch32)
@@ -336,6 +337,40 @@ def goldstein_bench(args, dev, with_cpu=True):
     return res
 
 
+def gaussian_bench(args, dev, with_cpu=True):
+    """NaN-aware Gaussian filter (utils_gaussian.py, the GPU path) on an 8192 x 8192 complex64
+    field, sigma 2 px (17 taps per axis): GPU time per call, GB/s of its algorithmic bytes (data
+    8 + out 8 B/px), and the float64 numpy restatement on a 2048 x 2048 crop (one thread; the
+    reference's CPU path is OpenCV's sepFilter2D, not available here)."""
+    import numpy as np
+    import torch
+    from core_amd.gaussian import gaussian_device
+    n = 8192
+    g = torch.Generator(device=dev).manual_seed(6)
+    z = torch.polar(0.5 + torch.rand(n, n, device=dev, generator=g), torch.rand(n, n, device=dev, generator=g) * 6.2832)
+    out = gaussian_device(z, None, 2.0)
+    ms, _ = time_steps(lambda: gaussian_device(z, None, 2.0), 10, 2)
+    res = {
+        "workload": "8192x8192 complex64 (random phase, amplitude 0.5-1.5), sigma 2 px, truncate 4, no weight",
+        "ms": round(ms, 3), "mpx_s": round(n * n / ms / 1e3, 1),
+        "alg_bytes_per_px": 16, "alg_gbs": round(16 * n * n / (ms / 1e3) / 1e9, 1),
+    }
+    del z, out
+    torch.cuda.empty_cache()
+    if with_cpu:
+        import time
+        from oracle import gaussian as G
+        rng = np.random.default_rng(6)
+        m = 2048
+        z = (0.5 + rng.uniform(0, 1, (m, m))) * np.exp(1j * rng.uniform(0, 6.2832, (m, m)))
+        t0 = time.time()
+        G.nanconv(z, None, 2.0)
+        dt = time.time() - t0
+        res["cpu_baseline"] = {"value": round(m * m / dt / 1e6, 3), "unit": "Mpx/s", "cores": 1, "kind": "port",
+                               "sample": f"2048x2048 crop, float64 numpy restatement, {dt:.2f} s"}
+    return res
+
+
 def cpu_baseline(psi, coh, rows=2048, cols=2048):
     """Reference CPU path (float32 torch restatement) on the top-left rows x cols of the workload."""
     import numpy as np
@@ -368,7 +403,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--no-goldstein", action="store_true")
+    ap.add_argument("--no-filters", action="store_true")
     ap.add_argument("--max-scale", action="store_true",
                     help="also solve the 40000 x 40000 workload (about 4 minutes, ~140 GB of HBM)")
     ap.add_argument("--only-max-scale", action="store_true", help="run only the 40000 x 40000 workload")
@@ -422,8 +457,9 @@ def main():
         result["batched"] = batched_stack(args, dev)
     if not args.no_adjoint:
         result["adjoint"] = adjoint_batch(args, dev)
-    if not args.no_goldstein:
-        result["goldstein"] = goldstein_bench(args, dev, with_cpu=not args.no_cpu)
+    if not args.no_filters:
+        result["filters"] = {"goldstein": goldstein_bench(args, dev, with_cpu=not args.no_cpu),
+                             "gaussian": gaussian_bench(args, dev, with_cpu=not args.no_cpu)}
     if args.max_scale:
         result["max_scale"] = max_scale(args, dev)
     if not args.no_cpu:
