@@ -124,3 +124,31 @@ def test_slab_batched_equals_plain_bitwise(hip):
             assert np.array_equal(ub, up, equal_nan=True) and np.array_equal(cb, cp)
     finally:
         _lib.set_column_mode(_lib.COLUMN_AUTO)
+
+
+def test_full_size_preconditioner_matches_scipy_fp64(hip):
+    """At 8192 x 8192 (the bench shape: register-stage slab columns, 128 x 64 split) the whole
+    transform chain (row DCT-II, three slab passes, row DCT-III) against scipy in FP64."""
+    from core_amd import _lib
+    from core_amd.unwrap2d import Workspace, image_view, pack, precondition_device
+    n = 8192
+    shapes = [(n, n)]
+    assert _lib.describe_columns(shapes) == ["slab_register"]
+    rng = np.random.default_rng(8)
+    r = rng.standard_normal((n, n)).astype(np.float32)
+    r += np.sin(np.arange(n, dtype=np.float32) / 700.0)[None, :]
+    print("preconditioner 8192^2 (slab_register)", flush=True)
+    ws = Workspace(shapes, hip)
+    z = precondition_device(pack([r], ws, hip), shapes, workspace=ws)
+    zg = image_view(z.cpu().numpy(), ws, 0).astype(np.float64)
+    ws.release()
+    import scipy.fft as sf
+    lam = 2 * np.cos(np.pi * np.arange(n) / n)[:, None] + 2 * np.cos(np.pi * np.arange(n) / n)[None, :] - 4
+    lam[0, 0] = 1.0
+    X = sf.dctn(r.astype(np.float64), type=2, workers=-1)
+    X /= -lam
+    X[0, 0] = 0.0
+    z64 = sf.idctn(X, type=2, workers=-1)
+    err = float(np.abs(zg - z64).max() / np.abs(z64).max())
+    print(f"  max rel err {err:.2e}", flush=True)
+    assert err < 2e-5
