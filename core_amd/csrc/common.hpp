@@ -40,10 +40,11 @@ constexpr int LONG_LDS = TW_LDS + LONG_BATCH_ELEMS + LONG_MAX_FACTOR;
 
 // Column pass on the row-major field (no transposes): slabs of SLAB_CP column pairs, the column
 // length H = N1 N2 transformed as a four-step FFT in three passes, each a batch of short FFTs in
-// LDS (slots x factor <= SLAB_E * threads, at most SLAB_MAX_THREADS threads).
+// LDS (slots x factor <= SLAB_E * threads, at most SLAB_MAX_THREADS threads for the batched-LDS
+// kernels).
 constexpr int SLAB_CP = 32;
 constexpr int SLAB_E = 16;
-constexpr int SLAB_MAX_THREADS = 512;
+constexpr int SLAB_MAX_THREADS = 1024;
 
 // One Stockham pass, precomputed on the host so the device does no integer division.
 struct PassDesc {

@@ -311,9 +311,10 @@ static std::shared_ptr<HostLong> get_long_plan(int N) {
 
 // Split of a 5-smooth column length for the slab passes: N = N1 N2 such that a K2 block (2 SLAB_CP
 // slots of length N2) and a K1 block (at least SLAB_CP slots of length N1) fit SLAB_E values per
-// thread; the split with the smallest larger block, then the larger N2.
+// thread of at most SLAB_MAX_THREADS; the split with the smallest larger block, then the larger
+// N2.  Lengths above MAX_FFT_LEN qualify too (e.g. 40000 = 250 x 160).
 static bool slab_split(int64_t N, int &N1, int &N2) {
-  if (N < 4 || N > MAX_FFT_LEN || !smooth235(N)) return false;
+  if (N < 4 || N > (int64_t)LONG_MAX_FACTOR * LONG_MAX_FACTOR || !smooth235(N)) return false;
   const int64_t cap = (int64_t)SLAB_E * SLAB_MAX_THREADS;
   int64_t best = -1, bestN2 = 0;
   for (int64_t b = 2; b <= N / 2; ++b) {
@@ -565,13 +566,16 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     d.rowplan = lw ? -1 : add_plan(w);
     d.colplan = lh ? -1 : add_plan(h);
     d.rowlong = lw ? add_long(w) : -1;
-    d.collong = lh ? add_long(h) : -1;
     d.cosH_off = add_cos(h);
     d.cosW_off = add_cos(w);
-    const bool want_slab = !lh && !lw && (g_col_mode == 3 || (g_col_mode == 0 && h >= g_slab_min));
+    // automatic mode: sides up to MAX_FFT_LEN only (at 40000 x 40000 the batched-LDS slab kernels
+    // of 40000 = 250 x 160 and the slab row order in the long-row passes were slower than the
+    // transposed two-pass path); mode 3 also takes long sides
+    const bool want_slab = g_col_mode == 3 || (g_col_mode == 0 && h >= g_slab_min && !lh && !lw);
     d.colslab = want_slab ? add_slab(h) : -1;
     d.nslab = (w + 2 * SLAB_CP - 1) / (2 * SLAB_CP);
     const bool slab = d.colslab >= 0;
+    d.collong = (lh && !slab) ? add_long(h) : -1;
     // transposed copy (W x PT), or the slab scratch (H rows of nslab x SLAB_CP float2 = P floats)
     Lo.offT[i] = Lo.totalT;
     d.offT = Lo.totalT;
@@ -584,7 +588,7 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
       lr1 = (int64_t)((h + 1) / 2) * P.nb1;
       lr2 = (int64_t)((h + 1) / 2) * P.nb2;
     }
-    if (lh) {
+    if (d.collong >= 0) {
       const LongPlan &P = Lo.lplans[d.collong];
       sz = std::max<int64_t>(sz, (int64_t)((w + 1) / 2) * P.spitch);
       lc1 = (int64_t)((w + 1) / 2) * P.nb1;

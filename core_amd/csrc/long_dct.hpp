@@ -105,6 +105,8 @@ struct LongLine {
   const LongPlan *L;
   int img, lp, blk;  // image, line pair, block within the line pair
   long long la;      // element offset of line a (line b = la + pitch)
+  long long ta, tb;  // offsets of lines a and b in the row-pass output T (its slab row order for
+                     // an image whose column pass runs on slabs, else la and la + pitch)
   long long pitch;
   bool hasb;
   float2 *S;         // scratch line of this pair
@@ -124,6 +126,13 @@ __device__ __forceinline__ bool long_line(const Ctx &c, int dir, int pass, int g
   ln.pitch = dir ? d.PT : d.P;
   ln.la = (dir ? d.offT : d.off) + (long long)(2 * ln.lp) * ln.pitch;
   ln.hasb = 2 * ln.lp + 1 < nlines;
+  ln.ta = ln.la;
+  ln.tb = ln.la + ln.pitch;
+  if (!dir && d.colslab >= 0) {
+    const SlabPlan &S = c.splans[d.colslab];
+    ln.ta = d.off + (long long)slab_trow(makhoul_pos(2 * ln.lp, d.H), S) * d.P;
+    ln.tb = ln.hasb ? d.off + (long long)slab_trow(makhoul_pos(2 * ln.lp + 1, d.H), S) * d.P : ln.ta;
+  }
   ln.S = c.S + d.offS + (long long)ln.lp * ln.L->spitch;
   return true;
 }
@@ -165,8 +174,8 @@ __global__ __launch_bounds__(LONG_THREADS) void k_long_p1(Ctx c, int dir, float 
   constexpr bool dct2 = (MODE == LONG_DCT2 || MODE == LONG_DCT2_UPDATE);
   constexpr int U = LONG_U;
   const int o = dct2 ? N2 - 1 : 0;
-  const float *a = src + ln.la;
-  const float *b = src + ln.la + ln.pitch;
+  const float *a = src + ln.ta;  // the DCT-III modes read T (or Tt) lines
+  const float *b = src + ln.tb;
   const int T = blockDim.x;
   double acc[2] = {0.0, 0.0};
   // blockDim.x is a multiple of B, so a thread keeps its slot (one FFT of the batch) and walks
@@ -349,7 +358,8 @@ __global__ __launch_bounds__(LONG_THREADS) void k_long_p2(Ctx c, int dir, float 
   }
   __syncthreads();
   batch_fft<E>(buf, BS, lb, tws, F);
-  float *da = dst + ln.la, *db = dst + ln.la + ln.pitch;
+  // the DCT-II writes T (or Tt) lines, the DCT-III the output field in row order
+  float *da = dst + (dct2 ? ln.ta : ln.la), *db = dst + (dct2 ? ln.tb : ln.la + ln.pitch);
   double acc[1] = {0.0};
   const int s = threadIdx.x & (B - 1), rstep = T >> lb;
   const int k1 = slot_index(s, C, ln.blk, N1, o);

@@ -138,15 +138,16 @@ int irls2d_plan_reset(void);
 
 /*
  * Column-pass strategy (no reference counterpart: the reference transposes with torch).
- * 0 = automatic (default): images whose column length is 2^a 3^b 5^c between 256 and 12288 run
- * the three four-step slab passes on the row-major field (no transposes); for the others the
+ * 0 = automatic (default): images with both sides <= 12288 whose column length is 2^a 3^b 5^c, at
+ * least 256, with a split N1 N2 (N1 <= 512, N2 <= 256) run the three four-step slab passes on the
+ * row-major field (no transposes); for the others the
  * column DCTs read column pairs straight from the row-major field when at least half of their
  * pixels have Bluestein column transforms (whose arithmetic hides the strided access), otherwise
  * the field is transposed, transformed and transposed back;
  * 1 = always transpose; 2 = always direct; 3 = slab passes for every image whose column length
- * allows them (2^a 3^b 5^c, 4 .. 12288), the automatic choice for the rest.  Columns longer than
- * 12288 always transpose.  Drops cached layouts and hipGraphs.  core_amd.autotune measures the
- * strategies for a batch.
+ * allows them (2^a 3^b 5^c with such a split, from 4), the automatic choice for the rest.  Other
+ * columns longer than 12288 transpose and run the two-pass long transform.  Drops cached layouts
+ * and hipGraphs.  core_amd.autotune measures the strategies for a batch.
  */
 int irls2d_set_column_mode(int mode);
 
@@ -188,7 +189,7 @@ int irls2d_goldstein(int64_t ny, int64_t nx, const void *phase, const float *cor
  */
 #define IRLS2D_COLUMNS_TRANSPOSE 0      /* transpose, LDS-resident column DCTs, transpose back */
 #define IRLS2D_COLUMNS_DIRECT 1         /* LDS-resident column DCTs on the row-major field */
-#define IRLS2D_COLUMNS_LONG 2           /* transpose, two-pass four-step DCTs (H > 12288) */
+#define IRLS2D_COLUMNS_LONG 2           /* transpose, two-pass four-step DCTs (H > 12288, no slab split) */
 #define IRLS2D_COLUMNS_SLAB 3           /* three four-step slab passes, batched LDS FFTs */
 #define IRLS2D_COLUMNS_SLAB_REGISTER 4  /* three four-step slab passes, register-stage FFTs */
 int irls2d_describe_columns(int n, const int64_t *H, const int64_t *W, int32_t *strategy);

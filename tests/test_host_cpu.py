@@ -209,11 +209,15 @@ def test_column_strategies(lib):
         assert d([(8192, 70), (1000, 7)]) == ["slab", "slab"]     # two splits: batched LDS FFTs
         assert d([(96, 130)]) == ["transpose"]                     # below the slab threshold
         assert d([(127, 211)]) == ["direct"]                       # Bluestein columns
-        assert d([(12800, 8)]) == ["long"]
-        assert d([(1000, 12500)]) == ["transpose"]                 # long rows keep the row-major T
+        assert d([(12800, 8)]) == ["long"]                         # long sides: transposed two-pass path
+        assert d([(1000, 12500)]) == ["transpose"]
         assert d([(8192, 70), (127, 211), (12800, 8)]) == ["slab_register", "transpose", "long"]
         lib.set_column_mode(lib.COLUMN_SLAB)
         assert d([(64, 48), (96, 130)]) == ["slab", "slab"]
+        assert d([(12800, 8)]) == ["slab"]                         # 12800 = 160 x 80
+        assert d([(40000, 64)]) == ["slab"]                        # 40000 = 250 x 160
+        assert d([(177147, 8)]) == ["long"]                        # 3^11: no split within the block limits
+        assert d([(1000, 12500)]) == ["slab"]                      # long rows write T in slab order too
         assert d([(127, 211)]) == ["direct"]                       # no slab split: automatic choice
         lib.set_column_mode(lib.COLUMN_TRANSPOSE)
         assert d([(8192, 8192)]) == ["transpose"]
