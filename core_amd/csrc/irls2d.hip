@@ -553,6 +553,20 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     return o;
   };
   for (int g = 0; g < NGEO; ++g) Lo.geo[g].assign(1, 0);
+  // Automatic mode takes the slab passes only with the register-stage kernels, which run one
+  // power-of-two split per batch: the split of the first image that has one.  Other column lengths
+  // keep the transposed path (on MI355X the batched-LDS slab kernels measured 0-9% slower than it at
+  // 6000^2, 3000 x 4000 and 1500 x 5000, and 4% slower at 40000^2).
+  int auto_f1 = 0, auto_f2 = 0;
+  auto auto_slab = [&](int64_t h, int64_t w, int &f1, int &f2) -> bool {
+    return h >= g_slab_min && h <= MAX_FFT_LEN && w <= MAX_FFT_LEN && slab_split(h, f1, f2) && slab_fast(f1, f2);
+  };
+  if (g_col_mode == 0) {
+    for (int i = 0; i < n && !auto_f1; ++i) {
+      int f1, f2;
+      if (auto_slab(H[i], W[i], f1, f2)) { auto_f1 = f1; auto_f2 = f2; }
+    }
+  }
   int maxM_row = 1, maxM_col = 1, ept_row = 16, ept_col = 16;
   for (int i = 0; i < n; ++i) {
     const int h = (int)H[i], w = (int)W[i];
@@ -568,10 +582,10 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     d.rowlong = lw ? add_long(w) : -1;
     d.cosH_off = add_cos(h);
     d.cosW_off = add_cos(w);
-    // automatic mode: sides up to MAX_FFT_LEN only (at 40000 x 40000 the batched-LDS slab kernels
-    // of 40000 = 250 x 160 and the slab row order in the long-row passes were slower than the
-    // transposed two-pass path); mode 3 also takes long sides
-    const bool want_slab = g_col_mode == 3 || (g_col_mode == 0 && h >= g_slab_min && !lh && !lw);
+    // mode 3 takes every 5-smooth column length, long sides included
+    int sf1 = 0, sf2 = 0;
+    const bool want_slab = g_col_mode == 3 || (g_col_mode == 0 && auto_f1 && auto_slab(h, w, sf1, sf2) &&
+                                               sf1 == auto_f1 && sf2 == auto_f2);
     d.colslab = want_slab ? add_slab(h) : -1;
     d.nslab = (w + 2 * SLAB_CP - 1) / (2 * SLAB_CP);
     const bool slab = d.colslab >= 0;

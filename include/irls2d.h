@@ -138,9 +138,10 @@ int irls2d_plan_reset(void);
 
 /*
  * Column-pass strategy (no reference counterpart: the reference transposes with torch).
- * 0 = automatic (default): images with both sides <= 12288 whose column length is 2^a 3^b 5^c, at
- * least 256, with a split N1 N2 (N1 <= 512, N2 <= 256) run the three four-step slab passes on the
- * row-major field (no transposes); for the others the
+ * 0 = automatic (default): images with both sides <= 12288 whose column length, at least 256,
+ * splits into a power-of-two pair N1 N2 (N1 16 .. 128, N2 16 .. 64: heights 256 .. 8192) with
+ * register-stage kernels run the three four-step slab passes on the row-major field (no
+ * transposes), for the split of the first such image of the batch; for the others the
  * column DCTs read column pairs straight from the row-major field when at least half of their
  * pixels have Bluestein column transforms (whose arithmetic hides the strided access), otherwise
  * the field is transposed, transformed and transposed back;

@@ -197,16 +197,21 @@ def test_autotune_candidates_are_valid(lib):
 
 
 def test_column_strategies(lib):
-    """Which column pass each image gets: slab passes for 2^a 3^b 5^c heights from 256 on (the
-    register-stage kernels when every slab image has the same power-of-two split), the column
-    kernel on the row-major field for Bluestein heights, the four-step long path above 12288."""
+    """Which column pass each image gets.  Automatic mode: the register-stage slab kernels for
+    heights from 256 on whose split is a power-of-two pair, for the split of the first such image
+    in the batch; the transposed pass for other heights (the batched-LDS slab kernels are
+    slower), the column kernel on the row-major field for Bluestein heights, the four-step long
+    path above 12288.  Slab mode: every 5-smooth height within the block limits."""
     d = lib.describe_columns
     try:
         lib.set_column_mode(lib.COLUMN_AUTO)
         assert d([(8192, 8192)]) == ["slab_register"]
         assert d([(8192, 70), (8192, 3)]) == ["slab_register"] * 2
         assert d([(256, 256)]) == ["slab_register"]
-        assert d([(8192, 70), (1000, 7)]) == ["slab", "slab"]     # two splits: batched LDS FFTs
+        assert d([(8192, 70), (1000, 7)]) == ["slab_register", "transpose"]  # 1000: no power-of-two split
+        assert d([(8192, 70), (2048, 9), (8192, 5)]) == ["slab_register", "transpose", "slab_register"]
+        assert d([(2048, 9), (8192, 70)]) == ["slab_register", "transpose"]  # the first split wins
+        assert d([(6000, 6000)]) == ["transpose"]
         assert d([(96, 130)]) == ["transpose"]                     # below the slab threshold
         assert d([(127, 211)]) == ["direct"]                       # Bluestein columns
         assert d([(12800, 8)]) == ["long"]                         # long sides: transposed two-pass path
@@ -214,6 +219,7 @@ def test_column_strategies(lib):
         assert d([(8192, 70), (127, 211), (12800, 8)]) == ["slab_register", "transpose", "long"]
         lib.set_column_mode(lib.COLUMN_SLAB)
         assert d([(64, 48), (96, 130)]) == ["slab", "slab"]
+        assert d([(8192, 70), (1000, 7)]) == ["slab", "slab"]     # two splits: batched LDS FFTs
         assert d([(12800, 8)]) == ["slab"]                         # 12800 = 160 x 80
         assert d([(40000, 64)]) == ["slab"]                        # 40000 = 250 x 160
         assert d([(177147, 8)]) == ["long"]                        # 3^11: no split within the block limits
