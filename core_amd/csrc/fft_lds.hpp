@@ -499,6 +499,15 @@ __device__ __forceinline__ float2 dct3_pre(const DctCtx &d, int k, float Xak, fl
   return cz;
 }
 
+// The DCT-III input value of slot k given its post-twiddle w = exp(-i pi k / 2N), both lines'
+// coefficients at k and at N - k.
+__device__ __forceinline__ void dct3_put(const DctCtx &d, int k, float2 w, float Xak, float Xbk, float Xan,
+                                         float Xbn) {
+  float2 cz = dct3_pre_w(w, Xak, Xbk, Xan, Xbn);
+  if (d.P->blue) cz = cmul(cz, d.tab[d.P->chirp_off + k]);
+  d.buf[pidx(k)] = cz;
+}
+
 // Stores the DCT-III input values of the pair (k, nk) given both lines' coefficients there.
 // Each pair of LDS slots is owned by one thread, so this can run in place over the spectrum.
 __device__ __forceinline__ void dct3_put_pair(const DctCtx &d, int k, int nk, float Xak, float Xbk, float Xan,

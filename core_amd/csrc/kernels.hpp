@@ -620,8 +620,8 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) void k_col(Ctx c, float *__restric
 
 // Row DCT-III: T -> out, with the partial sums of r.out when `dot` is set.
 template <int E>
-__global__ __launch_bounds__(MAX_FFT_THREADS) void k_row_dct3(Ctx c, const float *__restrict__ T_,
-                                                               float *__restrict__ out, int dot, int guard) {
+__global__ __launch_bounds__(MAX_FFT_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_row_dct3(
+    Ctx c, const float *__restrict__ T_, float *__restrict__ out, int dot, int guard) {
   extern __shared__ float2 lds[];
   const int img = find_img(c.geo.rowpair, c.n, blockIdx.x);
   const ImgDesc d = c.img[img];
@@ -633,35 +633,113 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) void k_row_dct3(Ctx c, const float
   long long ta, tb;
   t_rows(c, d, ra, ta, tb);
   const DctCtx D = dct_begin(lds, c.plans + d.rowplan, c.tab);
-  for (int k = threadIdx.x; k <= W / 2; k += T) {
-    const int nk = k ? W - k : 0;
-    const float Xak = T_[ta + k], Xan = T_[ta + nk];
-    const float Xbk = hasb ? T_[tb + k] : 0.f, Xbn = hasb ? T_[tb + nk] : 0.f;
-    dct3_put_pair(D, k, nk, Xak, Xbk, Xan, Xbn);
+  const float *r = c.f.r;
+  // float2 output path: samples 2i and 2i + 1 sit in the Makhoul slots i and W - 1 - i
+  const bool vec = !(W & 1) && !((reinterpret_cast<uintptr_t>(out) | reinterpret_cast<uintptr_t>(r)) & 7);
+  const unsigned J2 = W / 2;
+  const float2 *__restrict__ ra2 = reinterpret_cast<const float2 *>(r + oa);
+  const float2 *__restrict__ rb2 = reinterpret_cast<const float2 *>(r + oa + d.P);
+  // Input step over the pairs (j, W - j), 0 < j < (W + 1) / 2; slot 0 and, for even W, the
+  // self-paired slot W / 2 after the loop.  A thread issues the loads of DCT3_U pairs before it
+  // writes any of them (one memory latency for all); rows are uniform bases with 32-bit offsets, and
+  // the post-twiddle of slot W - j is exp(-i pi (W - j) / 2W) = (-w.y, -w.x) of w = exp(-i pi j / 2W).
+  const unsigned J = (W + 1) / 2;
+  const float *__restrict__ Ta = T_ + ta;
+  const float *__restrict__ Tb = hasb ? T_ + tb : Ta;
+  const float2 *__restrict__ mk = D.tab + D.P->mk_off;
+  const unsigned Jp = J > 1 ? J : 0;  // W <= 2: no pairs
+  for (unsigned j0 = threadIdx.x; j0 < Jp; j0 += DCT3_U * T) {
+    float xa[DCT3_U], xb[DCT3_U], ya[DCT3_U], yb[DCT3_U];
+    float2 wk[DCT3_U];
+#pragma unroll
+    for (int u = 0; u < DCT3_U; ++u) {
+      const unsigned j = min(max(j0 + u * T, 1u), J - 1);
+      xa[u] = Ta[j];
+      ya[u] = Ta[W - j];
+      xb[u] = Tb[j];
+      yb[u] = Tb[W - j];
+      wk[u] = mk[j];
+    }
+#pragma unroll
+    for (int u = 0; u < DCT3_U; ++u) {
+      const unsigned j = j0 + u * T;
+      if (j >= 1 && j < J) {
+        const float xbu = hasb ? xb[u] : 0.f, ybu = hasb ? yb[u] : 0.f;
+        dct3_put(D, j, wk[u], xa[u], xbu, ya[u], ybu);
+        dct3_put(D, W - j, make_float2(-wk[u].y, -wk[u].x), ya[u], ybu, xa[u], xbu);
+      }
+    }
+  }
+  if (threadIdx.x == 0) {
+    dct3_put(D, 0, mk[0], Ta[0], hasb ? Tb[0] : 0.f, 0.f, 0.f);
+    if (!(W & 1)) {
+      const float h = Ta[W / 2], hb = hasb ? Tb[W / 2] : 0.f;
+      dct3_put(D, W / 2, mk[W / 2], h, hb, h, hb);
+    }
+  }
+  // r of the first output batch, issued after the input step (a wait at the input loop's head would
+  // otherwise cover it) and before the transform, under which its latency hides (indices clamped so
+  // that every load is unconditional; a missing second line reads the first)
+  if (!hasb) rb2 = ra2;
+  float2 rav[DCT3_UO], rbv[DCT3_UO];
+  if (vec && dot) {
+#pragma unroll
+    for (int u = 0; u < DCT3_UO; ++u) {
+      const unsigned i = min(threadIdx.x + u * T, J2 - 1);
+      rav[u] = ra2[i];
+      rbv[u] = rb2[i];
+    }
   }
   dct3_run<E>(D);
   double acc[1] = {0.0};
-  const float *r = c.f.r;
-  constexpr int U = 4;
-  for (int m0 = threadIdx.x; m0 < W; m0 += U * T) {
-    float ra[U], rb[U];
-    if (dot) {
+  if (vec) {
+    float2 *__restrict__ oa2 = reinterpret_cast<float2 *>(out + oa);
+    float2 *__restrict__ ob2 = reinterpret_cast<float2 *>(out + oa + d.P);
+    const float s = D.P->invN;
+    for (unsigned i0 = threadIdx.x; i0 < J2; i0 += DCT3_UO * T) {
+      if (dot && i0 != threadIdx.x) {
+#pragma unroll
+        for (int u = 0; u < DCT3_UO; ++u) {
+          const unsigned i = min(i0 + u * T, J2 - 1);
+          rav[u] = ra2[i];
+          rbv[u] = rb2[i];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < DCT3_UO; ++u) {
+        const unsigned i = i0 + u * T;
+        if (i >= J2) continue;
+        const float2 w0 = dct_Z(D, i), w1 = dct_Z(D, W - 1 - i);
+        const float a0 = w0.x * s, b0 = -w0.y * s, a1 = w1.x * s, b1 = -w1.y * s;
+        oa2[i] = make_float2(a0, a1);
+        if (hasb) ob2[i] = make_float2(b0, b1);
+        if (dot)
+          acc[0] += ((double)rav[u].x * (double)a0 + (double)rav[u].y * (double)a1) +
+                    (hasb ? (double)rbv[u].x * (double)b0 + (double)rbv[u].y * (double)b1 : 0.0);
+      }
+    }
+  } else {
+    constexpr int U = 4;
+    for (int m0 = threadIdx.x; m0 < W; m0 += U * T) {
+      float rs[U], rt[U];
+      if (dot) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int m = m0 + u * T;
+          rs[u] = (m < W) ? r[oa + m] : 0.f;
+          rt[u] = (m < W && hasb) ? r[oa + d.P + m] : 0.f;
+        }
+      }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int m = m0 + u * T;
-        ra[u] = (m < W) ? r[oa + m] : 0.f;
-        rb[u] = (m < W && hasb) ? r[oa + d.P + m] : 0.f;
+        if (m >= W) break;
+        float a, b;
+        dct3_pair_value(D, m, a, b);
+        out[oa + m] = a;
+        if (hasb) out[oa + d.P + m] = b;
+        if (dot) acc[0] += (double)rs[u] * (double)a + (double)rt[u] * (double)b;
       }
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int m = m0 + u * T;
-      if (m >= W) break;
-      float a, b;
-      dct3_pair_value(D, m, a, b);
-      out[oa + m] = a;
-      if (hasb) out[oa + d.P + m] = b;
-      if (dot) acc[0] += (double)ra[u] * (double)a + (double)rb[u] * (double)b;
     }
   }
   if (dot) block_partial<1>(acc, c.partials);
