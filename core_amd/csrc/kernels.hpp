@@ -618,9 +618,101 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) void k_col(Ctx c, float *__restric
   }
 }
 
+// Input step of k_row_dct3 over the pairs (j, W - j), 0 < j < (W + 1) / 2, U per thread per
+// batch: a thread issues every load of a batch (both partners of both lines, the post-twiddle and,
+// for Bluestein, both chirps) before it writes any slot, so one memory latency covers the batch.
+// Rows are uniform bases with 32-bit offsets; the post-twiddle of slot W - j is
+// exp(-i pi (W - j) / 2W) = (-w.y, -w.x) of w = exp(-i pi j / 2W).
+template <int U, bool BLUE>
+__device__ __forceinline__ void dct3_input_pairs(const DctCtx &D, const float *__restrict__ Ta,
+                                                 const float *__restrict__ Tb, bool hasb, int W) {
+  const unsigned J = (W + 1) / 2, T = blockDim.x;
+  const float2 *__restrict__ mk = D.tab + D.P->mk_off;
+  const float2 *__restrict__ ch = D.tab + D.P->chirp_off;
+  for (unsigned j0 = threadIdx.x; j0 < J; j0 += U * T) {
+    float xa[U], xb[U], ya[U], yb[U];
+    float2 wk[U], cj[U], cn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned j = min(max(j0 + u * T, 1u), J - 1);
+      xa[u] = Ta[j];
+      ya[u] = Ta[W - j];
+      xb[u] = Tb[j];
+      yb[u] = Tb[W - j];
+      wk[u] = mk[j];
+      if (BLUE) {
+        cj[u] = ch[j];
+        cn[u] = ch[W - j];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned j = j0 + u * T;
+      if (j >= 1 && j < J) {
+        const float xbu = hasb ? xb[u] : 0.f, ybu = hasb ? yb[u] : 0.f;
+        float2 zk = dct3_pre_w(wk[u], xa[u], xbu, ya[u], ybu);
+        float2 zn = dct3_pre_w(make_float2(-wk[u].y, -wk[u].x), ya[u], ybu, xa[u], xbu);
+        if (BLUE) {
+          zk = cmul(zk, cj[u]);
+          zn = cmul(zn, cn[u]);
+        }
+        D.buf[pidx(j)] = zk;
+        D.buf[pidx(W - j)] = zn;
+      }
+    }
+  }
+}
+
+// Output step of k_row_dct3 for even W: samples 2i and 2i + 1 are the Makhoul slots i and
+// W - 1 - i (ascending and descending across the lanes: no bank conflicts), stored as float2.  The
+// r of the first batch is in (rav, rbv) when `first` is set (loaded before the transform); other
+// batches load their own.  For Bluestein the chirps of a batch are loaded together before any slot
+// is read.
+template <int U, bool BLUE>
+__device__ __forceinline__ void dct3_output_pairs(const DctCtx &D, float2 *__restrict__ oa2, float2 *__restrict__ ob2,
+                                                  const float2 *__restrict__ ra2, const float2 *__restrict__ rb2,
+                                                  float2 (&rav)[U], float2 (&rbv)[U], bool first, bool hasb, int dot,
+                                                  int W, double &acc) {
+  const unsigned J2 = W / 2, T = blockDim.x;
+  const float s = D.P->invN;
+  const float2 *__restrict__ ch = D.tab + D.P->chirp_off;
+  for (unsigned i0 = threadIdx.x; i0 < J2; i0 += U * T) {
+    float2 c0[U], c1[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned i = min(i0 + u * T, J2 - 1);
+      if (dot && (i0 != threadIdx.x || !first)) {
+        rav[u] = ra2[i];
+        rbv[u] = rb2[i];
+      }
+      if (BLUE) {
+        c0[u] = ch[i];
+        c1[u] = ch[W - 1 - i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned i = i0 + u * T;
+      if (i >= J2) continue;
+      float2 w0 = D.buf[pidx(i)], w1 = D.buf[pidx(W - 1 - i)];
+      if (BLUE) {
+        w0 = cmul(c0[u], cconj(w0));
+        w1 = cmul(c1[u], cconj(w1));
+      }
+      const float a0 = w0.x * s, b0 = -w0.y * s, a1 = w1.x * s, b1 = -w1.y * s;
+      oa2[i] = make_float2(a0, a1);
+      if (hasb) ob2[i] = make_float2(b0, b1);
+      if (dot)
+        acc += ((double)rav[u].x * (double)a0 + (double)rav[u].y * (double)a1) +
+               (hasb ? (double)rbv[u].x * (double)b0 + (double)rbv[u].y * (double)b1 : 0.0);
+    }
+  }
+}
+
 // Row DCT-III: T -> out, with the partial sums of r.out when `dot` is set.
+// E = 16 (lengths <= 8192): two workgroups per CU fit the LDS, so at most 128 VGPRs; E = 24: one.
 template <int E>
-__global__ __launch_bounds__(MAX_FFT_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_row_dct3(
+__global__ __launch_bounds__(MAX_FFT_THREADS) __attribute__((amdgpu_waves_per_eu(E == 16 ? 4 : 2))) void k_row_dct3(
     Ctx c, const float *__restrict__ T_, float *__restrict__ out, int dot, int guard) {
   extern __shared__ float2 lds[];
   const int img = find_img(c.geo.rowpair, c.n, blockIdx.x);
@@ -633,56 +725,33 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) __attribute__((amdgpu_waves_per_eu
   long long ta, tb;
   t_rows(c, d, ra, ta, tb);
   const DctCtx D = dct_begin(lds, c.plans + d.rowplan, c.tab);
+  const bool blue = D.P->blue;
   const float *r = c.f.r;
-  // float2 output path: samples 2i and 2i + 1 sit in the Makhoul slots i and W - 1 - i
-  const bool vec = !(W & 1) && !((reinterpret_cast<uintptr_t>(out) | reinterpret_cast<uintptr_t>(r)) & 7);
-  const unsigned J2 = W / 2;
-  const float2 *__restrict__ ra2 = reinterpret_cast<const float2 *>(r + oa);
-  const float2 *__restrict__ rb2 = reinterpret_cast<const float2 *>(r + oa + d.P);
-  // Input step over the pairs (j, W - j), 0 < j < (W + 1) / 2; slot 0 and, for even W, the
-  // self-paired slot W / 2 after the loop.  A thread issues the loads of DCT3_U pairs before it
-  // writes any of them (one memory latency for all); rows are uniform bases with 32-bit offsets, and
-  // the post-twiddle of slot W - j is exp(-i pi (W - j) / 2W) = (-w.y, -w.x) of w = exp(-i pi j / 2W).
-  const unsigned J = (W + 1) / 2;
+  // slot 0 and, for even W, the self-paired slot W / 2; then the pairs
   const float *__restrict__ Ta = T_ + ta;
   const float *__restrict__ Tb = hasb ? T_ + tb : Ta;
-  const float2 *__restrict__ mk = D.tab + D.P->mk_off;
-  const unsigned Jp = J > 1 ? J : 0;  // W <= 2: no pairs
-  for (unsigned j0 = threadIdx.x; j0 < Jp; j0 += DCT3_U * T) {
-    float xa[DCT3_U], xb[DCT3_U], ya[DCT3_U], yb[DCT3_U];
-    float2 wk[DCT3_U];
-#pragma unroll
-    for (int u = 0; u < DCT3_U; ++u) {
-      const unsigned j = min(max(j0 + u * T, 1u), J - 1);
-      xa[u] = Ta[j];
-      ya[u] = Ta[W - j];
-      xb[u] = Tb[j];
-      yb[u] = Tb[W - j];
-      wk[u] = mk[j];
-    }
-#pragma unroll
-    for (int u = 0; u < DCT3_U; ++u) {
-      const unsigned j = j0 + u * T;
-      if (j >= 1 && j < J) {
-        const float xbu = hasb ? xb[u] : 0.f, ybu = hasb ? yb[u] : 0.f;
-        dct3_put(D, j, wk[u], xa[u], xbu, ya[u], ybu);
-        dct3_put(D, W - j, make_float2(-wk[u].y, -wk[u].x), ya[u], ybu, xa[u], xbu);
-      }
-    }
-  }
   if (threadIdx.x == 0) {
+    const float2 *mk = D.tab + D.P->mk_off;
     dct3_put(D, 0, mk[0], Ta[0], hasb ? Tb[0] : 0.f, 0.f, 0.f);
     if (!(W & 1)) {
       const float h = Ta[W / 2], hb = hasb ? Tb[W / 2] : 0.f;
       dct3_put(D, W / 2, mk[W / 2], h, hb, h, hb);
     }
   }
-  // r of the first output batch, issued after the input step (a wait at the input loop's head would
-  // otherwise cover it) and before the transform, under which its latency hides (indices clamped so
-  // that every load is unconditional; a missing second line reads the first)
-  if (!hasb) rb2 = ra2;
+  if (W > 2) {
+    if (blue) dct3_input_pairs<DCT3_U / 2, true>(D, Ta, Tb, hasb, W);
+    else dct3_input_pairs<DCT3_U, false>(D, Ta, Tb, hasb, W);
+  }
+  // float2 output for even W and 8-byte aligned rows; r of the first output batch (not Bluestein:
+  // its output step needs the registers for the chirps) is issued here, after the input step (a wait at the input loop's head would otherwise cover it) and before the
+  // transform, under which its latency hides (index clamped so that every load is unconditional; a
+  // missing second line reads the first)
+  const bool vec = !(W & 1) && !((reinterpret_cast<uintptr_t>(out) | reinterpret_cast<uintptr_t>(r)) & 7);
+  const unsigned J2 = W / 2;
+  const float2 *__restrict__ ra2 = reinterpret_cast<const float2 *>(r + oa);
+  const float2 *__restrict__ rb2 = hasb ? reinterpret_cast<const float2 *>(r + oa + d.P) : ra2;
   float2 rav[DCT3_UO], rbv[DCT3_UO];
-  if (vec && dot) {
+  if (vec && dot && !blue) {
 #pragma unroll
     for (int u = 0; u < DCT3_UO; ++u) {
       const unsigned i = min(threadIdx.x + u * T, J2 - 1);
@@ -695,28 +764,11 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) __attribute__((amdgpu_waves_per_eu
   if (vec) {
     float2 *__restrict__ oa2 = reinterpret_cast<float2 *>(out + oa);
     float2 *__restrict__ ob2 = reinterpret_cast<float2 *>(out + oa + d.P);
-    const float s = D.P->invN;
-    for (unsigned i0 = threadIdx.x; i0 < J2; i0 += DCT3_UO * T) {
-      if (dot && i0 != threadIdx.x) {
-#pragma unroll
-        for (int u = 0; u < DCT3_UO; ++u) {
-          const unsigned i = min(i0 + u * T, J2 - 1);
-          rav[u] = ra2[i];
-          rbv[u] = rb2[i];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < DCT3_UO; ++u) {
-        const unsigned i = i0 + u * T;
-        if (i >= J2) continue;
-        const float2 w0 = dct_Z(D, i), w1 = dct_Z(D, W - 1 - i);
-        const float a0 = w0.x * s, b0 = -w0.y * s, a1 = w1.x * s, b1 = -w1.y * s;
-        oa2[i] = make_float2(a0, a1);
-        if (hasb) ob2[i] = make_float2(b0, b1);
-        if (dot)
-          acc[0] += ((double)rav[u].x * (double)a0 + (double)rav[u].y * (double)a1) +
-                    (hasb ? (double)rbv[u].x * (double)b0 + (double)rbv[u].y * (double)b1 : 0.0);
-      }
+    if (blue) {
+      float2 rav2[DCT3_UO / 2], rbv2[DCT3_UO / 2];
+      dct3_output_pairs<DCT3_UO / 2, true>(D, oa2, ob2, ra2, rb2, rav2, rbv2, false, hasb, dot, W, acc[0]);
+    } else {
+      dct3_output_pairs<DCT3_UO, false>(D, oa2, ob2, ra2, rb2, rav, rbv, true, hasb, dot, W, acc[0]);
     }
   } else {
     constexpr int U = 4;
