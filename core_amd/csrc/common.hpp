@@ -20,9 +20,20 @@ constexpr int TT = 64;
 // the transform kernels), <= 512 threads per workgroup (256 VGPRs per lane, no spills at E = 24).
 constexpr int MAX_EPT = 24;
 constexpr int MAX_FFT_THREADS = 512;
+// Compile-time boolean tag for generic lambdas.
+template <bool B>
+struct BoolC {
+  static constexpr bool value = B;
+};
+
 // Items per thread whose global loads k_row_dct3 issues together (input step, output step).
 constexpr int DCT3_U = 8;
 constexpr int DCT3_UO = 8;
+// Values per thread whose Bluestein tables (chirp, transformed chirp) are loaded together.
+constexpr int BLUE_U = 8;
+// Elements per thread whose global loads k_col issues together (load and store steps; half that
+// many coefficient pairs in its middle step).
+constexpr int COL_U = 8;
 constexpr int MAX_FFT_LEN = MAX_EPT * MAX_FFT_THREADS;  // 12288
 constexpr int MAX_PASS = 16;
 constexpr int MAX_NV = 3;  // values per block partial

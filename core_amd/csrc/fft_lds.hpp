@@ -419,11 +419,22 @@ template <int E>
 __device__ __forceinline__ void dct_fft(const DctCtx &d) {
   fft_lds<E>(d.buf, d.tws, &d.P->fft);
   if (d.P->blue) {
-    const float2 *bh = d.tab + d.P->bhat_off;
-    const int M = d.P->M;
-    for (int k = threadIdx.x; k < M; k += blockDim.x) {
-      const int pk = pidx(k);
-      d.buf[pk] = cconj(cmul(d.buf[pk], bh[k]));
+    // pointwise product with the transformed chirp; a thread loads BLUE_U of its values before it
+    // uses any (one memory latency per batch; index clamped so that every load is unconditional)
+    const float2 *__restrict__ bh = d.tab + d.P->bhat_off;
+    const unsigned M = d.P->M, T = blockDim.x;
+    for (unsigned k0 = threadIdx.x; k0 < M; k0 += BLUE_U * T) {
+      float2 b[BLUE_U];
+#pragma unroll
+      for (int u = 0; u < BLUE_U; ++u) b[u] = bh[min(k0 + u * T, M - 1)];
+#pragma unroll
+      for (int u = 0; u < BLUE_U; ++u) {
+        const unsigned k = k0 + u * T;
+        if (k < M) {
+          const int pk = pidx(k);
+          d.buf[pk] = cconj(cmul(d.buf[pk], b[u]));
+        }
+      }
     }
     __syncthreads();
     fft_lds<E>(d.buf, d.tws, &d.P->fft);

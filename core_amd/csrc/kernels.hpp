@@ -416,6 +416,184 @@ __device__ __forceinline__ void t_rows(const Ctx &c, const ImgDesc &d, int ra, l
   tb = ra + 1 < d.H ? d.off + (long long)slab_trow(makhoul_pos(ra + 1, d.H), L) * d.P : ta;
 }
 
+// The steps of k_col for a column pair of length H, templated on Bluestein.  Each issues the global
+// loads of a batch of U items per thread (data, chirps, float64 cosines, post-twiddles; indices
+// clamped so that every load is unconditional) before it uses any of them; the arithmetic is that
+// of dct2_pair_put / dct2_coef_pair / dct3_put_pair / dct3_pair_value.
+//
+// Load: element m of both lines (line b absent: zeros) into its Makhoul slot.  DIRECT reads one
+// float2 (or one float without line b) per row of the row-major field at pitch `stride`; otherwise
+// lines a and b are contiguous lines `stride` apart (rows of a field, or of the transposed field).
+template <int U, bool BLUE, bool DIRECT>
+__device__ __forceinline__ void lines_load(const DctCtx &D, const float *__restrict__ la, long long stride, bool hasb,
+                                         int H) {
+  const unsigned T = blockDim.x, Hu = H;
+  const float2 *__restrict__ ch = D.tab + D.P->chirp_off;
+  const float *__restrict__ lb = DIRECT ? la : (hasb ? la + stride : la);
+  for (unsigned m0 = threadIdx.x; m0 < Hu; m0 += U * T) {
+    float2 v[U], cm[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned m = min(m0 + u * T, Hu - 1);
+      if (DIRECT) {
+        const float *p = la + (long long)m * stride;
+        v[u] = hasb ? *(const float2 *)p : make_float2(p[0], 0.f);
+      } else {
+        v[u] = make_float2(la[m], lb[m]);
+      }
+      if (BLUE) cm[u] = ch[makhoul_pos(m, H)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned m = m0 + u * T;
+      if (m < Hu) {
+        float2 z = make_float2(v[u].x, hasb ? v[u].y : 0.f);
+        if (BLUE) z = cmul(z, cm[u]);
+        D.buf[pidx(makhoul_pos(m, H))] = z;
+      }
+    }
+  }
+}
+
+// Middle: for k = 0 .. H/2 and nk = (H - k) mod H, the DCT-II coefficients of both lines, divided by
+// -lambda (lambda rounded to float from float64 cosines; the hardware reciprocal, within 2 ulp of
+// the correctly rounded division; DC of line a = 0 for the first column), and the DCT-III input
+// values written back in place (each pair of slots is owned by one thread).
+template <int U, bool BLUE>
+__device__ __forceinline__ void col_middle(const DctCtx &D, const double *__restrict__ cosH, double cwa, double cwb,
+                                           bool hasb, bool first_col, int H) {
+  const unsigned T = blockDim.x, K = H / 2 + 1;
+  const float2 *__restrict__ mk = D.tab + D.P->mk_off;
+  const float2 *__restrict__ ch = D.tab + D.P->chirp_off;
+  auto scale = [&](unsigned k, double ck, float &Xa, float &Xb) {
+    const double c2 = 2.0 * ck;
+    const float lam_a = (float)(c2 + cwa - 4.0);
+    const float lam_b = (float)(c2 + cwb - 4.0);
+    Xa = (k == 0 && first_col) ? 0.f : -(Xa * __builtin_amdgcn_rcpf(lam_a));
+    Xb = hasb ? -(Xb * __builtin_amdgcn_rcpf(lam_b)) : 0.f;
+  };
+  for (unsigned k0 = threadIdx.x; k0 < K; k0 += U * T) {
+    float2 wk[U], wn[U], ck[U], cn[U];
+    double hk[U], hn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned k = min(k0 + u * T, K - 1), nk = k ? H - k : 0;
+      wk[u] = mk[k];
+      wn[u] = mk[nk];
+      hk[u] = cosH[k];
+      hn[u] = cosH[nk];
+      if (BLUE) {
+        ck[u] = ch[k];
+        cn[u] = ch[nk];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned k = k0 + u * T;
+      if (k >= K) continue;
+      const unsigned nk = k ? H - k : 0;
+      float2 Zk = D.buf[pidx(k)], Zn = D.buf[pidx(nk)];
+      if (BLUE) {
+        Zk = cmul(ck[u], cconj(Zk));
+        Zn = cmul(cn[u], cconj(Zn));
+      }
+      float Xak, Xbk, Xan, Xbn;
+      dct2_coef_w(Zk, Zn, wk[u], wn[u], Xak, Xbk, Xan, Xbn);
+      scale(k, hk[u], Xak, Xbk);
+      scale(nk, hn[u], Xan, Xbn);
+      if (k == 0) {
+        float2 z = dct3_pre_w(wk[u], Xak, Xbk, 0.f, 0.f);
+        if (BLUE) z = cmul(z, ck[u]);
+        D.buf[pidx(0)] = z;
+      } else {
+        float2 z = dct3_pre_w(wk[u], Xak, Xbk, Xan, Xbn);
+        if (BLUE) z = cmul(z, ck[u]);
+        D.buf[pidx(k)] = z;
+        if (nk != k) {
+          z = dct3_pre_w(wn[u], Xan, Xbn, Xak, Xbk);
+          if (BLUE) z = cmul(z, cn[u]);
+          D.buf[pidx(nk)] = z;
+        }
+      }
+    }
+  }
+}
+
+// Store: sample m of both lines (dct3_pair_value) back to the field.
+template <int U, bool BLUE, bool DIRECT>
+__device__ __forceinline__ void col_store(const DctCtx &D, float *__restrict__ la, long long stride, bool hasb,
+                                          int H) {
+  const unsigned T = blockDim.x, Hu = H;
+  const float2 *__restrict__ ch = D.tab + D.P->chirp_off;
+  const float s = D.P->invN;
+  for (unsigned m0 = threadIdx.x; m0 < Hu; m0 += U * T) {
+    float2 cm[U];
+    if (BLUE) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) cm[u] = ch[makhoul_pos(min(m0 + u * T, Hu - 1), H)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned m = m0 + u * T;
+      if (m >= Hu) continue;
+      float2 w = D.buf[pidx(makhoul_pos(m, H))];
+      if (BLUE) w = cmul(cm[u], cconj(w));
+      const float a = w.x * s, b = -w.y * s;
+      if (DIRECT) {
+        float *p = la + (long long)m * stride;
+        if (hasb) *(float2 *)p = make_float2(a, b);
+        else p[0] = a;
+      } else {
+        la[m] = a;
+        if (hasb) la[stride + m] = b;
+      }
+    }
+  }
+}
+
+// Coefficient store of the row DCT-II: for k = 0 .. W/2 and nk = (W - k) mod W, the DCT-II
+// coefficients of both lines (dct2_coef_pair) into the rows ta, tb of T; the post-twiddles and
+// chirps of a batch of U pairs are loaded before any is used.
+template <int U, bool BLUE>
+__device__ __forceinline__ void dct2_coef_store(const DctCtx &D, float *__restrict__ out, long long ta, long long tb,
+                                                bool hasb, int W) {
+  const unsigned T = blockDim.x, K = W / 2 + 1;
+  const float2 *__restrict__ mk = D.tab + D.P->mk_off;
+  const float2 *__restrict__ ch = D.tab + D.P->chirp_off;
+  for (unsigned k0 = threadIdx.x; k0 < K; k0 += U * T) {
+    float2 wk[U], wn[U], ck[U], cn[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned k = min(k0 + u * T, K - 1), nk = k ? W - k : 0;
+      wk[u] = mk[k];
+      wn[u] = mk[nk];
+      if (BLUE) {
+        ck[u] = ch[k];
+        cn[u] = ch[nk];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const unsigned k = k0 + u * T;
+      if (k >= K) continue;
+      const unsigned nk = k ? W - k : 0;
+      float2 Zk = D.buf[pidx(k)], Zn = D.buf[pidx(nk)];
+      if (BLUE) {
+        Zk = cmul(ck[u], cconj(Zk));
+        Zn = cmul(cn[u], cconj(Zn));
+      }
+      float Xak, Xbk, Xan, Xbn;
+      dct2_coef_w(Zk, Zn, wk[u], wn[u], Xak, Xbk, Xan, Xbn);
+      out[ta + k] = Xak;
+      if (hasb) out[tb + k] = Xbk;
+      if (nk != k) {
+        out[ta + nk] = Xan;
+        if (hasb) out[tb + nk] = Xbn;
+      }
+    }
+  }
+}
+
 template <bool FUSE_UPDATE, int E>
 __global__ __launch_bounds__(MAX_FFT_THREADS) void k_row_dct2(Ctx c, float *in_, float *__restrict__ out,
                                                                const float *__restrict__ p, int guard) {
@@ -439,60 +617,63 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) void k_row_dct2(Ctx c, float *in_,
     const float *Ap = c.f.Ap;
     const float alpha = (float)st->alpha;
     const int T = blockDim.x;
-    constexpr int U = 4;
-    for (int m0 = threadIdx.x; m0 < W; m0 += U * T) {
-      float xv[U][2], pv[U][2], rv[U][2], av[U][2];
+    // (BLUE: the chirps of the batch's Makhoul slots are loaded with it)
+    auto update = [&](auto blue_tag) {
+      constexpr bool BLUE = decltype(blue_tag)::value;
+      const float2 *__restrict__ ch = D.tab + D.P->chirp_off;
+      constexpr int U = 4;
+      for (int m0 = threadIdx.x; m0 < W; m0 += U * T) {
+        float xv[U][2], pv[U][2], rv[U][2], av[U][2];
+        float2 cm[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int m = m0 + u * T;
+        for (int u = 0; u < U; ++u) {
+          const int m = m0 + u * T;
+          if (BLUE) cm[u] = ch[makhoul_pos(min(m, W - 1), W)];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          xv[u][h] = pv[u][h] = rv[u][h] = av[u][h] = 0.f;
-          if (m < W && (h == 0 || hasb)) {
-            const long long k = oa + (long long)h * d.P + m;
-            xv[u][h] = x[k];
-            pv[u][h] = p[k];
-            rv[u][h] = in_[k];
-            av[u][h] = Ap[k];
+          for (int h = 0; h < 2; ++h) {
+            xv[u][h] = pv[u][h] = rv[u][h] = av[u][h] = 0.f;
+            if (m < W && (h == 0 || hasb)) {
+              const long long k = oa + (long long)h * d.P + m;
+              xv[u][h] = x[k];
+              pv[u][h] = p[k];
+              rv[u][h] = in_[k];
+              av[u][h] = Ap[k];
+            }
           }
         }
-      }
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int m = m0 + u * T;
-        if (m >= W) break;
-        float rn2[2];
+        for (int u = 0; u < U; ++u) {
+          const int m = m0 + u * T;
+          if (m >= W) break;
+          float rn2[2];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const float xn = xv[u][h] + alpha * pv[u][h];
-          const float rn = rv[u][h] - alpha * av[u][h];
-          rn2[h] = rn;
-          if (h == 0 || hasb) {
-            const long long k = oa + (long long)h * d.P + m;
-            x[k] = xn;
-            in_[k] = rn;
-            acc[0] += (double)rn * rn;
-            acc[1] += isfinite(xn) ? 0.0 : 1.0;
+          for (int h = 0; h < 2; ++h) {
+            const float xn = xv[u][h] + alpha * pv[u][h];
+            const float rn = rv[u][h] - alpha * av[u][h];
+            rn2[h] = rn;
+            if (h == 0 || hasb) {
+              const long long k = oa + (long long)h * d.P + m;
+              x[k] = xn;
+              in_[k] = rn;
+              acc[0] += (double)rn * rn;
+              acc[1] += isfinite(xn) ? 0.0 : 1.0;
+            }
           }
+          float2 z = make_float2(rn2[0], hasb ? rn2[1] : 0.f);
+          if (BLUE) z = cmul(z, cm[u]);
+          D.buf[pidx(makhoul_pos(m, W))] = z;
         }
-        dct2_pair_put(D, m, rn2[0], hasb ? rn2[1] : 0.f);
       }
-    }
-    dct2_pair_run<E>(D);
+    };
+    if (D.P->blue) update(BoolC<true>{});
+    else update(BoolC<false>{});
   } else {
-    dct2_pair_load<E>(D, in_ + oa, hasb ? in_ + oa + d.P : nullptr);
+    if (D.P->blue) lines_load<COL_U, true, false>(D, in_ + oa, d.P, hasb, W);
+    else lines_load<COL_U, false, false>(D, in_ + oa, d.P, hasb, W);
   }
-  for (int k = threadIdx.x; k <= W / 2; k += blockDim.x) {
-    const int nk = k ? W - k : 0;
-    float Xak, Xbk, Xan, Xbn;
-    dct2_coef_pair(D, k, nk, Xak, Xbk, Xan, Xbn);
-    out[ta + k] = Xak;
-    if (hasb) out[tb + k] = Xbk;
-    if (nk != k) {
-      out[ta + nk] = Xan;
-      if (hasb) out[tb + nk] = Xbn;
-    }
-  }
+  dct2_pair_run<E>(D);
+  if (D.P->blue) dct2_coef_store<COL_U / 2, true>(D, out, ta, tb, hasb, W);
+  else dct2_coef_store<COL_U / 2, false>(D, out, ta, tb, hasb, W);
   if (FUSE_UPDATE) block_partial<2>(acc, c.partials);
 }
 
@@ -559,63 +740,20 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) void k_col(Ctx c, float *__restric
   const bool hasb = ja + 1 < d.W;
   const int H = d.H;
   float *la = DIRECT ? Tt + d.off + ja : Tt + d.offT + (long long)ja * d.PT;
+  const long long stride = DIRECT ? d.P : d.PT;
   const DctCtx D = dct_begin(lds, c.plans + d.colplan, c.tab);
-  if (DIRECT) {
-    constexpr int U = 4;
-    const int T = blockDim.x;
-    for (int m0 = threadIdx.x; m0 < H; m0 += U * T) {
-      float2 v[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int m = m0 + u * T;
-        v[u] = make_float2(0.f, 0.f);
-        if (m < H) {
-          const float *p = la + (long long)m * d.P;
-          v[u] = hasb ? *(const float2 *)p : make_float2(p[0], 0.f);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (m0 + u * T < H) dct2_pair_put(D, m0 + u * T, v[u].x, v[u].y);
-    }
-    dct2_pair_run<E>(D);
-  } else {
-    dct2_pair_load<E>(D, la, hasb ? la + d.PT : nullptr);
-  }
+  const bool blue = D.P->blue;
+  if (blue) lines_load<COL_U, true, DIRECT>(D, la, stride, hasb, H);
+  else lines_load<COL_U, false, DIRECT>(D, la, stride, hasb, H);
+  dct2_pair_run<E>(D);
   const double *cosH = c.dtab + d.cosH_off;
   const double cwa = 2.0 * c.dtab[d.cosW_off + ja];
   const double cwb = hasb ? 2.0 * c.dtab[d.cosW_off + ja + 1] : 0.0;
-  const int T = blockDim.x;
-  // X / -lambda with lambda rounded to float from float64 cosines; the quotient uses the
-  // hardware reciprocal (within 2 ulp of the correctly rounded division).
-  auto scale = [&](int k, float &Xa, float &Xb) {
-    const double ch = 2.0 * cosH[k];
-    const float lam_a = (float)(ch + cwa - 4.0);
-    const float lam_b = (float)(ch + cwb - 4.0);
-    Xa = (k == 0 && ja == 0) ? 0.f : -(Xa * __builtin_amdgcn_rcpf(lam_a));
-    Xb = hasb ? -(Xb * __builtin_amdgcn_rcpf(lam_b)) : 0.f;
-  };
-  for (int k = threadIdx.x; k <= H / 2; k += T) {
-    const int nk = k ? H - k : 0;
-    float Xak, Xbk, Xan, Xbn;
-    dct2_coef_pair(D, k, nk, Xak, Xbk, Xan, Xbn);
-    scale(k, Xak, Xbk);
-    scale(nk, Xan, Xbn);
-    dct3_put_pair(D, k, nk, Xak, Xbk, Xan, Xbn);
-  }
+  if (blue) col_middle<COL_U / 2, true>(D, cosH, cwa, cwb, hasb, ja == 0, H);
+  else col_middle<COL_U / 2, false>(D, cosH, cwa, cwb, hasb, ja == 0, H);
   dct3_run<E>(D);
-  for (int m = threadIdx.x; m < H; m += T) {
-    float a, b;
-    dct3_pair_value(D, m, a, b);
-    if (DIRECT) {
-      float *p = la + (long long)m * d.P;
-      if (hasb) *(float2 *)p = make_float2(a, b);
-      else p[0] = a;
-    } else {
-      la[m] = a;
-      if (hasb) la[d.PT + m] = b;
-    }
-  }
+  if (blue) col_store<COL_U, true, DIRECT>(D, la, stride, hasb, H);
+  else col_store<COL_U, false, DIRECT>(D, la, stride, hasb, H);
 }
 
 // Input step of k_row_dct3 over the pairs (j, W - j), 0 < j < (W + 1) / 2, U per thread per
