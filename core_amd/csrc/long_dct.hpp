@@ -337,20 +337,17 @@ __global__ __launch_bounds__(LONG_THREADS) void k_long_p2(Ctx c, int dir, float 
     int pos[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int idx = i0 + u * T;
-      pos[u] = -1;
-      v[u] = make_float2(0.f, 0.f);
-      if (idx < (N2 << lb)) {
-        int q = (int)((float)idx * L.inv_n2);
-        int i = idx - q * N2;
-        if (i < 0) { --q; i += N2; }
-        if (i >= N2) { ++q; i -= N2; }
-        const int row = slot_index(q, C, ln.blk, N1, o);
-        if (row >= 0) {
-          v[u] = ln.S[row * N2 + i];
-          pos[u] = i * BS + q;
-        }
-      }
+      // unconditional loads: an index past the batch or an empty slot reads row 0 (not stored)
+      const int idx0 = i0 + u * T;
+      const bool in = idx0 < (N2 << lb);
+      const int idx = in ? idx0 : 0;
+      int q = (int)((float)idx * L.inv_n2);
+      int i = idx - q * N2;
+      if (i < 0) { --q; i += N2; }
+      if (i >= N2) { ++q; i -= N2; }
+      const int row = slot_index(q, C, ln.blk, N1, o);
+      v[u] = ln.S[(row >= 0 ? row : 0) * N2 + i];
+      pos[u] = (in && row >= 0) ? i * BS + q : -1;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
@@ -373,12 +370,10 @@ __global__ __launch_bounds__(LONG_THREADS) void k_long_p2(Ctx c, int dir, float 
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int k2 = q0 + u * rstep;
-        const int k = k1 + N1 * k2;
-        wk[u] = wn[u] = make_float2(1.f, 0.f);
-        if (k2 < k2_end && (k == 0 || 2 * k <= N)) {
-          wk[u] = mk[k];
-          wn[u] = mk[k ? N - k : 0];
-        }
+        const int kr = k1 + N1 * k2;
+        const int k = (k2 < k2_end && (kr == 0 || 2 * kr <= N)) ? kr : 0;  // unconditional loads
+        wk[u] = mk[k];
+        wn[u] = mk[k ? N - k : 0];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
@@ -400,15 +395,18 @@ __global__ __launch_bounds__(LONG_THREADS) void k_long_p2(Ctx c, int dir, float 
       }
     } else {
       float2 ra[U], rb[U];
+      if (dot) {
+        // unconditional loads: an element past the canonical ones reads element k1 (2 k1 < N), a
+        // missing line b reads line a; neither is used
+        const float *r = c.f.r + ln.la;
+        const float *rbp = ln.hasb ? r + ln.pitch : r;
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int k2 = q0 + u * rstep;
-        const int n = k1 + N1 * k2;
-        ra[u] = rb[u] = make_float2(0.f, 0.f);
-        if (dot && k2 < k2_end && 2 * n < N) {
-          const float *r = c.f.r + ln.la;
-          ra[u] = ld2(r, 2 * (long long)n);
-          if (ln.hasb) rb[u] = ld2(r + ln.pitch, 2 * (long long)n);
+        for (int u = 0; u < U; ++u) {
+          const int k2 = q0 + u * rstep;
+          const int n = k1 + N1 * k2;
+          const long long m = 2 * (long long)((k2 < k2_end && 2 * n < N) ? n : k1);
+          ra[u] = ld2(r, m);
+          rb[u] = ld2(rbp, m);
         }
       }
 #pragma unroll
