@@ -77,6 +77,35 @@ def test_slab_fast_preconditioner_matches_scipy_fp64(hip, shapes):
     ws.release()
 
 
+def test_preconditioner_unaligned_buffers(hip):
+    """Input and output arenas at a 4-byte (not 8-byte) offset: the row DCT-III then stores one
+    float at a time instead of float2 pairs, with the same arithmetic, so z is bitwise equal to the
+    aligned call's; widths even and odd, power-of-two, 5-smooth and Bluestein, slab and transposed
+    columns."""
+    import torch
+    from core_amd.unwrap2d import Workspace, image_view, pack, precondition_device
+    shapes = [(64, 48), (127, 211), (300, 642), (256, 256), (96, 130)]
+    rng = np.random.default_rng(13)
+    rs = [rng.normal(size=s).astype(np.float32) for s in shapes]
+    ws = Workspace(shapes, hip)
+    r = pack(rs, ws, hip)
+    print(f"preconditioner, aligned and 4-byte offset arenas {shapes}", flush=True)
+    z = precondition_device(r, shapes, workspace=ws)
+    rbig = torch.zeros(ws.total + 1, dtype=torch.float32, device=hip)
+    zbig = torch.full((ws.total + 1,), float("nan"), dtype=torch.float32, device=hip)
+    r1, z1 = rbig[1:], zbig[1:]
+    r1.copy_(r)
+    assert r1.data_ptr() % 8 == 4 and z1.data_ptr() % 8 == 4
+    precondition_device(r1, shapes, out=z1, workspace=ws)
+    zh, z1h = z.cpu().numpy(), z1.cpu().numpy()
+    for k, ((h, w), rk) in enumerate(zip(shapes, rs)):
+        a, b = image_view(zh, ws, k), image_view(z1h, ws, k)
+        assert np.array_equal(a, b), (h, w)
+        z64 = _precond64(rk)
+        assert np.abs(a.astype(np.float64) - z64).max() / np.abs(z64).max() < 2e-5, (h, w)
+    ws.release()
+
+
 @pytest.mark.parametrize("H,W,seed,fault,mask", [(96, 130, 1, True, 0.05), (250, 33, 2, False, 0.0),
                                                  (480, 642, 3, True, 0.0), (300, 1, 4, False, 0.0),
                                                  (512, 300, 5, True, 0.05), (1024, 96, 6, False, 0.0)])
