@@ -306,6 +306,20 @@ __device__ __forceinline__ void march_rows(const Ctx &c, const ImgDesc &d, const
   const bool own = c0 < W;
   float4 xu = X(t.i0 - 1), xc = X(t.i0);
   float4 wyu = (own && t.i0 > 0) ? ld4(wyi, d.off + (long long)(t.i0 - 1) * d.P + c0) : z4;
+  // The strip's edge columns for all its rows, loaded once: lane e < TILE_H holds the operand left
+  // of the strip in row i0 + e, lane TILE_H + e the one right of it, lane 2 TILE_H + e the x-weight
+  // left of it; row i reads them with readlane (a uniform lane index).
+  static_assert(3 * TILE_H <= 64, "edge columns of a tile fit one wave");
+  float edge = 0.f;
+  {
+    const int e = lane % TILE_H, grp = lane / TILE_H, ie = t.i0 + e;
+    if (ie < iend) {
+      if (grp == 0) edge = XE(ie, jw - 1);
+      else if (grp == 1) edge = XE(ie, jw + STRIP);
+      else if (grp == 2 && jw > 0) edge = wxi[d.off + (long long)ie * d.P + jw - 1];
+    }
+  }
+  const int edge_bits = __float_as_int(edge);
   for (int i = t.i0; i < iend; ++i) {
     const long long g = d.off + (long long)i * d.P + c0;
     const float4 xd = X(i + 1);
@@ -313,11 +327,12 @@ __device__ __forceinline__ void march_rows(const Ctx &c, const ImgDesc &d, const
     float xl = __shfl_up(xc.w, 1, 64);
     float wxl = __shfl_up(wx.w, 1, 64);
     float xr = __shfl_down(xc.x, 1, 64);
+    const int e = i - t.i0;
     if (lane == 0) {
-      xl = XE(i, jw - 1);
-      wxl = (jw > 0) ? wxi[d.off + (long long)i * d.P + jw - 1] : 0.f;
+      xl = __int_as_float(__builtin_amdgcn_readlane(edge_bits, e));
+      wxl = __int_as_float(__builtin_amdgcn_readlane(edge_bits, 2 * TILE_H + e));
     }
-    if (lane == 63) xr = XE(i, jw + STRIP);
+    if (lane == 63) xr = __int_as_float(__builtin_amdgcn_readlane(edge_bits, TILE_H + e));
     const float4 ax = apply_A4(i, c0, H, W, xc, xl, xr, xu, xd, wx, wxl, wyu, wy);
     if (own) E(i, g, xc, ax);
     xu = xc;
