@@ -15,8 +15,12 @@
  *   - Images are packed in "arenas": image i occupies H[i] rows of pitches[i] elements starting
  *     at element offsets[i], row-major (pixel (y, x) at offsets[i] + y * pitches[i] + x); both
  *     come from irls2d_arena_layout: pitches[i] is W[i] rounded up to a multiple of 64 and every
- *     offset is a multiple of 64, so rows are 256-byte aligned.  Padding elements are ignored on
- *     input and unspecified on output.
+ *     offset is a multiple of 64, so rows are 256-byte aligned relative to the arena base.
+ *     Padding elements are ignored on input and unspecified on output.
+ *   - Caller float32 arrays need only 4-byte alignment: the kernels make float2 / float4
+ *     accesses on the library's own workspace, and on a caller float32 array only after checking
+ *     its address (irls2d_precondition's output).  Complex64 arrays (irls2d_goldstein) need
+ *     their natural 8-byte alignment.
  *   - phase: float32 radians, NaN marks invalid pixels (the crop is done by the caller).
  *   - weight: float32 quality (NaN allowed) or NULL for uniform weights.
  *   - unwrapped: float32 output arena (the raw solver output u, before the host epilogue).
