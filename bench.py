@@ -13,7 +13,8 @@ Also reported:
                 WRITE_SIZE, copy-calibrated) from the committed profile, when present.
   solve_model   whole-solve algorithmic bytes N [100 + 10 (109 + 50 * 76)] / step time.
   batched       48 uneven SBAS-burst interferograms (H ~ U{1000..1500}, W ~ U{4000..6000},
-                seeds 1000 + i) solved in one batched graph launch vs 48 sequential plain calls.
+                seeds 1000 + i) solved in one batched graph launch vs 48 sequential plain calls
+                (--only-batched: this workload alone, with per-kernel durations).
   adjoint       16 x 2048 x 2048 forward solve and implicit adjoint (dL/dphase, dL/dweight).
   filters       the upstream Goldstein (psize 32) and Gaussian (sigma 2) filters on an 8192 x 8192
                 complex64 interferogram, with their numpy restatements timed on a 2048 x 2048
@@ -42,6 +43,7 @@ ALG_BPX = {
     "k_cg_pap": 24,           # z, p_old, wxi, wyi -> p_new, Ap
     "k_row_dct2_update": 28,  # x, p, r, Ap -> x, r, T
     "k_col": 8,               # Tt -> Tt
+    "k_col_direct": 8,        # T -> T, column pairs read and written in place
     "k_row_dct3": 12,         # T, r -> z
     "k_transpose": 8,         # T -> Tt / Tt -> T
     # column pass on slabs of the row-major field: T (4 B/px real) <-> S (one complex per pixel
@@ -141,9 +143,9 @@ def roofline(stats, npx):
     }
 
 
-def batched_stack(args, dev):
+def batched_stack(args, dev, kernels=False):
     import torch
-    from core_amd import solve_device, synthetic as S
+    from core_amd import solve_device, synthetic as S, _lib
     from core_amd.unwrap2d import Workspace
     wls = S.stack_48()
     ims = [S.generate(wl)[:2] for wl in wls]
@@ -156,6 +158,17 @@ def batched_stack(args, dev):
     out = torch.empty(ws.total, dtype=torch.float32, device=dev)
     ms_b, _ = time_steps(lambda: solve_device(ph, co, shapes, graph=True, out=out, workspace=ws, **kw),
                          args.batched_steps, 1)
+    kstats = None
+    if kernels:
+        # per-kernel durations of one event-timed eager batched solve
+        _lib.kernel_stats_reset()
+        solve_device(ph, co, shapes, graph=False, time_kernels=True, out=out, workspace=ws, **kw)
+        kstats = {}
+        for k, (tot, n) in sorted(_lib.kernel_stats().items(), key=lambda kv: -kv[1][0]):
+            e = {"total_ms": round(tot, 1), "launches": n, "mean_us": round(tot / n * 1e3, 1)}
+            if k in ALG_BPX:
+                e["gbs"] = round(ALG_BPX[k] * npx / (tot / n / 1e3) / 1e9, 1)
+            kstats[k] = e
     ws.release()
     del ws, ph, co, out
     torch.cuda.empty_cache()
@@ -175,7 +188,7 @@ def batched_stack(args, dev):
         w1.release()
     singles.clear()
     torch.cuda.empty_cache()
-    return {
+    res = {
         "workload": "48 SBAS bursts H~U{1000..1500} W~U{4000..6000}, seeds 1000+i, 10 IRLS x 50 PCG",
         "pixels": npx,
         "batched_ms": round(ms_b, 2), "batched_mpx_s": round(npx / ms_b / 1e3, 2),
@@ -183,6 +196,9 @@ def batched_stack(args, dev):
         "speedup": round(ms_s / ms_b, 3),
         "solve_model_frac_hbm": round(npx * model_bytes_per_px() / (ms_b / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
     }
+    if kstats is not None:
+        res["kernels"] = kstats
+    return res
 
 
 def adjoint_batch(args, dev):
@@ -407,6 +423,8 @@ def main():
     ap.add_argument("--max-scale", action="store_true",
                     help="also solve the 40000 x 40000 workload (about 4 minutes, ~140 GB of HBM)")
     ap.add_argument("--only-max-scale", action="store_true", help="run only the 40000 x 40000 workload")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run only the 48-burst batched stack, with per-kernel durations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -423,6 +441,9 @@ def main():
     n_tuned = autotune.apply()  # plans measured by `python -m core_amd.autotune` on this GPU type
     if args.only_max_scale:
         print(json.dumps({"max_scale": max_scale(args, dev)}), flush=True)
+        return
+    if args.only_batched:
+        print(json.dumps({"batched": batched_stack(args, dev, kernels=True)}), flush=True)
         return
     npx = 8192 * 8192
     psi, coh, ms, wall_ms, info, stats = plain_8192(args, dev)
