@@ -17,7 +17,7 @@ Candidates per length N (plan = fft_len, radix order, elements per thread):
   * Bluestein (prime factors > 5): the three 5-smooth lengths M >= 2N - 1 of lowest
     M * (passes + 1) and the lengths with compile-time schedules in that range, each with its
     default order;
-  * 16 or 24 elements per thread where the FFT length allows both.
+  * 12 (on up to 1024 threads), 16 or 24 elements per thread where the FFT length allows them.
 """
 from __future__ import annotations
 
@@ -102,7 +102,7 @@ def candidates(n: int):
                 scheds.append(odd + p2)
             if m != n:  # Bluestein: one order per length is enough
                 break
-        epts = [16, 24] if m <= 8192 else [24]
+        epts = [16, 12, 24] if m <= 8192 else [12, 24]
         seen = set()
         for sch in scheds:
             if len(sch) > 16:
@@ -217,9 +217,10 @@ def tune(shapes, device="cuda", reps=5, path=DEFAULT_FILE, save=True, verbose=Fa
             chosen[n] = {"fft_len": best[0], "radices": best[1], "ept": best[2], "ms": round(best_ms or 0.0, 4)}
             if verbose or (i + 1) % 10 == 0 or i + 1 == len(lengths):
                 print(f"autotune: {i + 1}/{len(lengths)} lengths ({time.time() - t0:.0f} s)", flush=True)
-        # one elements-per-thread value per launch: keep the per-length choices, or force 16 / 24
+        # one elements-per-thread value per launch: keep the per-length choices, or force one
         variants = {"per_length": {n: p["ept"] for n, p in chosen.items()},
-                    "all_16": {n: (16 if p["fft_len"] <= 8192 else 24) for n, p in chosen.items()},
+                    "all_16": {n: (16 if p["fft_len"] <= 8192 else 12) for n, p in chosen.items()},
+                    "all_12": {n: 12 for n in chosen},
                     "all_24": {n: 24 for n in chosen}}
         best_v, best_ms = None, None
         for name, epts in variants.items():

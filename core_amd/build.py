@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 # translation units, compiled in parallel: the host code and small kernels, and the transform
 # kernel instantiations (the slow ones to compile) in three units of their own
-SOURCES = ["irls2d.hip", "fft_e16.hip", "fft_e24.hip", "long_passes.hip", "goldstein.hip"]
+SOURCES = ["irls2d.hip", "fft_e12.hip", "fft_e16.hip", "fft_e24.hip", "long_passes.hip", "goldstein.hip"]
 SRC = os.path.join(CSRC, "irls2d.hip")
 OUT = os.path.join(HERE, "libirls2d.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -16,10 +16,15 @@ constexpr int TILE_THREADS = 256;
 constexpr int NUM_XCD = 8;  // workgroups are dispatched round-robin over the 8 XCDs of an MI355X
 // Transpose tile
 constexpr int TT = 64;
-// LDS FFT: at most E complex values per thread per pass (E = 16 or 24, a template parameter of
-// the transform kernels), <= 512 threads per workgroup (256 VGPRs per lane, no spills at E = 24).
+// LDS FFT: at most E complex values per thread per pass (E = 12, 16 or 24, a template parameter
+// of the transform kernels).  E = 16 and 24 run <= 512 threads per workgroup (256 VGPRs per lane,
+// no spills at E = 24); E = 12 runs <= 1024, which doubles the waves of a workgroup whose LDS
+// (lengths above 8192) leaves room for only one per CU.
 constexpr int MAX_EPT = 24;
 constexpr int MAX_FFT_THREADS = 512;
+constexpr int MAX_FFT_THREADS_WIDE = 1024;
+template <int E>
+constexpr int fft_threads_max() { return E == 12 ? MAX_FFT_THREADS_WIDE : MAX_FFT_THREADS; }
 // Compile-time boolean tag for generic lambdas.
 template <bool B>
 struct BoolC {
@@ -226,10 +231,12 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Block-wide sum of NV doubles; every thread receives the totals.  All threads must call.
-template <int NV>
+// Block-wide sum of NV doubles; every thread receives the totals.  All threads must call.  MW:
+// the most waves a workgroup of the calling kernel has (the array is sized for it only: a larger
+// one made the compiler spill in the 8192-point row kernel).
+template <int NV, int MW = MAX_FFT_THREADS / 64>
 __device__ __forceinline__ void block_sum(double (&v)[NV]) {
-  __shared__ double sh[NV][MAX_FFT_THREADS / 64];
+  __shared__ double sh[NV][MW];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
 #pragma unroll
   for (int q = 0; q < NV; ++q) v[q] = wave_sum(v[q]);
@@ -249,18 +256,18 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 
 // First stage of a deterministic two-stage reduction: the block total of each value goes to
 // partials[blockIdx.x * MAX_NV + q]; k_finalize sums an image's partials in block order.
-template <int NV>
+template <int NV, int MW = MAX_FFT_THREADS / 64>
 __device__ __forceinline__ void block_partial(double (&v)[NV], double *partials, int slot) {
-  block_sum<NV>(v);
+  block_sum<NV, MW>(v);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) partials[(long long)slot * MAX_NV + q] = v[q];
   }
 }
 
-template <int NV>
+template <int NV, int MW = MAX_FFT_THREADS / 64>
 __device__ __forceinline__ void block_partial(double (&v)[NV], double *partials) {
-  block_partial<NV>(v, partials, blockIdx.x);
+  block_partial<NV, MW>(v, partials, blockIdx.x);
 }
 
 __device__ __forceinline__ float clampf_nan(float v, float lo, float hi) {

@@ -371,7 +371,7 @@ __device__ __forceinline__ void fft_lds(float2 *buf, const float2 *tws, const Ff
     case 8192: fft_lds_ct<8192, E>(buf, tws); return;
     default: break;
   }
-  if constexpr (E >= 24) {
+  if constexpr (E >= 24 || E == 12) {  // 12 x 1024 and 24 x 512 threads reach 12288
     switch (P->ct_len) {
       case 10240: fft_lds_ct<10240, E>(buf, tws); return;
       case 12288: fft_lds_ct<12288, E>(buf, tws); return;

@@ -1,5 +1,5 @@
-// The transform kernels are instantiated in translation units of their own (fft_e16.hip,
-// fft_e24.hip, long_passes.hip) so that the build compiles them in parallel; the host
+// The transform kernels are instantiated in translation units of their own (fft_e12.hip,
+// fft_e16.hip, fft_e24.hip, long_passes.hip) so that the build compiles them in parallel; the host
 // translation unit (irls2d.hip) only declares these instantiations.
 #pragma once
 

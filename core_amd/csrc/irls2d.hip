@@ -23,7 +23,8 @@
 #include "instances.hpp"
 
 namespace irls2d {
-// defined in fft_e16.hip, fft_e24.hip and long_passes.hip
+// defined in fft_e12.hip, fft_e16.hip, fft_e24.hip and long_passes.hip
+IRLS2D_FFT_KERNELS(12, extern)
 IRLS2D_FFT_KERNELS(16, extern)
 IRLS2D_FFT_KERNELS(24, extern)
 IRLS2D_LONG_KERNELS(extern)
@@ -179,7 +180,10 @@ int supported(int64_t N) {
   return bluestein_len(N) <= MAX_FFT_LEN;
 }
 
-static int default_ept(int M) { return M <= 16 * MAX_FFT_THREADS ? 16 : 24; }
+// 16 values per thread up to 8192 points (two workgroups of 8 waves per CU); above, 12 on up to
+// 1024 threads (one workgroup of 16 waves: the LDS leaves room for one).
+static int default_ept(int M) { return M <= 16 * MAX_FFT_THREADS ? 16 : 12; }
+static int ept_threads_max(int E) { return E == 12 ? MAX_FFT_THREADS_WIDE : MAX_FFT_THREADS; }
 
 static std::shared_ptr<HostDct> get_plan(int N) {
   std::lock_guard<std::mutex> lk(g_plan_mu);
@@ -455,10 +459,13 @@ static int64_t arena_layout(int n, const int64_t *H, const int64_t *W, int64_t *
 // Elements per thread (16 or 24: the largest any plan of the launch asks for, and enough for L
 // within MAX_FFT_THREADS threads) and threads per workgroup for transforms of length <= L.
 static void fft_shape(int L, int ept, int &E, int &T) {
-  E = std::max(ept, default_ept(L));
+  // ept: the plan's choice for the longest transform of the launch (0 = default); 16 cannot
+  // cover lengths above 8192
+  E = ept ? ept : default_ept(L);
+  if (E * ept_threads_max(E) < L) E = default_ept(L);
   int t = (L + E - 1) / E;
   t = (t + 63) / 64 * 64;
-  T = std::max(64, std::min(MAX_FFT_THREADS, t));
+  T = std::max(64, std::min(ept_threads_max(E), t));
 }
 
 static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
@@ -567,7 +574,8 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
       if (auto_slab(H[i], W[i], f1, f2)) { auto_f1 = f1; auto_f2 = f2; }
     }
   }
-  int maxM_row = 1, maxM_col = 1, ept_row = 16, ept_col = 16;
+  // one elements-per-thread value per launch: that of the plan with the longest transform
+  int maxM_row = 1, maxM_col = 1, ept_row = 0, ept_col = 0;
   for (int i = 0; i < n; ++i) {
     const int h = (int)H[i], w = (int)W[i];
     ImgDesc d;
@@ -611,13 +619,13 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     d.offS = Lo.totalS;
     Lo.totalS += (sz + 31) / 32 * 32;
     Lo.desc.push_back(d);
-    if (!lw) {
-      maxM_row = std::max(maxM_row, Lo.plans[d.rowplan].M);
-      ept_row = std::max(ept_row, Lo.plans[d.rowplan].ept);
+    if (!lw && Lo.plans[d.rowplan].M > maxM_row) {
+      maxM_row = Lo.plans[d.rowplan].M;
+      ept_row = Lo.plans[d.rowplan].ept;
     }
-    if (!lh && !slab) {
-      maxM_col = std::max(maxM_col, Lo.plans[d.colplan].M);
-      ept_col = std::max(ept_col, Lo.plans[d.colplan].ept);
+    if (!lh && !slab && Lo.plans[d.colplan].M > maxM_col) {
+      maxM_col = Lo.plans[d.colplan].M;
+      ept_col = Lo.plans[d.colplan].ept;
     }
     const int64_t tiles = ((int64_t)(h + TILE_H - 1) / TILE_H) * ((w + TILE_W - 1) / TILE_W);
     const int64_t ttiles = slab ? 0 : ((int64_t)(h + TT - 1) / TT) * ((w + TT - 1) / TT);
@@ -863,9 +871,11 @@ static void row_dct2(const Layout &Lo, const Ctx &c, bool fuse, float *src, floa
   const dim3 g(Lo.geo[1].back()), b(Lo.row_threads);
   if (fuse) {
     if (Lo.row_E == 16) LAUNCH("k_row_dct2_update", (k_row_dct2<true, 16>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
+    else if (Lo.row_E == 12) LAUNCH("k_row_dct2_update", (k_row_dct2<true, 12>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
     else LAUNCH("k_row_dct2_update", (k_row_dct2<true, 24>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
   } else {
     if (Lo.row_E == 16) LAUNCH("k_row_dct2", (k_row_dct2<false, 16>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
+    else if (Lo.row_E == 12) LAUNCH("k_row_dct2", (k_row_dct2<false, 12>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
     else LAUNCH("k_row_dct2", (k_row_dct2<false, 24>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
   }
 }
@@ -878,6 +888,7 @@ static void row_dct3(const Layout &Lo, const Ctx &c, const float *src, float *ds
   if (Lo.geo[1].back() == 0) return;
   const dim3 g(Lo.geo[1].back()), b(Lo.row_threads);
   if (Lo.row_E == 16) LAUNCH("k_row_dct3", (k_row_dct3<16>), g, b, Lo.row_lds, s, c, src, dst, dot, guard);
+  else if (Lo.row_E == 12) LAUNCH("k_row_dct3", (k_row_dct3<12>), g, b, Lo.row_lds, s, c, src, dst, dot, guard);
   else LAUNCH("k_row_dct3", (k_row_dct3<24>), g, b, Lo.row_lds, s, c, src, dst, dot, guard);
 }
 
@@ -922,9 +933,11 @@ static void col_pass(const Layout &Lo, const Ctx &c, int guard, hipStream_t s) {
   const dim3 g(Lo.geo[2].back()), b(Lo.col_threads);
   if (Lo.col_direct) {
     if (Lo.col_E == 16) LAUNCH("k_col_direct", (k_col<16, true>), g, b, Lo.col_lds, s, c, c.f.T, guard);
+    else if (Lo.col_E == 12) LAUNCH("k_col_direct", (k_col<12, true>), g, b, Lo.col_lds, s, c, c.f.T, guard);
     else LAUNCH("k_col_direct", (k_col<24, true>), g, b, Lo.col_lds, s, c, c.f.T, guard);
   } else {
     if (Lo.col_E == 16) LAUNCH("k_col", (k_col<16, false>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
+    else if (Lo.col_E == 12) LAUNCH("k_col", (k_col<12, false>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
     else LAUNCH("k_col", (k_col<24, false>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
   }
 }
@@ -1053,7 +1066,10 @@ static int set_attrs() {
                        (const void *)k_row_dct2<true, 16>,  (const void *)k_row_dct2<true, 24>,
                        (const void *)k_col<16, false>,      (const void *)k_col<24, false>,
                        (const void *)k_col<16, true>,       (const void *)k_col<24, true>,
-                       (const void *)k_row_dct3<16>,        (const void *)k_row_dct3<24>};
+                       (const void *)k_row_dct3<16>,        (const void *)k_row_dct3<24>,
+                       (const void *)k_row_dct2<false, 12>, (const void *)k_row_dct2<true, 12>,
+                       (const void *)k_col<12, false>,      (const void *)k_col<12, true>,
+                       (const void *)k_row_dct3<12>};
   for (const void *f : fns) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   const int llds = LONG_LDS * (int)sizeof(float2);
   const void *lfns[] = {(const void *)k_long_p1<LONG_DCT2, LONG_E>, (const void *)k_long_p1<LONG_DCT2_UPDATE, LONG_E>,
@@ -1257,7 +1273,7 @@ int plan_override(int64_t N, int64_t fft_len, const int *rad, int nrad, int ept)
       return fail(IRLS2D_EINVAL, "fft_len must be 5-smooth, >= N and <= 12288");
     if (fft_len != N && fft_len < 2 * N - 1) return fail(IRLS2D_EINVAL, "a Bluestein length must be >= 2N - 1");
     if (fft_len == N && !smooth235(N)) return fail(IRLS2D_EINVAL, "N has a prime factor > 5: needs Bluestein");
-    if (ept != 0 && ept != 16 && ept != 24) return fail(IRLS2D_EINVAL, "ept must be 0, 16 or 24");
+    if (ept != 0 && ept != 12 && ept != 16 && ept != 24) return fail(IRLS2D_EINVAL, "ept must be 0, 12, 16 or 24");
     if (ept == 16 && fft_len > 16 * MAX_FFT_THREADS) return fail(IRLS2D_EINVAL, "ept 16 covers lengths <= 8192");
     PlanOverride o;
     o.fft_len = (int)fft_len;

@@ -610,7 +610,7 @@ __device__ __forceinline__ void dct2_coef_store(const DctCtx &D, float *__restri
 }
 
 template <bool FUSE_UPDATE, int E>
-__global__ __launch_bounds__(MAX_FFT_THREADS) void k_row_dct2(Ctx c, float *in_, float *__restrict__ out,
+__global__ __launch_bounds__(fft_threads_max<E>()) void k_row_dct2(Ctx c, float *in_, float *__restrict__ out,
                                                                const float *__restrict__ p, int guard) {
   extern __shared__ float2 lds[];
   const int img = find_img(c.geo.rowpair, c.n, blockIdx.x);
@@ -689,7 +689,7 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) void k_row_dct2(Ctx c, float *in_,
   dct2_pair_run<E>(D);
   if (D.P->blue) dct2_coef_store<COL_U / 2, true>(D, out, ta, tb, hasb, W);
   else dct2_coef_store<COL_U / 2, false>(D, out, ta, tb, hasb, W);
-  if (FUSE_UPDATE) block_partial<2>(acc, c.partials);
+  if (FUSE_UPDATE) block_partial<2, fft_threads_max<E>() / 64>(acc, c.partials);
 }
 
 #ifndef IRLS2D_TRANSFORMS_ONLY
@@ -745,7 +745,7 @@ __global__ __launch_bounds__(256) void k_transpose(Ctx c, const float *__restric
 // row) instead of the transposed copy; blocks are remapped so that each XCD takes a contiguous
 // range of column pairs, whose rows then share that XCD's L2 lines.
 template <int E, bool DIRECT>
-__global__ __launch_bounds__(MAX_FFT_THREADS) void k_col(Ctx c, float *__restrict__ Tt, int guard) {
+__global__ __launch_bounds__(fft_threads_max<E>()) void k_col(Ctx c, float *__restrict__ Tt, int guard) {
   extern __shared__ float2 lds[];
   const int blk = DIRECT ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int img = find_img(c.geo.colpair, c.n, blk);
@@ -863,9 +863,10 @@ __device__ __forceinline__ void dct3_output_pairs(const DctCtx &D, float2 *__res
 }
 
 // Row DCT-III: T -> out, with the partial sums of r.out when `dot` is set.
-// E = 16 (lengths <= 8192): two workgroups per CU fit the LDS, so at most 128 VGPRs; E = 24: one.
+// E = 16 (lengths <= 8192): two workgroups of 8 waves per CU fit the LDS, so at most 128 VGPRs;
+// E = 12: one of 16 waves, the same cap; E = 24: one of 8 waves.
 template <int E>
-__global__ __launch_bounds__(MAX_FFT_THREADS) __attribute__((amdgpu_waves_per_eu(E == 16 ? 4 : 2))) void k_row_dct3(
+__global__ __launch_bounds__(fft_threads_max<E>()) __attribute__((amdgpu_waves_per_eu(E == 24 ? 2 : 4))) void k_row_dct3(
     Ctx c, const float *__restrict__ T_, float *__restrict__ out, int dot, int guard) {
   extern __shared__ float2 lds[];
   const int img = find_img(c.geo.rowpair, c.n, blockIdx.x);
@@ -947,7 +948,7 @@ __global__ __launch_bounds__(MAX_FFT_THREADS) __attribute__((amdgpu_waves_per_eu
       }
     }
   }
-  if (dot) block_partial<1>(acc, c.partials);
+  if (dot) block_partial<1, fft_threads_max<E>() / 64>(acc, c.partials);
 }
 
 #ifndef IRLS2D_TRANSFORMS_ONLY

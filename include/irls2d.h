@@ -130,7 +130,9 @@ int irls2d_precondition(int n, const int64_t *H, const int64_t *W, const float *
  * irls2d_plan_override sets the transform plan of DCT length n (1 .. 12288): fft_len = n for a
  * direct FFT (n 5-smooth) or a 5-smooth Bluestein length >= 2n - 1, `radices` the Stockham pass
  * order (nradices entries from {2, 3, 4, 5, 8, 16} multiplying to fft_len; 0 entries = the
- * default order), ept the elements per thread (16, 24, or 0 = default; 16 needs fft_len <= 8192).
+ * default order), ept the elements per thread (12 on up to 1024 threads, 16 or 24 on up to 512,
+ * or 0 = default: 16 up to 8192 points, 12 above; 16 needs fft_len <= 8192).  One value applies
+ * per launch: that of the longest transform in the batch.
  * fft_len = 0 removes the override.  Changing plans drops every cached layout, uploaded table set
  * and hipGraph (calls must not be in flight).  irls2d_plan_describe reports the plan in use
  * (radices: room for 16 entries); irls2d_plan_reset removes all overrides.

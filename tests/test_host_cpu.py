@@ -162,6 +162,9 @@ def test_plan_override_validation_and_roundtrip(lib):
         assert fl == 8192 and np.prod(rad) == 8192 and ept == 16
         lib.plan_override(8192, 8192, [8, 8, 8, 8, 2], 24)
         assert lib.plan_describe(8192) == (8192, [8, 8, 8, 8, 2], 24)
+        lib.plan_override(8192, 8192, [16, 16, 16, 2], 12)   # 12 per thread on up to 1024 threads
+        assert lib.plan_describe(8192) == (8192, [16, 16, 16, 2], 12)
+        assert lib.plan_describe(12288)[2] == 12              # the default above 8192
         b0 = lib.workspace_size([(16, 4999)])
         lib.plan_override(4999, 12288, [], 0)               # a longer Bluestein length
         assert lib.plan_describe(4999)[0] == 12288
