@@ -75,3 +75,31 @@ def test_tuned_plans_keep_parity(hip, tmp_path):
             assert np.array_equal(cc, cc64)
     finally:
         A.reset()
+
+
+def test_committed_plans_are_exact(hip):
+    """Every plan in core_amd/tuned_plans.json for this GPU (what bench.py applies): the
+    preconditioner of a 24-row image of that width against scipy FP64."""
+    import json
+    from core_amd import autotune as A, _lib
+    from core_amd.unwrap2d import Workspace, image_view, pack, precondition_device
+    with open(A.DEFAULT_FILE) as f:
+        plans = json.load(f).get(A._device_key(hip), {})
+    print(f"{len(plans)} committed plans", flush=True)
+    try:
+        assert A.apply() == len(plans)
+        rng = np.random.default_rng(5)
+        for n, p in sorted(plans.items(), key=lambda kv: int(kv[0])):
+            n = int(n)
+            assert _lib_describe(n) == (p["fft_len"], p["radices"], p["ept"])
+            shapes = [(24, n)]
+            r = rng.normal(size=shapes[0]).astype(np.float32)
+            ws = Workspace(shapes, hip)
+            z = precondition_device(pack([r], ws, hip), shapes, workspace=ws)
+            zg = image_view(z.cpu().numpy(), ws, 0).astype(np.float64)
+            ws.release()
+            z64 = _precond64(r)
+            err = np.abs(zg - z64).max() / np.abs(z64).max()
+            assert err < 2e-5, (n, p, err)
+    finally:
+        A.reset()
