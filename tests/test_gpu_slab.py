@@ -77,6 +77,35 @@ def test_slab_fast_preconditioner_matches_scipy_fp64(hip, shapes):
     ws.release()
 
 
+@pytest.mark.parametrize("ept", [0, 24])
+def test_long_rows_twelve_per_thread(hip, ept):
+    """5-smooth rows above 8192 points: 12 values per thread on up to 1024 threads (the default)
+    and 24 on 512 (a plan override), with compile-time (12288, 10240) and runtime-described
+    (9000 = 2^3 3^2 5^3) Stockham schedules, against scipy FP64."""
+    from core_amd import _lib
+    from core_amd.unwrap2d import Workspace, image_view, pack, precondition_device
+    shapes = [(8, 12288), (7, 10240), (9, 9000)]
+    rng = np.random.default_rng(14)
+    rs = [rng.normal(size=s).astype(np.float32) for s in shapes]
+    try:
+        for _, w in shapes:
+            fl, rad, e = _lib.plan_describe(w)
+            assert e == 12
+            if ept:
+                _lib.plan_override(w, fl, rad, ept)
+        print(f"long rows, ept {ept or 'default'}: {shapes}", flush=True)
+        ws = Workspace(shapes, hip)
+        zh = precondition_device(pack(rs, ws, hip), shapes, workspace=ws).cpu().numpy()
+        for k, ((h, w), r) in enumerate(zip(shapes, rs)):
+            z64 = _precond64(r)
+            err = np.abs(image_view(zh, ws, k).astype(np.float64) - z64).max() / np.abs(z64).max()
+            print(f"  {h}x{w}: max rel err {err:.2e}", flush=True)
+            assert err < 2e-5, (h, w, err)
+        ws.release()
+    finally:
+        _lib.plan_reset()
+
+
 def test_preconditioner_unaligned_buffers(hip):
     """Input and output arenas at a 4-byte (not 8-byte) offset: the row DCT-III then stores one
     float at a time instead of float2 pairs, with the same arithmetic, so z is bitwise equal to the
