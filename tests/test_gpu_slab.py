@@ -106,6 +106,25 @@ def test_long_rows_twelve_per_thread(hip, ept):
         _lib.plan_reset()
 
 
+def test_mixed_row_lengths_share_one_thread_shape(hip):
+    """One launch, one elements-per-thread value: rows of 12288 (12 per thread, 1024 threads) in
+    the same batch as 8192-point direct, Bluestein (5000, 211) and short rows, against scipy
+    FP64."""
+    from core_amd.unwrap2d import Workspace, image_view, pack, precondition_device
+    shapes = [(16, 8192), (12, 5000), (8, 12288), (10, 211), (6, 300)]
+    rng = np.random.default_rng(15)
+    rs = [rng.normal(size=s).astype(np.float32) for s in shapes]
+    print(f"mixed row lengths {shapes}", flush=True)
+    ws = Workspace(shapes, hip)
+    zh = precondition_device(pack(rs, ws, hip), shapes, workspace=ws).cpu().numpy()
+    for k, ((h, w), r) in enumerate(zip(shapes, rs)):
+        z64 = _precond64(r)
+        err = np.abs(image_view(zh, ws, k).astype(np.float64) - z64).max() / np.abs(z64).max()
+        print(f"  {h}x{w}: max rel err {err:.2e}", flush=True)
+        assert err < 2e-5, (h, w, err)
+    ws.release()
+
+
 def test_preconditioner_unaligned_buffers(hip):
     """Input and output arenas at a 4-byte (not 8-byte) offset: the row DCT-III then stores one
     float at a time instead of float2 pairs, with the same arithmetic, so z is bitwise equal to the
