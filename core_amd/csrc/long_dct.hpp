@@ -57,9 +57,50 @@ __device__ __forceinline__ void batch_pass(float2 *buf, int BS, int lb, const fl
   __syncthreads();
 }
 
-// Forward FFTs of all B sequences.  Requires B * L <= E * blockDim.x.
+// Compile-time form of batch_pass for a constant length, radix, stride product and batch (2^LB
+// sequences): every index, quotient and twiddle step is a constant.  Same arithmetic.
+template <int R, int NS, int L, int LB, int E>
+__device__ __forceinline__ void batch_pass_ct(float2 *buf, const float2 *tws) {
+  constexpr int B = 1 << LB, BS = B + 1, NB = L / R, NBF = NB << LB, MB = (E + R - 1) / R;
+  constexpr int TWMUL = L / (NS * R);
+  const int T = blockDim.x, tid = threadIdx.x;
+  float2 v[MB][R];
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int g = tid + q * T;
+    if (g < NBF) {
+      const int s = g & (B - 1), j = g >> LB;
+#pragma unroll
+      for (int t = 0; t < R; ++t) v[q][t] = buf[(j + t * NB) * BS + s];
+      if (NS > 1) twiddle_inputs<R>(v[q], tws, (j % NS) * TWMUL);
+      Dft<R>::run(v[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int g = tid + q * T;
+    if (g < NBF) {
+      const int s = g & (B - 1), j = g >> LB;
+      const int k = NS == 1 ? 0 : j % NS;
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int t = 0; t < R; ++t) buf[(base + t * NS) * BS + s] = v[q][t];
+    }
+  }
+  __syncthreads();
+}
+
+// Forward FFTs of all B sequences.  Requires B * L <= E * blockDim.x.  The default plan of
+// 200-point lines in batches of 32 (40000 = 200 x 200) runs the compile-time passes.
 template <int E>
 __device__ __forceinline__ void batch_fft(float2 *buf, int BS, int lb, const float2 *tws, const FftPlan *P) {
+  if (P->L == 200 && lb == 5 && P->npass == 3 && P->pass[0].R == 8 && P->pass[1].R == 5 && P->pass[2].R == 5) {
+    batch_pass_ct<8, 1, 200, 5, E>(buf, tws);
+    batch_pass_ct<5, 8, 200, 5, E>(buf, tws);
+    batch_pass_ct<5, 40, 200, 5, E>(buf, tws);
+    return;
+  }
   const int np = P->npass;
   for (int s = 0; s < np; ++s) {
     const PassDesc pd = P->pass[s];

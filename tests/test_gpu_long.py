@@ -117,7 +117,7 @@ def test_long_graph_equals_eager(hip):
     ws.release()
 
 
-PRECOND_SHAPES = [(64, 48), (127, 211), (1, 1000), (16, 12500), (12800, 8), (9, 15625), (12, 40000)]
+PRECOND_SHAPES = [(64, 48), (127, 211), (1, 1000), (16, 12500), (12800, 8), (9, 15625), (12, 40000), (40000, 6)]
 
 
 def test_preconditioner_matches_scipy_fp64(hip):
