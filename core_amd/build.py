@@ -35,12 +35,26 @@ def command(out: str = OUT, extra=()):
     return [HIPCC, *FLAGS, "-shared", *extra, *[os.path.join(CSRC, f) for f in SOURCES], "-o", out]
 
 
-def build(verbose: bool = True) -> str:
+def _newest_header() -> float:
+    """Modification time of the newest header (every unit includes them)."""
+    paths = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")]
+    return max(os.path.getmtime(p) for p in paths + [os.path.join(ROOT, "include", "irls2d.h")])
+
+
+def build(verbose: bool = True, incremental: bool = False) -> str:
+    """Compile every unit and link the library; with `incremental`, units whose object file is
+    newer than every source and header are not recompiled."""
     objdir = os.path.join(HERE, "_obj")
     os.makedirs(objdir, exist_ok=True)
+    headers = _newest_header() if incremental else None
     procs, objs = [], []
     for f in SOURCES:
         obj = os.path.join(objdir, os.path.splitext(f)[0] + ".o")
+        objs_ok = (incremental and os.path.exists(obj) and
+                   os.path.getmtime(obj) > max(headers, os.path.getmtime(os.path.join(CSRC, f))))
+        if objs_ok:
+            objs.append(obj)
+            continue
         cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, f), "-o", obj]
         if verbose:
             print(" ".join(shlex.quote(c) for c in cmd), flush=True)
@@ -59,5 +73,5 @@ def build(verbose: bool = True) -> str:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, incremental="--incremental" in sys.argv[1:])
     sys.exit(0)

@@ -25,10 +25,16 @@ constexpr int EP_BLOCK_PX = EP_THREADS * EP_PER_THREAD;  // pitched indices per 
 constexpr int EP_HASH = 128;                              // LDS slots of the per-block label table
 constexpr int EP_LDS_BINS = 8192;                         // histogram bins kept in LDS
 
-// k as the reference computes it in float32: round((u - psi) / (2 pi)), half to even.
+// Cycle counts beyond +-EP_KMAX saturate there (a diverged but finite u); the host then takes
+// the median from the float32 values themselves instead of from the histogram.
+constexpr int EP_KMAX = 1 << 30;
+
+// k as the reference computes it in float32: round((u - psi) / (2 pi)), half to even, saturated
+// to [-EP_KMAX, EP_KMAX] (a float outside the int range has no defined conversion).
 __device__ __forceinline__ int ep_k(float u, float psi) {
   const float two_pi = 6.28318530717958647692f;
-  return (int)rintf((u - psi) / two_pi);
+  const float k = rintf((u - psi) / two_pi);
+  return k >= (float)EP_KMAX ? EP_KMAX : (k <= -(float)EP_KMAX ? -EP_KMAX : (int)k);
 }
 
 template <typename T>
@@ -78,7 +84,7 @@ __global__ __launch_bounds__(EP_THREADS) void k_ep_scan(int H, int W, int P, con
   }
 }
 
-// hist[k - kmin] += 1 for every valid pixel (all finite by then); nb = kmax - kmin + 1.
+// hist[k - kmin] += 1 for every valid pixel (all finite by then) whose k lies in [kmin, kmin + nb).
 __global__ __launch_bounds__(EP_THREADS) void k_ep_hist(int H, int W, int P, const float *__restrict__ psi,
                                                         const float *__restrict__ u, int kmin, int nb, int *hist) {
   __shared__ int lh[EP_LDS_BINS];
@@ -92,6 +98,7 @@ __global__ __launch_bounds__(EP_THREADS) void k_ep_hist(int H, int W, int P, con
     const float p = psi[i];
     if (isnan(p)) continue;
     const int b = ep_k(u[i], p) - kmin;
+    if (b < 0 || b >= nb) continue;  // outside the caller's range: not counted
     if (local) atomicAdd(&lh[b], 1);
     else atomicAdd(&hist[b], 1);
   }

@@ -432,6 +432,7 @@ struct Layout {
          o_plans = 0, o_lplans = 0, o_splans = 0, o_tab = 0, o_dtab = 0, o_partials = 0, o_partials2 = 0, bytes = 0,
          meta_bytes = 0;
   uint64_t sig = 0;
+  std::vector<unsigned char> meta;  // host image of the metadata region (packed once per layout)
 };
 
 static uint64_t shape_sig(int n, const int64_t *H, const int64_t *W) {
@@ -1010,12 +1011,14 @@ static void enqueue(const Layout &Lo, const Ctx &c, const float *phase, const fl
 }
 
 // ---------------------------------------------------------------------------------------------
-// Caches: layouts per shape signature, uploaded metadata per workspace, graphs per call signature
+// Caches: layouts per shape signature, graphs per call signature.  The metadata region of a
+// workspace (descriptors, plans, tables) is uploaded on every call: the library never assumes
+// that a workspace still holds what an earlier call wrote, because a freed workspace's address can
+// come back for another one.
 // ---------------------------------------------------------------------------------------------
 
 static std::mutex g_mu;
 static std::map<uint64_t, std::shared_ptr<Layout>> g_layouts;
-static std::map<void *, uint64_t> g_uploaded;  // workspace -> signature of the metadata it holds
 
 struct GraphKey {
   void *ws;
@@ -1031,8 +1034,29 @@ struct GraphKey {
                     o.eps);
   }
 };
-static std::map<GraphKey, hipGraphExec_t> g_graphs;
+// Instantiated graphs, least recently used first out beyond MAX_GRAPHS (each holds a few thousand
+// kernel nodes; callers that replay with new arena addresses every call would otherwise grow it).
+constexpr size_t MAX_GRAPHS = 8;
+struct GraphEntry {
+  hipGraphExec_t exec;
+  uint64_t last_use;
+  int dev;
+};
+static std::map<GraphKey, GraphEntry> g_graphs;
+static uint64_t g_graph_clock = 0;
 static std::map<int, hipStream_t> g_capture_stream;  // per device
+
+// Pinned staging of the metadata upload, two slots per device used in turn: a slot is refilled
+// only after the copy that last read it has completed (its event), so the host can run one call
+// ahead of the device without a stream synchronisation per call.
+struct Staging {
+  void *buf[2] = {nullptr, nullptr};
+  size_t cap[2] = {0, 0};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool pending[2] = {false, false};
+  int next = 0;
+};
+static std::map<int, Staging> g_staging;
 
 static int get_layout(int n, const int64_t *H, const int64_t *W, std::shared_ptr<Layout> &out) {
   const uint64_t sig = shape_sig(n, H, W);
@@ -1048,6 +1072,7 @@ static int get_layout(int n, const int64_t *H, const int64_t *W, std::shared_ptr
   auto Lo = std::make_shared<Layout>();
   int rc = build_layout(n, H, W, *Lo);
   if (rc) return rc;
+  Lo->meta = pack_meta(*Lo);
   std::lock_guard<std::mutex> lk(g_mu);
   if (g_layouts.size() > 64) g_layouts.clear();
   g_layouts[sig] = Lo;
@@ -1083,21 +1108,33 @@ static int set_attrs() {
   return IRLS2D_OK;
 }
 
-// Uploads the plans, tables and descriptors of a layout unless this workspace already holds them.
+// Uploads the plans, tables and descriptors of a layout into the workspace's metadata region,
+// stream-ordered on `s` (through the pinned staging slots of the current device).
 static int upload_meta(const Layout &Lo, void *workspace, hipStream_t s) {
-  bool need_upload;
-  {
-    std::lock_guard<std::mutex> lk(g_mu);
-    auto it = g_uploaded.find(workspace);
-    need_upload = (it == g_uploaded.end() || it->second != Lo.sig);
+  if (Lo.meta.empty()) return IRLS2D_OK;
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_mu);
+  Staging &S = g_staging[dev];
+  const int k = S.next;
+  S.next ^= 1;
+  if (!S.ev[k]) HIP_TRY(hipEventCreateWithFlags(&S.ev[k], hipEventDisableTiming));
+  if (S.pending[k]) {
+    HIP_TRY(hipEventSynchronize(S.ev[k]));  // the copy that last read this slot has finished
+    S.pending[k] = false;
   }
-  if (need_upload) {
-    std::vector<unsigned char> meta = pack_meta(Lo);
-    HIP_TRY(hipMemcpyAsync((unsigned char *)workspace + Lo.o_meta, meta.data(), meta.size(), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    std::lock_guard<std::mutex> lk(g_mu);
-    g_uploaded[workspace] = Lo.sig;
+  if (S.cap[k] < Lo.meta.size()) {
+    if (S.buf[k]) HIP_TRY(hipHostFree(S.buf[k]));
+    S.buf[k] = nullptr;
+    S.cap[k] = 0;
+    const size_t cap = align_up(Lo.meta.size(), (size_t)1 << 20);
+    HIP_TRY(hipHostMalloc(&S.buf[k], cap, hipHostMallocDefault));
+    S.cap[k] = cap;
   }
+  std::memcpy(S.buf[k], Lo.meta.data(), Lo.meta.size());
+  HIP_TRY(hipMemcpyAsync((unsigned char *)workspace + Lo.o_meta, S.buf[k], Lo.meta.size(), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipEventRecord(S.ev[k], s));
+  S.pending[k] = true;
   return IRLS2D_OK;
 }
 
@@ -1153,7 +1190,10 @@ int solve(int n, const int64_t *H, const int64_t *W, const float *phase, const f
   {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_graphs.find(key);
-    if (it != g_graphs.end()) exec = it->second;
+    if (it != g_graphs.end()) {
+      exec = it->second.exec;
+      it->second.last_use = ++g_graph_clock;
+    }
   }
   if (!exec) {
     hipGraph_t graph;
@@ -1168,7 +1208,17 @@ int solve(int n, const int64_t *H, const int64_t *W, const float *phase, const f
     HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     HIP_TRY(hipGraphDestroy(graph));
     std::lock_guard<std::mutex> lk(g_mu);
-    g_graphs[key] = exec;
+    while (g_graphs.size() >= MAX_GRAPHS) {
+      auto lru = g_graphs.begin();
+      for (auto it = g_graphs.begin(); it != g_graphs.end(); ++it)
+        if (it->second.last_use < lru->second.last_use) lru = it;
+      // an evicted graph may still run on its capture stream: let it finish first
+      auto cs_it = g_capture_stream.find(lru->second.dev);
+      if (cs_it != g_capture_stream.end()) HIP_TRY(hipStreamSynchronize(cs_it->second));
+      (void)hipGraphExecDestroy(lru->second.exec);
+      g_graphs.erase(lru);
+    }
+    g_graphs[key] = GraphEntry{exec, ++g_graph_clock, dev};
   }
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreateWithFlags(&e0, hipEventDisableTiming));
@@ -1253,10 +1303,13 @@ int precondition(int n, const int64_t *H, const int64_t *W, const float *r, floa
 // Drops every cached layout, uploaded table set and graph (plans changed).
 static void drop_caches() {
   std::lock_guard<std::mutex> lk(g_mu);
-  for (auto &g : g_graphs) (void)hipGraphExecDestroy(g.second);
+  for (auto &g : g_graphs) {
+    auto cs = g_capture_stream.find(g.second.dev);
+    if (cs != g_capture_stream.end()) (void)hipStreamSynchronize(cs->second);
+    (void)hipGraphExecDestroy(g.second.exec);
+  }
   g_graphs.clear();
   g_layouts.clear();
-  g_uploaded.clear();
 }
 
 static bool valid_radix(int r) { return r == 2 || r == 3 || r == 4 || r == 5 || r == 8 || r == 16; }
@@ -1582,13 +1635,14 @@ int release(void *workspace) {
   std::lock_guard<std::mutex> lk(g_mu);
   for (auto it = g_graphs.begin(); it != g_graphs.end();) {
     if (!workspace || it->first.ws == workspace) {
-      (void)hipGraphExecDestroy(it->second);
+      auto cs = g_capture_stream.find(it->second.dev);
+      if (cs != g_capture_stream.end()) (void)hipStreamSynchronize(cs->second);
+      (void)hipGraphExecDestroy(it->second.exec);
       it = g_graphs.erase(it);
     } else {
       ++it;
     }
   }
-  if (workspace) g_uploaded.erase(workspace); else g_uploaded.clear();
   return IRLS2D_OK;
 }
 

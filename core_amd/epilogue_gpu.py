@@ -18,6 +18,10 @@ from . import _lib
 
 TWO_PI = 2 * np.pi
 _I32_MAX, _I32_MIN = 2 ** 31 - 1, -2 ** 31
+# k saturates at +-2^30 in the kernels (irls2d_ep_scan); above HIST_MAX_BINS bins (or at the
+# saturation bound) the median comes from the float32 k values on the host instead.
+K_SAT = 2 ** 30
+HIST_MAX_BINS = 1 << 24
 
 
 def _bind(L):
@@ -49,6 +53,16 @@ def _median_from_hist(hist, kmin):
     return np.float32(np.mean(np.array([a, b], np.float32)))
 
 
+def _median_host(phase, u, offset, P, H, W):
+    """np.median(np.round((u - psi) / 2 pi)) over the valid pixels in float32, as
+    utils_unwrap2d.py:1671-1674 computes it (used when k does not fit a histogram)."""
+    ph = phase[offset:offset + H * P].view(H, P)[:, :W].cpu().numpy()
+    un = u[offset:offset + H * P].view(H, P)[:, :W].cpu().numpy()
+    v = ~np.isnan(ph)
+    kv = np.round((un[v] - ph[v]) / (2 * np.pi))
+    return np.float32(np.median(kv))
+
+
 def epilogue_device(phase, u, offset, pitch, H, W, conncomp_size, stream=None):
     """Post-process image (offset, pitch, H, W) of the device arenas `phase` and `u` (float32
     torch tensors; u is updated in place).  Returns (unwrapped float32 [H, W], conncomp uint16
@@ -71,9 +85,13 @@ def epilogue_device(phase, u, offset, pitch, H, W, conncomp_size, stream=None):
     if nvalid == 0:
         return np.full((H, W), np.nan, np.float32), np.zeros((H, W), np.uint16)
     nb = kmax - kmin + 1
-    hist = torch.zeros(nb, dtype=torch.int32, device=dev)
-    _lib.check(L.irls2d_ep_hist(H, W, P, ph_p, u_p, kmin, nb, hist.data_ptr(), s), "irls2d_ep_hist")
-    kmed = _median_from_hist(hist.cpu().numpy().astype(np.int64), kmin)
+    if kmin <= -K_SAT or kmax >= K_SAT or nb > HIST_MAX_BINS:
+        # a diverged (finite) solution: the reference's np.median of the float32 k values
+        kmed = _median_host(phase, u, offset, P, H, W)
+    else:
+        hist = torch.zeros(nb, dtype=torch.int32, device=dev)
+        _lib.check(L.irls2d_ep_hist(H, W, P, ph_p, u_p, kmin, nb, hist.data_ptr(), s), "irls2d_ep_hist")
+        kmed = _median_from_hist(hist.cpu().numpy().astype(np.int64), kmin)
     # the reference subtracts k_median * 2 pi in float32 (utils_unwrap2d.py:1675)
     shift = np.float32(kmed * TWO_PI) if kmed != 0 else np.float32(0.0)
 

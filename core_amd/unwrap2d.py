@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import threading
 import time
+import weakref
 
 import numpy as np
 import torch
@@ -55,8 +56,19 @@ def resolve_device(device) -> torch.device:
     return dev
 
 
+def _release_address(ptr: int):
+    """Drop the library's cached graphs that refer to the workspace at `ptr`."""
+    try:
+        _lib.load().irls2d_release(ptr)
+    except Exception:  # interpreter shutdown: the library may already be gone
+        pass
+
+
 class Workspace:
-    """Device workspace for one set of shapes (reused across calls with the same shapes)."""
+    """Device workspace for one set of shapes (reused across calls with the same shapes).
+
+    The library's cached graphs for this workspace are released when the object is collected
+    (or by `release()`), before its device memory returns to torch's allocator."""
 
     _cache: dict = {}
 
@@ -67,6 +79,7 @@ class Workspace:
         self.nbytes = _lib.workspace_size(self.shapes)
         self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=device)
         self.info = torch.zeros(4 * len(self.shapes), dtype=torch.int32, device=device)
+        self._finalizer = weakref.finalize(self, _release_address, self.buf.data_ptr())
 
     @classmethod
     def get(cls, shapes, device):
@@ -85,6 +98,7 @@ class Workspace:
         return ws
 
     def release(self):
+        self._finalizer.detach()
         _lib.check(_lib.load().irls2d_release(self.buf.data_ptr()), "irls2d_release")
 
     @classmethod
@@ -232,32 +246,12 @@ def _solve_group(jobs, results, dev, max_iter, tol, cg_max_iter, cg_tol, epsilon
     has_w = [j[2] is not None for j in jobs]
     with _gpu_lock, torch.cuda.device(dev):
         ws = Workspace.get(shapes, dev) if cache else Workspace(shapes, dev)
-        ph_t = pack([j[1][j[4][0]:j[4][1], j[4][2]:j[4][3]] for j in jobs], ws, dev)
-        w_t = None
-        if any(has_w):
-            wl = []
-            for j, hw in zip(jobs, has_w):
-                r0, r1, c0, c1 = j[4]
-                if hw:
-                    wl.append(j[2][r0:r1, c0:c1])
-                else:  # uniform weight 1 - nan (utils_unwrap2d.py:1239-1241) via weight 1 on valid pixels
-                    wl.append(np.ones((r1 - r0, c1 - c0), np.float32))
-            w_t = pack(wl, ws, dev)
-        t1 = time.time()
-        u_t, info = solve_device(ph_t, w_t, shapes, max_iter, tol, cg_max_iter, cg_tol, epsilon,
-                                 graph=graph, workspace=ws)
-        info_h = info.cpu().numpy().copy()
-        if epilogue == 'device':
-            # per-pixel epilogue work on the device; only the images come back to the host
-            eps = [epilogue_device(ph_t, u_t, ws.offsets[idx], ws.pitches[idx], h, w, conncomp_size)
-                   for idx, (h, w) in enumerate(shapes)]
-        else:
-            u_host = torch.empty(ws.total, dtype=torch.float32, pin_memory=True)
-            u_host.copy_(u_t, non_blocking=True)
-        torch.cuda.synchronize(dev)
-        t2 = time.time()
-        if not cache:
-            del ph_t, w_t, u_t, info
+        try:
+            eps, info_h, u_host, t1, t2 = _run_group(jobs, ws, dev, shapes, has_w, max_iter, tol, cg_max_iter,
+                                                     cg_tol, epsilon, conncomp_size, graph, epilogue)
+        finally:
+            if not cache:
+                ws.release()  # its device memory returns to torch's allocator with the object
     if epilogue != 'device':
         uh = u_host.numpy()
         eps = []
@@ -271,12 +265,42 @@ def _solve_group(jobs, results, dev, max_iter, tol, cg_max_iter, cg_tol, epsilon
         results[k] = pp.uncrop(ep[0], ep[1], box, ph.shape)
         if debug:
             print(f'  image {k}: {h}x{w} irls_iters={inf[1]} cg_iters={inf[2]} status={inf[0]} valid={inf[3]}')
-    if not cache:
-        ws.release()  # its device memory returns to torch's allocator with the object
     if debug:
         t3 = time.time()
         print(f'TIMING irls_unwrap_2d_batch ({len(jobs)} images) on {dev}: upload {(t1 - t0) * 1e3:.1f} ms, '
               f'solve+download {(t2 - t1) * 1e3:.1f} ms, epilogue {(t3 - t2) * 1e3:.1f} ms')
+
+
+def _run_group(jobs, ws, dev, shapes, has_w, max_iter, tol, cg_max_iter, cg_tol, epsilon, conncomp_size, graph,
+               epilogue):
+    """Upload, solve and (device epilogue) post-process one group on `ws`; returns the per-image
+    epilogue results (device) or the downloaded u arena (host epilogue)."""
+    ph_t = pack([j[1][j[4][0]:j[4][1], j[4][2]:j[4][3]] for j in jobs], ws, dev)
+    w_t = None
+    if any(has_w):
+        wl = []
+        for j, hw in zip(jobs, has_w):
+            r0, r1, c0, c1 = j[4]
+            if hw:
+                wl.append(j[2][r0:r1, c0:c1])
+            else:  # uniform weight 1 - nan (utils_unwrap2d.py:1239-1241) via weight 1 on valid pixels
+                wl.append(np.ones((r1 - r0, c1 - c0), np.float32))
+        w_t = pack(wl, ws, dev)
+    t1 = time.time()
+    u_t, info = solve_device(ph_t, w_t, shapes, max_iter, tol, cg_max_iter, cg_tol, epsilon,
+                             graph=graph, workspace=ws)
+    info_h = info.cpu().numpy().copy()
+    eps, u_host = None, None
+    if epilogue == 'device':
+        # per-pixel epilogue work on the device; only the images come back to the host
+        eps = [epilogue_device(ph_t, u_t, ws.offsets[idx], ws.pitches[idx], h, w, conncomp_size)
+               for idx, (h, w) in enumerate(shapes)]
+    else:
+        u_host = torch.empty(ws.total, dtype=torch.float32, pin_memory=True)
+        u_host.copy_(u_t, non_blocking=True)
+    torch.cuda.synchronize(dev)
+    t2 = time.time()
+    return eps, info_h, u_host, t1, t2
 
 
 def irls_unwrap_2d(phase, weight=None, device='auto', max_iter=50, tol=1e-3,

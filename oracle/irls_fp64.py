@@ -56,14 +56,18 @@ def eigenvalues(H, W, dtype=np.float64):
     return lam
 
 
+# pocketfft threads over independent lines only, so the result does not depend on this
+WORKERS = -1
+
+
 def dct2(x):
     """Row DCT-II then column DCT-II (utils_unwrap2d.py:1337-1350)."""
-    return sfft.dct(sfft.dct(x, type=2, axis=1), type=2, axis=0)
+    return sfft.dct(sfft.dct(x, type=2, axis=1, workers=WORKERS), type=2, axis=0, workers=WORKERS)
 
 
 def idct2(X):
     """Column inverse then row inverse (utils_unwrap2d.py:1352-1365)."""
-    return sfft.idct(sfft.idct(X, type=2, axis=0), type=2, axis=1)
+    return sfft.idct(sfft.idct(X, type=2, axis=0, workers=WORKERS), type=2, axis=1, workers=WORKERS)
 
 
 def laplacian(x, wx, wy):

@@ -30,7 +30,7 @@ def dct(x: torch.Tensor) -> torch.Tensor:
     x = x.contiguous().view(-1, N)
     v = torch.cat([x[:, ::2], x[:, 1::2].flip([1])], dim=1)
     Vc = torch.view_as_real(torch.fft.fft(v, dim=1))
-    k = -torch.arange(N, dtype=x.dtype) * math.pi / (2 * N)
+    k = -torch.arange(N, dtype=x.dtype, device=x.device) * math.pi / (2 * N)
     Wr = torch.cos(k)
     Wi = torch.sin(k)
     V = Vc[:, :, 0] * Wr - Vc[:, :, 1] * Wi
@@ -42,7 +42,7 @@ def idct(X: torch.Tensor) -> torch.Tensor:
     shape = X.shape
     N = shape[-1]
     Xv = X.contiguous().view(-1, N) / 2
-    k = torch.arange(N, dtype=X.dtype) * math.pi / (2 * N)
+    k = torch.arange(N, dtype=X.dtype, device=X.device) * math.pi / (2 * N)
     Wr = torch.cos(k)
     Wi = torch.sin(k)
     Vtr = Xv
@@ -58,12 +58,15 @@ def idct(X: torch.Tensor) -> torch.Tensor:
 
 
 def solve(phi_np, w_np, valid_np, max_iter=50, tol=1e-3, cg_max_iter=20, cg_tol=1e-4, epsilon=1e-2,
-          stats=None):
-    """utils_unwrap2d.py:1247-1632 on a filled, cropped grid, float32 on the CPU device."""
-    dtype = torch.float32
-    phi = torch.from_numpy(np.ascontiguousarray(phi_np, np.float32))
-    w = torch.from_numpy(np.ascontiguousarray(w_np, np.float32))
-    valid = torch.from_numpy(np.ascontiguousarray(valid_np, bool))
+          stats=None, dtype=torch.float32, device='cpu'):
+    """utils_unwrap2d.py:1247-1632 on a filled, cropped grid, float32 on the CPU device.
+
+    `dtype` / `device` run the same torch operations in another precision or on a torch device
+    (the reference itself runs them on whatever device torch is given): float64 gives an FP64
+    trajectory of the same algorithm for grids too large for the numpy oracle."""
+    phi = torch.from_numpy(np.ascontiguousarray(phi_np, np.float32)).to(device=device, dtype=dtype)
+    w = torch.from_numpy(np.ascontiguousarray(w_np, np.float32)).to(device=device, dtype=dtype)
+    valid = torch.from_numpy(np.ascontiguousarray(valid_np, bool)).to(device)
     H, W = phi.shape
 
     dx_t = torch.zeros_like(phi)
@@ -82,8 +85,8 @@ def solve(phi_np, w_np, valid_np, max_iter=50, tol=1e-3, cg_max_iter=20, cg_tol=
     wy[:-1, :] *= (valid[:-1, :] & valid[1:, :]).to(dtype)
     del w
 
-    ii = torch.arange(H, dtype=torch.float64)
-    jj = torch.arange(W, dtype=torch.float64)
+    ii = torch.arange(H, dtype=torch.float64, device=device)
+    jj = torch.arange(W, dtype=torch.float64, device=device)
     lam = (2 * torch.cos(torch.pi * ii / H).unsqueeze(1) + 2 * torch.cos(torch.pi * jj / W).unsqueeze(0) - 4).to(dtype)
     lam[0, 0] = 1.0
 
@@ -199,12 +202,14 @@ def solve(phi_np, w_np, valid_np, max_iter=50, tol=1e-3, cg_max_iter=20, cg_tol=
         u = u_best
     if stats is not None:
         stats['irls_iters'] = n_it
-    return u.numpy().astype(np.float32)
+    out = u.cpu().numpy()
+    return out.astype(np.float32) if dtype == torch.float32 else out
 
 
 def irls_unwrap_2d(phase, weight=None, max_iter=50, tol=1e-3, cg_max_iter=20, cg_tol=1e-4,
-                   epsilon=1e-2, conncomp_size=30, return_raw=False):
-    """The reference call on the CPU device (utils_unwrap2d.py:1082-1766)."""
+                   epsilon=1e-2, conncomp_size=30, return_raw=False, dtype=torch.float32, device='cpu'):
+    """The reference call on the CPU device (utils_unwrap2d.py:1082-1766); `dtype` / `device` as
+    in `solve` (the epilogue always runs on the host, as the reference's does)."""
     if phase.ndim != 2:
         raise ValueError(f"irls_unwrap_2d expects 2D phase array, got shape {phase.shape}")
     pro = post.prologue(phase, weight)
@@ -212,7 +217,7 @@ def irls_unwrap_2d(phase, weight=None, max_iter=50, tol=1e-3, cg_max_iter=20, cg
         out = (np.full_like(phase, np.nan), np.zeros(phase.shape, np.uint16))
         return out + (None,) if return_raw else out
     crop, ph_c, nm, pf, wf = pro
-    u = solve(pf, wf, ~nm, max_iter, tol, cg_max_iter, cg_tol, epsilon)
+    u = solve(pf, wf, ~nm, max_iter, tol, cg_max_iter, cg_tol, epsilon, dtype=dtype, device=device)
     ep = post.epilogue(u, ph_c, nm, conncomp_size)
     if ep is None:
         out = (np.full(phase.shape, np.nan, np.float32), np.zeros(phase.shape, np.uint16))
