@@ -21,8 +21,14 @@ Also reported:
                 crop (--no-filters).
   max_scale     (--max-scale / --only-max-scale) 40000 x 40000 solve with two-pass transforms,
                 per-kernel GB/s, and cycle-correctness against the generator's noiseless phase.
+  kernel_roofline  every PCG kernel's mean launch time and fraction of HBM peak, side by side.
+  end_to_end    the drop-in irls_unwrap_2d on the 8192 x 8192 host arrays (crop, H2D, solve,
+                device epilogue, D2H); `value` times the solver on inputs already in HBM.
   cpu_baseline  the float32 torch-CPU restatement of the reference solver (oracle.irls_torch32)
-                on a bounded sample of the same workload, on this host's cores.
+                on a bounded sample of the same workload, and on the 1024 x 1024 CPU workload in
+                full, on this host's cores (--no-cpu skips this leg and `accuracy`).
+  accuracy      per workload: RMS against the FP64 trajectory, cycle agreement with the float32
+                reference path, conncomp equality (the oracles run in the cpu_baseline leg).
 """
 from __future__ import annotations
 
@@ -120,6 +126,17 @@ def plain_8192(args, dev):
     del ws, ph, co, out
     torch.cuda.empty_cache()
     return psi, coh, ms, wall_ms, info, stats
+
+
+def kernel_fracs(stats, npx, table=ALG_BPX):
+    """Every kernel with algorithmic bytes: its mean launch duration and fraction of HBM peak."""
+    out = {}
+    for k, (tot, n) in sorted(stats.items(), key=lambda kv: -kv[1][0]):
+        if k in table:
+            us = tot / n * 1e3
+            gbs = table[k] * npx / (us / 1e6) / 1e9
+            out[k] = {"mean_us": round(us, 2), "gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
 
 
 def roofline(stats, npx):
@@ -387,27 +404,116 @@ def gaussian_bench(args, dev, with_cpu=True):
     return res
 
 
-def cpu_baseline(psi, coh, rows=2048, cols=2048):
-    """Reference CPU path (float32 torch restatement) on the top-left rows x cols of the workload."""
+def end_to_end_8192(args, dev, psi, coh):
+    """The drop-in call on the 8192 x 8192 host arrays: crop, H2D, solve (eager), device epilogue,
+    D2H, as a caller of the reference's irls_unwrap_2d sees it.  Returns (timing dict, output)."""
+    import torch
+    from core_amd import irls_unwrap_2d
+    kw = dict(device='cuda', max_iter=N_IRLS, tol=0.0, cg_max_iter=N_CG, cg_tol=0.0)
+    out = irls_unwrap_2d(psi, coh, **kw)  # warm-up: plans, workspace
+    times = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = irls_unwrap_2d(psi, coh, **kw)
+        times.append(time.perf_counter() - t0)
+    ms = min(times) * 1e3
+    npx = psi.size
+    return {
+        "workload": "drop-in irls_unwrap_2d on the 8192x8192 numpy inputs: crop, H2D, solve (eager launches), "
+                    "device epilogue (k-median, 4-connected labels, circular-mean alignment), D2H",
+        "ms": round(ms, 1), "mpx_s": round(npx / ms / 1e3, 2),
+    }, out
+
+
+def _accuracy(u, cc, u64, cc64, u32, psi):
+    """RMS vs the FP64 trajectory, cycle agreement with the float32 reference path, conncomp
+    equality and the rewrap of the congruent output psi + 2 pi k (FP64), over valid pixels."""
+    import numpy as np
+    v = ~np.isnan(psi)
+    d = u[v].astype(np.float64) - u64[v].astype(np.float64)
+    k = np.round((u[v].astype(np.float64) - psi[v]) / (2 * np.pi))
+    k32 = np.round((u32[v].astype(np.float64) - psi[v]) / (2 * np.pi))
+    cong = psi[v].astype(np.float64) + 2 * np.pi * k
+    rms = float(np.sqrt(np.mean(d * d)))
+    agree = float(np.mean(k == k32))
+    return {"rms_vs_fp64_rad": float(f"{rms:.3e}"), "k_agree_vs_fp32": round(agree, 6),
+            "conncomp_equal": bool(np.array_equal(cc, cc64)),
+            "rewrap_max_rad": float(f"{float(np.abs(np.angle(np.exp(1j * (cong - psi[v])))).max()):.1e}"),
+            "pass": bool(rms <= 1e-3 and agree >= 0.9999 and np.array_equal(cc, cc64))}
+
+
+def cpu_baseline_leg(psi, coh, e2e_out, dev, rows=2048, cols=2048):
+    """The reference path, where the oracle runs (nothing here is timed as the GPU's value):
+
+    cpu_baseline  the float32 torch restatement of the reference solver (oracle.irls_torch32), on
+                  this host's cores, on the top-left rows x cols of the 8192 x 8192 workload (the
+                  sample of the metric's workload) and on the 1024 x 1024 CPU workload in full;
+    accuracy      per workload, the GPU's drop-in output against the FP64 trajectory (RMS) and the
+                  reference's float32 path (cycle agreement): 1024^2 against the CPU oracles (numpy
+                  FP64, torch float32), 8192^2, three stack bursts and the 40000^2 generator at 2048^2
+                  against the same torch restatement run on the HIP device in float64 and float32."""
     import numpy as np
     import torch
-    from oracle import irls_torch32, post
+    from core_amd import irls_unwrap_2d, irls_unwrap_2d_batch, synthetic as S
+    from oracle import irls_fp64, irls_torch32, post
     ncores = len(os.sched_getaffinity(0))
     threads = min(16, ncores)
     torch.set_num_threads(threads)
-    ph = np.ascontiguousarray(psi[:rows, :cols])
-    wt = np.ascontiguousarray(coh[:rows, :cols])
-    t0 = time.perf_counter()
-    pro = post.prologue(ph, wt)
-    _, ph_c, nm, pf, wf = pro
-    u = irls_torch32.solve(pf, wf, ~nm, N_IRLS, 0.0, N_CG, 0.0, 1e-2)
-    post.epilogue(u, ph_c, nm, 30)
-    dt = time.perf_counter() - t0
-    return {
+    fixed = dict(max_iter=N_IRLS, tol=0.0, cg_max_iter=N_CG, cg_tol=0.0)
+
+    def timed_cpu(ph, wt):
+        t0 = time.perf_counter()
+        _, ph_c, nm, pf, wf = post.prologue(ph, wt)
+        u = irls_torch32.solve(pf, wf, ~nm, N_IRLS, 0.0, N_CG, 0.0, 1e-2)
+        out = post.epilogue(u, ph_c, nm, 30)
+        return time.perf_counter() - t0, out
+
+    dt, _ = timed_cpu(np.ascontiguousarray(psi[:rows, :cols]), np.ascontiguousarray(coh[:rows, :cols]))
+    base = {
         "value": round(rows * cols / dt / 1e6, 4), "unit": "Mpx/s", "cores": threads, "kind": "port",
         "sample": f"top-left {rows}x{cols} crop of the 8192x8192 workload, 10 IRLS x 50 PCG, float32 "
                   f"torch-CPU restatement incl. host epilogue, {dt:.1f} s on {threads} of {ncores} host threads",
     }
+    # the 1024 x 1024 CPU workload in full
+    p0, c0, _ = S.generate(S.single_1024())
+    dt0, ep32 = timed_cpu(p0, c0)
+    base["cpu_workload_1024"] = {"value": round(p0.size / dt0 / 1e6, 4), "unit": "Mpx/s", "seconds": round(dt0, 2),
+                                 "cores": threads, "sample": "1024x1024 CPU workload in full, 10 IRLS x 50 PCG"}
+    acc = {}
+    u, cc = irls_unwrap_2d(p0, c0, device='cuda', **fixed)
+    u64, cc64 = irls_fp64.irls_unwrap_2d(p0, c0, **fixed)
+    acc["cpu_workload_1024"] = {"oracles": "numpy FP64 and torch float32 on the CPU",
+                                **_accuracy(u, cc, u64, cc64, ep32[0], p0)}
+
+    def on_device(ph, wt):
+        r64 = irls_torch32.irls_unwrap_2d(ph, wt, dtype=torch.float64, device=dev, **fixed)
+        torch.cuda.empty_cache()
+        r32 = irls_torch32.irls_unwrap_2d(ph, wt, dtype=torch.float32, device=dev, **fixed)
+        torch.cuda.empty_cache()
+        return r64, r32
+
+    r64, r32 = on_device(psi, coh)
+    acc["plain_8192"] = {"oracles": "torch restatement on the HIP device, float64 and float32",
+                         **_accuracy(e2e_out[0], e2e_out[1], r64[0], r64[1], r32[0], psi)}
+    del r64, r32
+    wls = S.stack_48()
+    small = sorted(range(len(wls)), key=lambda i: wls[i].H * wls[i].W)[:3]
+    ims = [S.generate(wls[i])[:2] for i in small]
+    res = irls_unwrap_2d_batch([a for a, _ in ims], [b for _, b in ims], device='cuda', **fixed)
+    bursts = []
+    for i, (ph, wt), (u, cc) in zip(small, ims, res):
+        r64, r32 = on_device(ph, wt)
+        bursts.append({"burst": i, "shape": [wls[i].H, wls[i].W], **_accuracy(u, cc, r64[0], r64[1], r32[0], ph)})
+    acc["stack_48_smallest3"] = {"oracles": "torch restatement on the HIP device, float64 and float32",
+                                 "note": "solved as one batch; an image's solve is bitwise independent of its "
+                                         "batch-mates", "bursts": bursts}
+    p4, c4, _ = S.generate(S.scaled(S.single_40000(), 2048, 2048))
+    u, cc = irls_unwrap_2d(p4, c4, device='cuda', **fixed)
+    r64, r32 = on_device(p4, c4)
+    acc["max_scale_generator_2048"] = {"oracles": "torch restatement on the HIP device, float64 and float32",
+                                       **_accuracy(u, cc, r64[0], r64[1], r32[0], p4)}
+    return base, acc
 
 
 def main():
@@ -448,6 +554,7 @@ def main():
     npx = 8192 * 8192
     psi, coh, ms, wall_ms, info, stats = plain_8192(args, dev)
     dom, roof = roofline(stats, npx)
+    e2e, e2e_out = end_to_end_8192(args, dev, psi, coh)
     result = {
         "metric": METRIC,
         "value": round(npx / ms / 1e3, 3),
@@ -467,6 +574,8 @@ def main():
             "solver_info": {"irls_iters": info[1], "pcg_iters": info[2], "valid_px": info[3]},
         },
         "roofline": roof,
+        "kernel_roofline": kernel_fracs(stats, npx),
+        "end_to_end": e2e,
         "solve_model": {
             "bytes_per_px": model_bytes_per_px(),
             "achieved_gbs": round(npx * model_bytes_per_px() / (ms / 1e3) / 1e9, 1),
@@ -484,7 +593,7 @@ def main():
     if args.max_scale:
         result["max_scale"] = max_scale(args, dev)
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(psi, coh)
+        result["cpu_baseline"], result["accuracy"] = cpu_baseline_leg(psi, coh, e2e_out, dev)
     print(json.dumps(result), flush=True)
 
 

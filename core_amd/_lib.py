@@ -45,6 +45,7 @@ EXPORTS = (
     "irls2d_ep_sums",
     "irls2d_ep_apply",
     "irls2d_release",
+    "irls2d_debug_inject",
     "irls2d_kernel_stats",
     "irls2d_kernel_stats_reset",
 )
@@ -116,6 +117,8 @@ def load():
                                    ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
                                    ctypes.c_size_t, ctypes.c_void_p]
     L.irls2d_goldstein.restype = ctypes.c_int
+    L.irls2d_debug_inject.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.irls2d_debug_inject.restype = ctypes.c_int
     L.irls2d_release.argtypes = [ctypes.c_void_p]
     L.irls2d_release.restype = ctypes.c_int
     L.irls2d_kernel_stats.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), i64p]

@@ -145,6 +145,8 @@ struct ImgDesc {
   int cosW_off;    // double offset of cos(pi j / W), j < W
   int colslab;     // SlabPlan index when the column pass runs on slabs (offT is then its scratch), else -1
   int nslab;       // slabs of the image: ceil(W / (2 SLAB_CP))
+  int rowpart0;    // first block-partial slot of the image's row-pair blocks (short rows)
+  int coldirect;   // 1: the column DCTs read column pairs straight from the row-major field
 };
 
 // Per-image solver state.  Written by k_state_init and k_finalize (one block per image), read by
@@ -176,12 +178,15 @@ struct Fields {
   unsigned char *valid;
 };
 
-struct Geo {  // block start of each image per launch geometry, [n + 1] each
+// Block start of each image per launch geometry, [n + 1] each.  The transform passes run one
+// launch per class of images that share a kernel instantiation (elements per thread, direct or
+// transposed columns, slab split); rowpair / colpair / slab1 / slab2 then point at that class's
+// table, in which the images of other classes have no blocks.
+struct Geo {
   const int *tile, *rowpair, *colpair, *trans;
   const int *rowL1, *rowL2, *colL1, *colL2;  // long-line passes (images with W or H > MAX_FFT_LEN)
   const int *slab1, *slab2;                  // slab column passes: K1 / K3, and K2
 };
-constexpr int NGEO = 10;
 
 struct Ctx {
   const ImgDesc *img;
@@ -196,6 +201,7 @@ struct Ctx {
   float2 *S;          // long-transform scratch arena
   Geo geo;
   Fields f;
+  unsigned char *ws_base;  // workspace base (host side: class tables are offsets from it)
   int n;
   float eps_sq;
   double tol, cg_tol;

@@ -407,6 +407,23 @@ constexpr int64_t FIELD_PAD = 149 * 64;
 
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// Fixed block-start tables of a layout; the class tables (LaunchClass::geo / geo2) follow them.
+enum { G_TILE = 0, G_TRANS, G_ROWL1, G_ROWL2, G_COLL1, G_COLL2, G_FIXED };
+
+// Images whose transform pass shares one kernel instantiation: one launch per class and pass.
+// Every choice is a function of the image's own shape (and the global mode / plan overrides), so an
+// image's arithmetic does not depend on the other images of its batch.
+struct LaunchClass {
+  int E = 16;          // elements per thread (row / column DCT classes)
+  bool direct = false; // column class: pairs read straight from the row-major field
+  int f1 = 0, f2 = 0;  // slab class: register-stage split N1 x N2 (0 = batched-LDS kernels)
+  int geo = -1;        // index of the class's block-start table (rows, columns, slab K1 / K3)
+  int geo2 = -1;       // slab class: table of the K2 blocks
+  int threads = 64;    // workgroup size
+  size_t lds = 0;      // dynamic LDS bytes
+  int maxM = 1;        // longest FFT among the class's images
+};
+
 struct Layout {
   int n = 0;
   std::vector<int64_t> H, W, off, pitch, offT;
@@ -414,7 +431,8 @@ struct Layout {
   int64_t fstride = 0;  // elements between consecutive workspace fields
   // metadata (host images of the device region)
   std::vector<ImgDesc> desc;
-  std::vector<int> geo[NGEO];  // tile, rowpair, colpair, trans, rowL1, rowL2, colL1, colL2, slab1, slab2
+  std::vector<std::vector<int>> geo;  // G_FIXED fixed tables, then the class tables
+  std::vector<LaunchClass> rowcls, colcls, slabcls;
   std::vector<DctPlan> plans;
   std::vector<LongPlan> lplans;
   std::vector<SlabPlan> splans;
@@ -422,15 +440,12 @@ struct Layout {
   std::vector<double> dtab;
   int max_blocks = 0, max_blocks2 = 0;
   int64_t totalS = 0;  // float2 elements of the long-transform scratch arena
-  int row_threads = 64, col_threads = 64, row_E = 16, col_E = 16;
-  size_t row_lds = 0, col_lds = 0, long_lds = 0, slab_lds = 0;
-  int slab_threads = 64;
-  int slab_f1 = 0, slab_f2 = 0;  // N1, N2 of the register-stage kernels when every slab image uses them
-  bool col_direct = false;  // column pass straight on the row-major field (no transposes)
+  size_t long_lds = 0;
   // byte offsets inside the workspace
-  size_t o_fields = 0, o_Tt = 0, o_S = 0, o_valid = 0, o_meta = 0, o_desc = 0, o_state = 0, o_geo[NGEO] = {},
-         o_plans = 0, o_lplans = 0, o_splans = 0, o_tab = 0, o_dtab = 0, o_partials = 0, o_partials2 = 0, bytes = 0,
+  size_t o_fields = 0, o_Tt = 0, o_S = 0, o_valid = 0, o_meta = 0, o_desc = 0, o_state = 0, o_plans = 0,
+         o_lplans = 0, o_splans = 0, o_tab = 0, o_dtab = 0, o_partials = 0, o_partials2 = 0, bytes = 0,
          meta_bytes = 0;
+  std::vector<size_t> o_geo;
   uint64_t sig = 0;
   std::vector<unsigned char> meta;  // host image of the metadata region (packed once per layout)
 };
@@ -455,18 +470,6 @@ static int64_t arena_layout(int n, const int64_t *H, const int64_t *W, int64_t *
     tot += H[i] * pitch_of(W[i]);
   }
   return tot;
-}
-
-// Elements per thread (16 or 24: the largest any plan of the launch asks for, and enough for L
-// within MAX_FFT_THREADS threads) and threads per workgroup for transforms of length <= L.
-static void fft_shape(int L, int ept, int &E, int &T) {
-  // ept: the plan's choice for the longest transform of the launch (0 = default); 16 cannot
-  // cover lengths above 8192
-  E = ept ? ept : default_ept(L);
-  if (E * ept_threads_max(E) < L) E = default_ept(L);
-  int t = (L + E - 1) / E;
-  t = (t + 63) / 64 * 64;
-  T = std::max(64, std::min(ept_threads_max(E), t));
 }
 
 static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
@@ -527,7 +530,7 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
   };
   // slab column plan of length N, or -1 when N has no slab split or its factors' plans are not
   // plain FFTs of their own length (a Bluestein override)
-  std::map<int, int> slab_idx;
+  std::map<int, int> slab_idx, slab_threads_of;
   auto add_slab = [&](int N) -> int {
     auto it = slab_idx.find(N);
     if (it != slab_idx.end()) return it->second;
@@ -544,9 +547,7 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
         P.mk_off += base;
         Lo.splans.push_back(P);
         idx = (int)Lo.splans.size() - 1;
-        const int e1 = P.N1 * ((1 << P.lb1) + 1), e2 = P.N2 * (2 * SLAB_CP + 1);
-        Lo.slab_lds = std::max(Lo.slab_lds, (size_t)(TW_LDS + std::max(e1, e2)) * sizeof(float2));
-        Lo.slab_threads = std::max(Lo.slab_threads, h->threads);
+        slab_threads_of[idx] = h->threads;
       }
     }
     slab_idx[N] = idx;
@@ -560,26 +561,56 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     cos_off[N] = o;
     return o;
   };
-  for (int g = 0; g < NGEO; ++g) Lo.geo[g].assign(1, 0);
-  // Automatic mode takes the slab passes only with the register-stage kernels, which run one
-  // power-of-two split per batch: the split of the first image that has one.  Other column lengths
-  // keep the transposed path (on MI355X the batched-LDS slab kernels measured 0-9% slower than it at
-  // 6000^2, 3000 x 4000 and 1500 x 5000, and 4% slower at 40000^2).
-  int auto_f1 = 0, auto_f2 = 0;
+  Lo.geo.assign(G_FIXED, std::vector<int>(1, 0));
+  // Automatic mode takes the slab passes only with the register-stage kernels (power-of-two splits);
+  // other column lengths keep the transposed or direct path (on MI355X the batched-LDS slab kernels
+  // measured 0-9% slower than the transposes at 6000^2, 3000 x 4000 and 1500 x 5000, and 4% slower
+  // at 40000^2).
   auto auto_slab = [&](int64_t h, int64_t w, int &f1, int &f2) -> bool {
     return h >= g_slab_min && h <= MAX_FFT_LEN && w <= MAX_FFT_LEN && slab_split(h, f1, f2) && slab_fast(f1, f2);
   };
-  if (g_col_mode == 0) {
-    for (int i = 0; i < n && !auto_f1; ++i) {
-      int f1, f2;
-      if (auto_slab(H[i], W[i], f1, f2)) { auto_f1 = f1; auto_f2 = f2; }
-    }
-  }
-  // one elements-per-thread value per launch: that of the plan with the longest transform
-  int maxM_row = 1, maxM_col = 1, ept_row = 0, ept_col = 0;
+  // class lookup (creating the class and its empty block tables on first use)
+  auto new_table = [&]() -> int {
+    Lo.geo.emplace_back(1, 0);
+    return (int)Lo.geo.size() - 1;
+  };
+  auto row_class = [&](int E) -> int {
+    for (size_t k = 0; k < Lo.rowcls.size(); ++k)
+      if (Lo.rowcls[k].E == E) return (int)k;
+    LaunchClass L;
+    L.E = E;
+    L.geo = new_table();
+    Lo.rowcls.push_back(L);
+    return (int)Lo.rowcls.size() - 1;
+  };
+  auto col_class = [&](int E, bool direct) -> int {
+    for (size_t k = 0; k < Lo.colcls.size(); ++k)
+      if (Lo.colcls[k].E == E && Lo.colcls[k].direct == direct) return (int)k;
+    LaunchClass L;
+    L.E = E;
+    L.direct = direct;
+    L.geo = new_table();
+    Lo.colcls.push_back(L);
+    return (int)Lo.colcls.size() - 1;
+  };
+  auto slab_class = [&](int f1, int f2) -> int {
+    for (size_t k = 0; k < Lo.slabcls.size(); ++k)
+      if (Lo.slabcls[k].f1 == f1 && Lo.slabcls[k].f2 == f2) return (int)k;
+    LaunchClass L;
+    L.f1 = f1;
+    L.f2 = f2;
+    L.geo = new_table();
+    L.geo2 = new_table();
+    Lo.slabcls.push_back(L);
+    return (int)Lo.slabcls.size() - 1;
+  };
+  // per image: (table, blocks) pairs added to the tables; every other table gets 0 blocks
+  std::vector<std::vector<std::pair<int, int64_t>>> adds(n);
+  int rowparts = 0;
   for (int i = 0; i < n; ++i) {
     const int h = (int)H[i], w = (int)W[i];
     ImgDesc d;
+    std::memset(&d, 0, sizeof(d));
     d.off = Lo.off[i];
     d.H = h;
     d.W = w;
@@ -591,94 +622,97 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     d.rowlong = lw ? add_long(w) : -1;
     d.cosH_off = add_cos(h);
     d.cosW_off = add_cos(w);
-    // mode 3 takes every 5-smooth column length, long sides included
+    // slab column pass: automatic mode on this image's own register split; mode 3 on every
+    // 5-smooth column length, long sides included
     int sf1 = 0, sf2 = 0;
-    const bool want_slab = g_col_mode == 3 || (g_col_mode == 0 && auto_f1 && auto_slab(h, w, sf1, sf2) &&
-                                               sf1 == auto_f1 && sf2 == auto_f2);
+    const bool want_slab = g_col_mode == 3 || (g_col_mode == 0 && auto_slab(h, w, sf1, sf2));
     d.colslab = want_slab ? add_slab(h) : -1;
     d.nslab = (w + 2 * SLAB_CP - 1) / (2 * SLAB_CP);
     const bool slab = d.colslab >= 0;
     d.collong = (lh && !slab) ? add_long(h) : -1;
+    // direct column DCTs when this image's own column transform is Bluestein (its arithmetic hides
+    // the strided access), transposes otherwise (measured on MI355X: -16% per PCG iteration for the
+    // 48-burst stack with direct, +16% at 8192^2)
+    const bool colk = !slab && !lh;
+    d.coldirect = colk && (g_col_mode == 2 || ((g_col_mode == 0 || g_col_mode == 3) && Lo.plans[d.colplan].blue));
     // transposed copy (W x PT), or the slab scratch (H rows of nslab x SLAB_CP float2 = P floats)
     Lo.offT[i] = Lo.totalT;
     d.offT = Lo.totalT;
     Lo.totalT += slab ? (int64_t)h * d.P : (int64_t)w * d.PT;
     int64_t sz = 0;
-    int64_t lr1 = 0, lr2 = 0, lc1 = 0, lc2 = 0;
     if (lw) {
       const LongPlan &P = Lo.lplans[d.rowlong];
       sz = std::max<int64_t>(sz, (int64_t)((h + 1) / 2) * P.spitch);
-      lr1 = (int64_t)((h + 1) / 2) * P.nb1;
-      lr2 = (int64_t)((h + 1) / 2) * P.nb2;
+      adds[i].push_back({G_ROWL1, (int64_t)((h + 1) / 2) * P.nb1});
+      adds[i].push_back({G_ROWL2, (int64_t)((h + 1) / 2) * P.nb2});
     }
     if (d.collong >= 0) {
       const LongPlan &P = Lo.lplans[d.collong];
       sz = std::max<int64_t>(sz, (int64_t)((w + 1) / 2) * P.spitch);
-      lc1 = (int64_t)((w + 1) / 2) * P.nb1;
-      lc2 = (int64_t)((w + 1) / 2) * P.nb2;
+      adds[i].push_back({G_COLL1, (int64_t)((w + 1) / 2) * P.nb1});
+      adds[i].push_back({G_COLL2, (int64_t)((w + 1) / 2) * P.nb2});
     }
     d.offS = Lo.totalS;
     Lo.totalS += (sz + 31) / 32 * 32;
-    Lo.desc.push_back(d);
-    if (!lw && Lo.plans[d.rowplan].M > maxM_row) {
-      maxM_row = Lo.plans[d.rowplan].M;
-      ept_row = Lo.plans[d.rowplan].ept;
+    adds[i].push_back({G_TILE, ((int64_t)(h + TILE_H - 1) / TILE_H) * ((w + TILE_W - 1) / TILE_W)});
+    if (!slab && !d.coldirect)
+      adds[i].push_back({G_TRANS, ((int64_t)(h + TT - 1) / TT) * ((w + TT - 1) / TT)});
+    if (!lw) {
+      const DctPlan &R = Lo.plans[d.rowplan];
+      LaunchClass &C = Lo.rowcls[row_class(R.ept)];
+      C.maxM = std::max(C.maxM, R.M);
+      adds[i].push_back({C.geo, (h + 1) / 2});
+      d.rowpart0 = rowparts;
+      rowparts += (h + 1) / 2;
     }
-    if (!lh && !slab && Lo.plans[d.colplan].M > maxM_col) {
-      maxM_col = Lo.plans[d.colplan].M;
-      ept_col = Lo.plans[d.colplan].ept;
+    if (colk) {
+      const DctPlan &Cp = Lo.plans[d.colplan];
+      LaunchClass &C = Lo.colcls[col_class(Cp.ept, d.coldirect)];
+      C.maxM = std::max(C.maxM, Cp.M);
+      adds[i].push_back({C.geo, (w + 1) / 2});
     }
-    const int64_t tiles = ((int64_t)(h + TILE_H - 1) / TILE_H) * ((w + TILE_W - 1) / TILE_W);
-    const int64_t ttiles = slab ? 0 : ((int64_t)(h + TT - 1) / TT) * ((w + TT - 1) / TT);
-    int64_t ls1 = 0, ls2 = 0;
     if (slab) {
       const SlabPlan &P = Lo.splans[d.colslab];
-      ls1 = (int64_t)P.nb1 * d.nslab;
-      ls2 = (int64_t)P.nb2 * d.nslab;
+      const bool fast = slab_fast(P.N1, P.N2);
+      LaunchClass &C = Lo.slabcls[slab_class(fast ? P.N1 : 0, fast ? P.N2 : 0)];
+      C.threads = std::max(C.threads, fast ? std::max(256, 4 * P.N2) : slab_threads_of[d.colslab]);
+      const int e1 = P.N1 * ((1 << P.lb1) + 1), e2 = P.N2 * (2 * SLAB_CP + 1);
+      C.lds = std::max(C.lds, (size_t)(TW_LDS + std::max(e1, e2)) * sizeof(float2));
+      adds[i].push_back({C.geo, (int64_t)P.nb1 * d.nslab});
+      adds[i].push_back({C.geo2, (int64_t)P.nb2 * d.nslab});
     }
-    const int64_t geo_add[NGEO] = {tiles, lw ? 0 : (h + 1) / 2, (lh || slab) ? 0 : (w + 1) / 2, ttiles,
-                                   lr1, lr2, lc1, lc2, ls1, ls2};
-    for (int g = 0; g < NGEO; ++g) {
-      const int64_t v = (int64_t)Lo.geo[g].back() + geo_add[g];
+    Lo.desc.push_back(d);
+  }
+  // block-start tables: image i's blocks follow those of images 0 .. i-1 in every table
+  for (auto &g : Lo.geo) g.assign(1, 0);
+  for (int i = 0; i < n; ++i) {
+    std::vector<int64_t> add(Lo.geo.size(), 0);
+    for (auto &pr : adds[i]) add[pr.first] += pr.second;
+    for (size_t g = 0; g < Lo.geo.size(); ++g) {
+      const int64_t v = (int64_t)Lo.geo[g].back() + add[g];
       if (v > INT32_MAX) return fail(IRLS2D_EUNSUPPORTED, "too many blocks in one launch");
       Lo.geo[g].push_back((int)v);
     }
   }
-  for (int g = 0; g < 4; ++g) Lo.max_blocks = std::max(Lo.max_blocks, Lo.geo[g].back());
-  for (int g = 4; g < 8; ++g) Lo.max_blocks2 = std::max(Lo.max_blocks2, Lo.geo[g].back());
-  {
-    // register-stage slab kernels when every slab image has the same power-of-two split
-    int f1 = -1, f2 = -1;
-    for (const SlabPlan &P : Lo.splans) {
-      if (f1 < 0) { f1 = P.N1; f2 = P.N2; }
-      else if (P.N1 != f1 || P.N2 != f2) f1 = 0;
+  Lo.max_blocks = std::max(std::max(Lo.geo[G_TILE].back(), rowparts), 1);
+  Lo.max_blocks2 = std::max(std::max(Lo.geo[G_ROWL1].back(), Lo.geo[G_ROWL2].back()),
+                            std::max(Lo.geo[G_COLL1].back(), Lo.geo[G_COLL2].back()));
+  // launch shapes per class: the workgroup covers the class's longest transform
+  auto shape_of = [&](LaunchClass &C) {
+    int t = (C.maxM + C.E - 1) / C.E;
+    t = (t + 63) / 64 * 64;
+    C.threads = std::max(64, std::min(ept_threads_max(C.E), t));
+    C.lds = (size_t)(TW_LDS + lds_seq_len(C.maxM)) * sizeof(float2);
+  };
+  for (auto &C : Lo.rowcls) shape_of(C);
+  for (auto &C : Lo.colcls) shape_of(C);
+  for (auto &C : Lo.slabcls)
+    if (C.f1) {
+      // register-stage kernels: K1 / K3 on 256 threads, K2 on 4 N2; exchange buffer, then the K2
+      // tables of 2 N2 float2 twiddles and 2 N2 doubles
+      C.lds = (size_t)(TW_LDS + std::max(C.f1 * SLAB_CP * (128 / C.f1), C.f2 * 2 * SLAB_CP + 4 * C.f2)) *
+              sizeof(float2);
     }
-    if (f1 > 0 && slab_fast(f1, f2)) {
-      Lo.slab_f1 = f1;
-      Lo.slab_f2 = f2;
-      // K2: exchange buffer, then tables of 2 N2 float2 twiddles and 2 N2 doubles
-      Lo.slab_lds = (size_t)(TW_LDS + std::max(f1 * SLAB_CP * (128 / f1), f2 * 2 * SLAB_CP + 4 * f2)) * sizeof(float2);
-    }
-  }
-  {
-    // The transposes cost 8 B/px each; reading column pairs straight from the row-major field
-    // (float2 per row, L2-shared between neighbouring blocks) costs more than that when the column
-    // FFT is short and cheap, but hides under it when the column transforms are Bluestein
-    // (measured on MI355X: -16% per PCG iteration for the 48-burst stack, +16% at 8192^2).
-    double blue_px = 0, all_px = 0;
-    for (int i = 0; i < n; ++i) {
-      if (Lo.desc[i].colslab >= 0 || Lo.desc[i].colplan < 0) continue;  // no column kernel
-      const double px = (double)H[i] * (double)W[i];
-      all_px += px;
-      if (Lo.desc[i].colplan >= 0 && Lo.plans[Lo.desc[i].colplan].blue) blue_px += px;
-    }
-    const bool auto_direct = all_px > 0 && blue_px >= 0.5 * all_px;
-    Lo.col_direct = Lo.geo[6].back() == 0 && (g_col_mode == 2 || ((g_col_mode == 0 || g_col_mode == 3) && auto_direct));
-  }
-  fft_shape(maxM_row, ept_row, Lo.row_E, Lo.row_threads);
-  fft_shape(maxM_col, ept_col, Lo.col_E, Lo.col_threads);
-  Lo.row_lds = (size_t)(TW_LDS + lds_seq_len(maxM_row)) * sizeof(float2);
-  Lo.col_lds = (size_t)(TW_LDS + lds_seq_len(maxM_col)) * sizeof(float2);
   // workspace map
   size_t o = 0;
   Lo.fstride = Lo.total + FIELD_PAD;
@@ -699,7 +733,8 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
   Lo.o_meta = o;  // everything below is uploaded from the host in one copy
   Lo.o_desc = o;
   o = align_up(o + n * sizeof(ImgDesc), 256);
-  for (int g = 0; g < NGEO; ++g) {
+  Lo.o_geo.resize(Lo.geo.size());
+  for (size_t g = 0; g < Lo.geo.size(); ++g) {
     Lo.o_geo[g] = o;
     o = align_up(o + (n + 1) * sizeof(int), 256);
   }
@@ -724,7 +759,7 @@ static std::vector<unsigned char> pack_meta(const Layout &Lo) {
     if (nbytes) std::memcpy(buf.data() + (off - Lo.o_meta), src, nbytes);
   };
   put(Lo.o_desc, Lo.desc.data(), Lo.desc.size() * sizeof(ImgDesc));
-  for (int g = 0; g < NGEO; ++g) put(Lo.o_geo[g], Lo.geo[g].data(), Lo.geo[g].size() * sizeof(int));
+  for (size_t g = 0; g < Lo.geo.size(); ++g) put(Lo.o_geo[g], Lo.geo[g].data(), Lo.geo[g].size() * sizeof(int));
   put(Lo.o_plans, Lo.plans.data(), Lo.plans.size() * sizeof(DctPlan));
   put(Lo.o_lplans, Lo.lplans.data(), Lo.lplans.size() * sizeof(LongPlan));
   put(Lo.o_splans, Lo.splans.data(), Lo.splans.size() * sizeof(SlabPlan));
@@ -795,16 +830,15 @@ static Ctx make_ctx(const Layout &Lo, unsigned char *ws, float *u, const Params 
   c.tab = (const float2 *)(ws + Lo.o_tab);
   c.dtab = (const double *)(ws + Lo.o_dtab);
   c.partials = (double *)(ws + Lo.o_partials);
-  c.geo.tile = (const int *)(ws + Lo.o_geo[0]);
-  c.geo.rowpair = (const int *)(ws + Lo.o_geo[1]);
-  c.geo.colpair = (const int *)(ws + Lo.o_geo[2]);
-  c.geo.trans = (const int *)(ws + Lo.o_geo[3]);
-  c.geo.rowL1 = (const int *)(ws + Lo.o_geo[4]);
-  c.geo.rowL2 = (const int *)(ws + Lo.o_geo[5]);
-  c.geo.colL1 = (const int *)(ws + Lo.o_geo[6]);
-  c.geo.colL2 = (const int *)(ws + Lo.o_geo[7]);
-  c.geo.slab1 = (const int *)(ws + Lo.o_geo[8]);
-  c.geo.slab2 = (const int *)(ws + Lo.o_geo[9]);
+  c.geo.tile = (const int *)(ws + Lo.o_geo[G_TILE]);
+  c.geo.trans = (const int *)(ws + Lo.o_geo[G_TRANS]);
+  c.geo.rowL1 = (const int *)(ws + Lo.o_geo[G_ROWL1]);
+  c.geo.rowL2 = (const int *)(ws + Lo.o_geo[G_ROWL2]);
+  c.geo.colL1 = (const int *)(ws + Lo.o_geo[G_COLL1]);
+  c.geo.colL2 = (const int *)(ws + Lo.o_geo[G_COLL2]);
+  // class tables: set per launch (class_ctx)
+  c.geo.rowpair = c.geo.colpair = c.geo.slab1 = c.geo.slab2 = nullptr;
+  c.ws_base = ws;
   float *F = (float *)(ws + Lo.o_fields);
   float *fld[NFIELD];
   for (int k = 0; k < NFIELD; ++k) fld[k] = F + (size_t)k * Lo.fstride;
@@ -831,7 +865,7 @@ static void fin(const Layout &Lo, const Ctx &c, int kind, int it, hipStream_t s)
 // can share a batch, each launch covering the images of its kind.
 static void long_pass1(const Layout &Lo, const Ctx &c, int dir, int mode, float *src, const float *p, int guard,
                        hipStream_t s) {
-  const dim3 g(Lo.geo[dir ? 6 : 4].back()), b(LONG_THREADS);
+  const dim3 g(Lo.geo[dir ? G_COLL1 : G_ROWL1].back()), b(LONG_THREADS);
   const size_t l = Lo.long_lds;
   switch (mode) {
     case LONG_DCT2:
@@ -853,7 +887,7 @@ static void long_pass1(const Layout &Lo, const Ctx &c, int dir, int mode, float 
 
 static void long_pass2(const Layout &Lo, const Ctx &c, int dir, int mode, float *dst, int dot, int guard,
                        hipStream_t s) {
-  const dim3 g(Lo.geo[dir ? 7 : 5].back()), b(LONG_THREADS);
+  const dim3 g(Lo.geo[dir ? G_COLL2 : G_ROWL2].back()), b(LONG_THREADS);
   if (mode == LONG_DCT2)
     LAUNCH(dir ? "k_long_p2_dct2_cols" : "k_long_p2_dct2_rows", (k_long_p2<LONG_DCT2, LONG_E>), g, b, Lo.long_lds,
            s, c, dir, dst, dot, guard);
@@ -862,92 +896,119 @@ static void long_pass2(const Layout &Lo, const Ctx &c, int dir, int mode, float 
            s, c, dir, dst, dot, guard);
 }
 
+// Context of one class launch: the class's block-start tables in place of rowpair / colpair /
+// slab1 / slab2.
+static Ctx class_ctx(const Layout &Lo, const Ctx &c, const LaunchClass &C) {
+  Ctx k = c;
+  const int *t = (const int *)(c.ws_base + Lo.o_geo[C.geo]);
+  k.geo.rowpair = k.geo.colpair = k.geo.slab1 = t;
+  k.geo.slab2 = C.geo2 >= 0 ? (const int *)(c.ws_base + Lo.o_geo[C.geo2]) : nullptr;
+  return k;
+}
+
 static void row_dct2(const Layout &Lo, const Ctx &c, bool fuse, float *src, float *dst, const float *p, int guard,
                      hipStream_t s) {
-  if (Lo.geo[4].back() > 0) {
+  if (Lo.geo[G_ROWL1].back() > 0) {
     long_pass1(Lo, c, 0, fuse ? LONG_DCT2_UPDATE : LONG_DCT2, src, p, guard, s);
     long_pass2(Lo, c, 0, LONG_DCT2, dst, 0, guard, s);
   }
-  if (Lo.geo[1].back() == 0) return;
-  const dim3 g(Lo.geo[1].back()), b(Lo.row_threads);
-  if (fuse) {
-    if (Lo.row_E == 16) LAUNCH("k_row_dct2_update", (k_row_dct2<true, 16>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
-    else if (Lo.row_E == 12) LAUNCH("k_row_dct2_update", (k_row_dct2<true, 12>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
-    else LAUNCH("k_row_dct2_update", (k_row_dct2<true, 24>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
-  } else {
-    if (Lo.row_E == 16) LAUNCH("k_row_dct2", (k_row_dct2<false, 16>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
-    else if (Lo.row_E == 12) LAUNCH("k_row_dct2", (k_row_dct2<false, 12>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
-    else LAUNCH("k_row_dct2", (k_row_dct2<false, 24>), g, b, Lo.row_lds, s, c, src, dst, p, guard);
+  for (const LaunchClass &C : Lo.rowcls) {
+    const int nb = Lo.geo[C.geo].back();
+    if (nb == 0) continue;
+    const Ctx k = class_ctx(Lo, c, C);
+    const dim3 g(nb), b(C.threads);
+    if (fuse) {
+      if (C.E == 16) LAUNCH("k_row_dct2_update", (k_row_dct2<true, 16>), g, b, C.lds, s, k, src, dst, p, guard);
+      else if (C.E == 12) LAUNCH("k_row_dct2_update", (k_row_dct2<true, 12>), g, b, C.lds, s, k, src, dst, p, guard);
+      else LAUNCH("k_row_dct2_update", (k_row_dct2<true, 24>), g, b, C.lds, s, k, src, dst, p, guard);
+    } else {
+      if (C.E == 16) LAUNCH("k_row_dct2", (k_row_dct2<false, 16>), g, b, C.lds, s, k, src, dst, p, guard);
+      else if (C.E == 12) LAUNCH("k_row_dct2", (k_row_dct2<false, 12>), g, b, C.lds, s, k, src, dst, p, guard);
+      else LAUNCH("k_row_dct2", (k_row_dct2<false, 24>), g, b, C.lds, s, k, src, dst, p, guard);
+    }
   }
 }
 
 static void row_dct3(const Layout &Lo, const Ctx &c, const float *src, float *dst, int dot, int guard, hipStream_t s) {
-  if (Lo.geo[4].back() > 0) {
+  if (Lo.geo[G_ROWL1].back() > 0) {
     long_pass1(Lo, c, 0, LONG_DCT3, const_cast<float *>(src), nullptr, guard, s);
     long_pass2(Lo, c, 0, LONG_DCT3, dst, dot, guard, s);
   }
-  if (Lo.geo[1].back() == 0) return;
-  const dim3 g(Lo.geo[1].back()), b(Lo.row_threads);
-  if (Lo.row_E == 16) LAUNCH("k_row_dct3", (k_row_dct3<16>), g, b, Lo.row_lds, s, c, src, dst, dot, guard);
-  else if (Lo.row_E == 12) LAUNCH("k_row_dct3", (k_row_dct3<12>), g, b, Lo.row_lds, s, c, src, dst, dot, guard);
-  else LAUNCH("k_row_dct3", (k_row_dct3<24>), g, b, Lo.row_lds, s, c, src, dst, dot, guard);
+  for (const LaunchClass &C : Lo.rowcls) {
+    const int nb = Lo.geo[C.geo].back();
+    if (nb == 0) continue;
+    const Ctx k = class_ctx(Lo, c, C);
+    const dim3 g(nb), b(C.threads);
+    if (C.E == 16) LAUNCH("k_row_dct3", (k_row_dct3<16>), g, b, C.lds, s, k, src, dst, dot, guard);
+    else if (C.E == 12) LAUNCH("k_row_dct3", (k_row_dct3<12>), g, b, C.lds, s, k, src, dst, dot, guard);
+    else LAUNCH("k_row_dct3", (k_row_dct3<24>), g, b, C.lds, s, k, src, dst, dot, guard);
+  }
 }
 
 static void col_pass(const Layout &Lo, const Ctx &c, int guard, hipStream_t s) {
-  if (Lo.geo[8].back() > 0) {
-    const dim3 g1(Lo.geo[8].back()), g2(Lo.geo[9].back());
-    const size_t l = Lo.slab_lds;
+  for (const LaunchClass &C : Lo.slabcls) {
+    const int nb1 = Lo.geo[C.geo].back(), nb2 = Lo.geo[C.geo2].back();
+    if (nb1 == 0) continue;
+    const Ctx k = class_ctx(Lo, c, C);
+    const dim3 g1(nb1), g2(nb2);
+    const size_t l = C.lds;
     const float *T = c.f.T;
-    if (Lo.slab_f1) {
-      const dim3 b1(256), b2(4 * Lo.slab_f2);
-      switch (Lo.slab_f1) {
-        case 16: LAUNCH("k_slab1_f", k_slab1_f<16>, g1, b1, l, s, c, T, guard); break;
-        case 32: LAUNCH("k_slab1_f", k_slab1_f<32>, g1, b1, l, s, c, T, guard); break;
-        case 64: LAUNCH("k_slab1_f", k_slab1_f<64>, g1, b1, l, s, c, T, guard); break;
-        default: LAUNCH("k_slab1_f", k_slab1_f<128>, g1, b1, l, s, c, T, guard); break;
+    if (C.f1) {
+      const dim3 b1(256), b2(4 * C.f2);
+      switch (C.f1) {
+        case 16: LAUNCH("k_slab1_f", k_slab1_f<16>, g1, b1, l, s, k, T, guard); break;
+        case 32: LAUNCH("k_slab1_f", k_slab1_f<32>, g1, b1, l, s, k, T, guard); break;
+        case 64: LAUNCH("k_slab1_f", k_slab1_f<64>, g1, b1, l, s, k, T, guard); break;
+        default: LAUNCH("k_slab1_f", k_slab1_f<128>, g1, b1, l, s, k, T, guard); break;
       }
-      switch (Lo.slab_f2) {
-        case 16: LAUNCH("k_slab2_f", k_slab2_f<16>, g2, b2, l, s, c, guard); break;
-        case 32: LAUNCH("k_slab2_f", k_slab2_f<32>, g2, b2, l, s, c, guard); break;
-        default: LAUNCH("k_slab2_f", k_slab2_f<64>, g2, b2, l, s, c, guard); break;
+      switch (C.f2) {
+        case 16: LAUNCH("k_slab2_f", k_slab2_f<16>, g2, b2, l, s, k, guard); break;
+        case 32: LAUNCH("k_slab2_f", k_slab2_f<32>, g2, b2, l, s, k, guard); break;
+        default: LAUNCH("k_slab2_f", k_slab2_f<64>, g2, b2, l, s, k, guard); break;
       }
-      switch (Lo.slab_f1) {
-        case 16: LAUNCH("k_slab3_f", k_slab3_f<16>, g1, b1, l, s, c, c.f.T, guard); break;
-        case 32: LAUNCH("k_slab3_f", k_slab3_f<32>, g1, b1, l, s, c, c.f.T, guard); break;
-        case 64: LAUNCH("k_slab3_f", k_slab3_f<64>, g1, b1, l, s, c, c.f.T, guard); break;
-        default: LAUNCH("k_slab3_f", k_slab3_f<128>, g1, b1, l, s, c, c.f.T, guard); break;
+      switch (C.f1) {
+        case 16: LAUNCH("k_slab3_f", k_slab3_f<16>, g1, b1, l, s, k, c.f.T, guard); break;
+        case 32: LAUNCH("k_slab3_f", k_slab3_f<32>, g1, b1, l, s, k, c.f.T, guard); break;
+        case 64: LAUNCH("k_slab3_f", k_slab3_f<64>, g1, b1, l, s, k, c.f.T, guard); break;
+        default: LAUNCH("k_slab3_f", k_slab3_f<128>, g1, b1, l, s, k, c.f.T, guard); break;
       }
     } else {
-      const dim3 b(Lo.slab_threads);
-      LAUNCH("k_slab1", (k_slab1<SLAB_E>), g1, b, l, s, c, T, guard);
-      LAUNCH("k_slab2", (k_slab2<SLAB_E>), g2, b, l, s, c, guard);
-      LAUNCH("k_slab3", (k_slab3<SLAB_E>), g1, b, l, s, c, c.f.T, guard);
+      const dim3 b(C.threads);
+      LAUNCH("k_slab1", (k_slab1<SLAB_E>), g1, b, l, s, k, T, guard);
+      LAUNCH("k_slab2", (k_slab2<SLAB_E>), g2, b, l, s, k, guard);
+      LAUNCH("k_slab3", (k_slab3<SLAB_E>), g1, b, l, s, k, c.f.T, guard);
     }
   }
-  if (Lo.geo[6].back() > 0) {
+  if (Lo.geo[G_COLL1].back() > 0) {
     long_pass1(Lo, c, 1, LONG_DCT2, c.f.Tt, nullptr, guard, s);
     long_pass2(Lo, c, 1, LONG_DCT2, c.f.Tt, 0, guard, s);
     long_pass1(Lo, c, 1, LONG_DCT3_SCALE, c.f.Tt, nullptr, guard, s);
     long_pass2(Lo, c, 1, LONG_DCT3, c.f.Tt, 0, guard, s);
   }
-  if (Lo.geo[2].back() == 0) return;
-  const dim3 g(Lo.geo[2].back()), b(Lo.col_threads);
-  if (Lo.col_direct) {
-    if (Lo.col_E == 16) LAUNCH("k_col_direct", (k_col<16, true>), g, b, Lo.col_lds, s, c, c.f.T, guard);
-    else if (Lo.col_E == 12) LAUNCH("k_col_direct", (k_col<12, true>), g, b, Lo.col_lds, s, c, c.f.T, guard);
-    else LAUNCH("k_col_direct", (k_col<24, true>), g, b, Lo.col_lds, s, c, c.f.T, guard);
-  } else {
-    if (Lo.col_E == 16) LAUNCH("k_col", (k_col<16, false>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
-    else if (Lo.col_E == 12) LAUNCH("k_col", (k_col<12, false>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
-    else LAUNCH("k_col", (k_col<24, false>), g, b, Lo.col_lds, s, c, c.f.Tt, guard);
+  for (const LaunchClass &C : Lo.colcls) {
+    const int nb = Lo.geo[C.geo].back();
+    if (nb == 0) continue;
+    const Ctx k = class_ctx(Lo, c, C);
+    const dim3 g(nb), b(C.threads);
+    if (C.direct) {
+      if (C.E == 16) LAUNCH("k_col_direct", (k_col<16, true>), g, b, C.lds, s, k, c.f.T, guard);
+      else if (C.E == 12) LAUNCH("k_col_direct", (k_col<12, true>), g, b, C.lds, s, k, c.f.T, guard);
+      else LAUNCH("k_col_direct", (k_col<24, true>), g, b, C.lds, s, k, c.f.T, guard);
+    } else {
+      if (C.E == 16) LAUNCH("k_col", (k_col<16, false>), g, b, C.lds, s, k, c.f.Tt, guard);
+      else if (C.E == 12) LAUNCH("k_col", (k_col<12, false>), g, b, C.lds, s, k, c.f.Tt, guard);
+      else LAUNCH("k_col", (k_col<24, false>), g, b, C.lds, s, k, c.f.Tt, guard);
+    }
   }
 }
 
+// T <-> Tt for the images whose columns run on the transposed field (transposed column DCTs and
+// long columns).
 static void transpose(const Layout &Lo, const Ctx &c, int dir, int guard, hipStream_t s) {
-  if (Lo.col_direct || Lo.geo[3].back() == 0) return;
+  if (Lo.geo[G_TRANS].back() == 0) return;
   const float *src = dir == 0 ? c.f.T : c.f.Tt;
   float *dst = dir == 0 ? c.f.Tt : c.f.T;
-  LAUNCH("k_transpose", k_transpose, dim3(Lo.geo[3].back()), dim3(256), 0, s, c, src, dst, dir, guard);
+  LAUNCH("k_transpose", k_transpose, dim3(Lo.geo[G_TRANS].back()), dim3(256), 0, s, c, src, dst, dir, guard);
 }
 
 // z = M^{-1} src (utils_unwrap2d.py:1390-1403); `dot` adds r.z (DOT_RZ_INIT / DOT_RZ).
@@ -963,8 +1024,13 @@ static void precond(const Layout &Lo, const Ctx &c, float *src, float *dst, int 
 
 // PCG on A = D^T W D with W from c.f.wxi / c.f.wyi, right-hand side c.f.b, x = c.f.u updated in
 // place (utils_unwrap2d.py:1405-1474): r = b - A x, z = M^{-1} r, then cg_max_iter iterations.
-static void pcg(const Layout &Lo, const Ctx &c, int cg_max_iter, int it, hipStream_t s) {
-  const int gt = Lo.geo[0].back();
+// Test hook (irls2d_debug_inject): in PCG iteration g_inject_cg of IRLS iteration g_inject_irls,
+// p(0, 0) of every active image becomes +inf after alpha is formed, so that the update x += alpha p
+// is non-finite.  -1 = off; a solve then runs eagerly even when a graph was asked for.
+static int g_inject_irls = -1, g_inject_cg = -1;
+
+static void pcg(const Layout &Lo, const Ctx &c, int cg_max_iter, int it, hipStream_t s, int inject_cg = -1) {
+  const int gt = Lo.geo[G_TILE].back();
   const dim3 tb(TILE_THREADS);
   LAUNCH("k_cg_init", k_cg_init, dim3(gt), tb, 0, s, c);
   fin(Lo, c, FIN_CG_INIT, it, s);
@@ -974,6 +1040,7 @@ static void pcg(const Layout &Lo, const Ctx &c, int cg_max_iter, int it, hipStre
     float *pnew = (i & 1) ? c.f.p0 : c.f.p1;
     LAUNCH("k_cg_pap", k_cg_pap, dim3(gt), tb, 0, s, c, (const float *)pold, pnew);
     fin(Lo, c, FIN_PAP, it, s);
+    if (i == inject_cg) LAUNCH("k_inject", k_inject, dim3(Lo.n), dim3(64), 0, s, c, pnew);
     row_dct2(Lo, c, true, c.f.r, c.f.T, pnew, GUARD_CG, s);
     fin(Lo, c, FIN_UPDATE, it, s);
     transpose(Lo, c, 0, GUARD_CG, s);
@@ -986,7 +1053,7 @@ static void pcg(const Layout &Lo, const Ctx &c, int cg_max_iter, int it, hipStre
 
 static void enqueue(const Layout &Lo, const Ctx &c, const float *phase, const float *weight, int32_t *info,
                     const Params &p, hipStream_t s) {
-  const int gt = Lo.geo[0].back();
+  const int gt = Lo.geo[G_TILE].back();
   const dim3 tb(TILE_THREADS);
   LAUNCH("k_state_init", k_state_init, dim3(1), dim3(64), 0, s, c);
   LAUNCH("k_setup", k_setup, dim3(gt), tb, 0, s, c, phase, weight);
@@ -1002,7 +1069,7 @@ static void enqueue(const Layout &Lo, const Ctx &c, const float *phase, const fl
     LAUNCH("k_irls_update", k_irls_update, dim3(gt), tb, 0, s, c);
     fin(Lo, c, FIN_IRLS_UPDATE, it, s);
     LAUNCH("k_irls_commit", k_irls_commit, dim3(gt), tb, 0, s, c, it);
-    pcg(Lo, c, p.cg_max_iter, it, s);
+    pcg(Lo, c, p.cg_max_iter, it, s, it == g_inject_irls ? g_inject_cg : -1);
     LAUNCH("k_irls_check", k_irls_check, dim3(gt), tb, 0, s, c);
     fin(Lo, c, FIN_IRLS_CHECK, it, s);
     LAUNCH("k_restore", k_restore, dim3(gt), tb, 0, s, c, it);
@@ -1160,7 +1227,7 @@ int solve(int n, const int64_t *H, const int64_t *W, const float *phase, const f
   if (rc) return rc;
   Params p{max_iter, cg_max_iter, tol, cg_tol, epsilon};
   Ctx c = make_ctx(*Lo, ws, out, p);
-  if (!(flags & IRLS2D_FLAG_GRAPH)) {
+  if (!(flags & IRLS2D_FLAG_GRAPH) || g_inject_irls >= 0) {
     KTimer timer;
     if (flags & IRLS2D_FLAG_TIME_KERNELS) t_timer = &timer;
     enqueue(*Lo, c, phase, weight, info, p, s);
@@ -1236,7 +1303,7 @@ int solve(int n, const int64_t *H, const int64_t *W, const float *phase, const f
 static void enqueue_adjoint(const Layout &Lo, const Ctx &c, const float *phase, const float *weight,
                             const float *u, const float *g, float *gphase, float *gweight, int32_t *info,
                             int cg_max_iter, hipStream_t s) {
-  const int gt = Lo.geo[0].back();
+  const int gt = Lo.geo[G_TILE].back();
   const dim3 tb(TILE_THREADS);
   LAUNCH("k_state_init", k_state_init, dim3(1), dim3(64), 0, s, c);
   LAUNCH("k_setup", k_setup, dim3(gt), tb, 0, s, c, phase, weight);
@@ -1683,6 +1750,13 @@ int irls2d_solve(int n, const int64_t *H, const int64_t *W, const float *phase, 
 
 int irls2d_release(void *workspace) { return irls2d::release(workspace); }
 
+int irls2d_debug_inject(int irls_iter, int cg_iter) {
+  if (irls_iter < 0 || cg_iter < 0) irls_iter = cg_iter = -1;
+  irls2d::g_inject_irls = irls_iter;
+  irls2d::g_inject_cg = cg_iter;
+  return IRLS2D_OK;
+}
+
 int64_t irls2d_ep_blocks(int64_t H, int64_t P) { return irls2d::ep_blocks(H, P); }
 
 int irls2d_ep_scan(int64_t H, int64_t W, int64_t P, const float *phase, const float *u, int32_t *stats, void *stream) {
@@ -1756,9 +1830,14 @@ int irls2d_describe_columns(int n, const int64_t *H, const int64_t *W, int32_t *
   if (rc) return rc;
   for (int i = 0; i < n; ++i) {
     const irls2d::ImgDesc &d = Lo->desc[i];
-    if (d.colslab >= 0) strategy[i] = Lo->slab_f1 ? IRLS2D_COLUMNS_SLAB_REGISTER : IRLS2D_COLUMNS_SLAB;
-    else if (d.collong >= 0) strategy[i] = IRLS2D_COLUMNS_LONG;
-    else strategy[i] = Lo->col_direct ? IRLS2D_COLUMNS_DIRECT : IRLS2D_COLUMNS_TRANSPOSE;
+    if (d.colslab >= 0) {
+      const irls2d::SlabPlan &P = Lo->splans[d.colslab];
+      strategy[i] = irls2d::slab_fast(P.N1, P.N2) ? IRLS2D_COLUMNS_SLAB_REGISTER : IRLS2D_COLUMNS_SLAB;
+    } else if (d.collong >= 0) {
+      strategy[i] = IRLS2D_COLUMNS_LONG;
+    } else {
+      strategy[i] = d.coldirect ? IRLS2D_COLUMNS_DIRECT : IRLS2D_COLUMNS_TRANSPOSE;
+    }
   }
   return IRLS2D_OK;
 }

@@ -24,8 +24,9 @@
  *   - phase: float32 radians, NaN marks invalid pixels (the crop is done by the caller).
  *   - weight: float32 quality (NaN allowed) or NULL for uniform weights.
  *   - unwrapped: float32 output arena (the raw solver output u, before the host epilogue).
- *   - Every call is stream-ordered on `stream` (a hipStream_t, NULL = default stream) and
- *     does not synchronise the host except when workspace tables are (re)uploaded.
+ *   - Every call is stream-ordered on `stream` (a hipStream_t, NULL = default stream).  The
+ *     transform tables and image descriptors are uploaded into the workspace on every call
+ *     (through pinned staging; the host waits at most for the upload two calls back).
  *   - Return value: IRLS2D_OK or an error code; irls2d_last_error() gives the message.
  */
 #ifndef IRLS2D_H
@@ -131,8 +132,9 @@ int irls2d_precondition(int n, const int64_t *H, const int64_t *W, const float *
  * direct FFT (n 5-smooth) or a 5-smooth Bluestein length >= 2n - 1, `radices` the Stockham pass
  * order (nradices entries from {2, 3, 4, 5, 8, 16} multiplying to fft_len; 0 entries = the
  * default order), ept the elements per thread (12 on up to 1024 threads, 16 or 24 on up to 512,
- * or 0 = default: 16 up to 8192 points, 12 above; 16 needs fft_len <= 8192).  One value applies
- * per launch: that of the longest transform in the batch.
+ * or 0 = default: 16 up to 8192 points, 12 above; 16 needs fft_len <= 8192).  Each image's row and
+ * column transforms run with their own plan's value (one launch per value in a batch), so an
+ * image's result does not depend on the other images of its batch.
  * fft_len = 0 removes the override.  Changing plans drops every cached layout, uploaded table set
  * and hipGraph (calls must not be in flight).  irls2d_plan_describe reports the plan in use
  * (radices: room for 16 entries); irls2d_plan_reset removes all overrides.
@@ -144,17 +146,18 @@ int irls2d_plan_reset(void);
 
 /*
  * Column-pass strategy (no reference counterpart: the reference transposes with torch).
- * 0 = automatic (default): images with both sides <= 12288 whose column length, at least 256,
- * splits into a power-of-two pair N1 N2 (N1 16 .. 128, N2 16 .. 64: heights 256 .. 8192) with
- * register-stage kernels run the three four-step slab passes on the row-major field (no
- * transposes), for the split of the first such image of the batch; for the others the
- * column DCTs read column pairs straight from the row-major field when at least half of their
- * pixels have Bluestein column transforms (whose arithmetic hides the strided access), otherwise
- * the field is transposed, transformed and transposed back;
+ * 0 = automatic (default), decided per image from its own shape: images with both sides <= 12288
+ * whose column length, at least 256, splits into a power-of-two pair N1 N2 (N1 16 .. 128, N2
+ * 16 .. 64: heights 256 .. 8192) run the three four-step slab passes with register-stage kernels on
+ * the row-major field (no transposes; one launch per split in a batch); the other images read
+ * their column pairs straight from the row-major field when their column transform is Bluestein
+ * (whose arithmetic hides the strided access), otherwise their rows are transposed, transformed
+ * and transposed back;
  * 1 = always transpose; 2 = always direct; 3 = slab passes for every image whose column length
  * allows them (2^a 3^b 5^c with such a split, from 4), the automatic choice for the rest.  Other
- * columns longer than 12288 transpose and run the two-pass long transform.  Drops cached layouts
- * and hipGraphs.  core_amd.autotune measures the strategies for a batch.
+ * columns longer than 12288 transpose and run the two-pass long transform.  Since every choice is
+ * per image, an image's result is the same alone and in any batch (under one mode and one plan
+ * set).  Drops cached layouts and hipGraphs.  core_amd.autotune measures the strategies.
  */
 int irls2d_set_column_mode(int mode);
 
@@ -238,10 +241,21 @@ int irls2d_ep_apply(int64_t H, int64_t W, int64_t P, float *u, const int32_t *pa
                     const double *adjust, const uint16_t *relabel, uint16_t *conncomp, void *stream);
 
 /*
- * Release cached hipGraphs and plans that refer to `workspace` (NULL = all).  Call before
- * freeing a workspace that was used with IRLS2D_FLAG_GRAPH.
+ * Release the cached hipGraphs that refer to `workspace` (NULL = all), after the graph runs in
+ * flight have finished.  Call before freeing a workspace that was used with IRLS2D_FLAG_GRAPH (the
+ * cache keeps at most 8 graphs, least recently used evicted first).
  */
 int irls2d_release(void *workspace);
+
+/*
+ * Test hook (no reference counterpart): from the next irls2d_solve on, in PCG iteration cg_iter
+ * (0-based) of IRLS iteration irls_iter, the search direction at pixel (0, 0) of every image whose
+ * PCG is still running is set to +inf after alpha is formed, which makes the update of x
+ * non-finite and exercises the reference's non-finite-x break and u_best restore
+ * (utils_unwrap2d.py:1444-1449, :1599-1604), unreachable from finite float32 inputs.  A negative
+ * argument disarms it.  While armed, solves run eagerly (IRLS2D_FLAG_GRAPH is ignored).
+ */
+int irls2d_debug_inject(int irls_iter, int cg_iter);
 
 /*
  * Per-kernel timing accumulated by calls made with IRLS2D_FLAG_TIME_KERNELS.  Fills up to `max`

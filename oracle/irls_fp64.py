@@ -56,18 +56,14 @@ def eigenvalues(H, W, dtype=np.float64):
     return lam
 
 
-# pocketfft threads over independent lines only, so the result does not depend on this
-WORKERS = -1
-
-
 def dct2(x):
     """Row DCT-II then column DCT-II (utils_unwrap2d.py:1337-1350)."""
-    return sfft.dct(sfft.dct(x, type=2, axis=1, workers=WORKERS), type=2, axis=0, workers=WORKERS)
+    return sfft.dct(sfft.dct(x, type=2, axis=1), type=2, axis=0)
 
 
 def idct2(X):
     """Column inverse then row inverse (utils_unwrap2d.py:1352-1365)."""
-    return sfft.idct(sfft.idct(X, type=2, axis=0, workers=WORKERS), type=2, axis=1, workers=WORKERS)
+    return sfft.idct(sfft.idct(X, type=2, axis=0), type=2, axis=1)
 
 
 def laplacian(x, wx, wy):
@@ -104,10 +100,11 @@ def precondition(r, lam):
     return idct2(X)
 
 
-def pcg(b, wx, wy, x, lam, max_iter, tol):
+def pcg(b, wx, wy, x, lam, max_iter, tol, inject_cg=-1):
     """PCG with the reference's clamps and break tests (utils_unwrap2d.py:1405-1474).
 
-    Updates x in place; returns (x, n_iters, reason).
+    Updates x in place; returns (x, n_iters, reason).  `inject_cg` mirrors the library's test hook
+    (irls2d_debug_inject): in that iteration p[0, 0] = +inf once alpha is formed.
     """
     r = b - laplacian(x, wx, wy)
     if not np.isfinite(r).all():
@@ -125,6 +122,8 @@ def pcg(b, wx, wy, x, lam, max_iter, tol):
             reason = 'pAp'
             break
         alpha = float(np.clip(rz / pAp, -1e6, 1e6))
+        if i == inject_cg:
+            p[0, 0] = np.inf
         x += alpha * p
         r -= alpha * Ap
         if not np.isfinite(x).all():
@@ -146,11 +145,12 @@ def pcg(b, wx, wy, x, lam, max_iter, tol):
 
 
 def solve(phi, w, valid, max_iter=50, tol=1e-3, cg_max_iter=20, cg_tol=1e-4, epsilon=1e-2,
-          dtype=np.float64, trace=None):
+          dtype=np.float64, trace=None, inject=None):
     """The device part of irls_unwrap_2d (utils_unwrap2d.py:1251-1632) on a filled, cropped grid.
 
     phi, w: filled float arrays; valid: bool.  Returns u (dtype).
-    `trace`, when a list, receives one dict per IRLS iteration.
+    `trace`, when a list, receives one dict per IRLS iteration.  `inject` = (IRLS iteration, PCG
+    iteration) of the test hook (see `pcg`).
     """
     H, W = phi.shape
     tx, ty = targets(phi, dtype)
@@ -187,7 +187,8 @@ def solve(phi, w, valid, max_iter=50, tol=1e-3, cg_max_iter=20, cg_tol=1e-4, eps
         wxi = np.clip(wx / np.sqrt(rx * rx + eps_sq), 1e-6, 1e6)
         wyi = np.clip(wy / np.sqrt(ry * ry + eps_sq), 1e-6, 1e6)
         b = divergence_rhs(tx, ty, wxi, wyi)
-        u, n_cg, reason = pcg(b, wxi, wyi, u, lam, cg_max_iter, cg_tol)
+        u, n_cg, reason = pcg(b, wxi, wyi, u, lam, cg_max_iter, cg_tol,
+                              inject[1] if inject is not None and inject[0] == it else -1)
         if not np.isfinite(u).all():
             u = u_best.copy()
             if trace is not None:
@@ -204,7 +205,7 @@ def solve(phi, w, valid, max_iter=50, tol=1e-3, cg_max_iter=20, cg_tol=1e-4, eps
 
 
 def irls_unwrap_2d(phase, weight=None, max_iter=50, tol=1e-3, cg_max_iter=20, cg_tol=1e-4,
-                   epsilon=1e-2, conncomp_size=30, dtype=np.float64, return_raw=False):
+                   epsilon=1e-2, conncomp_size=30, dtype=np.float64, return_raw=False, inject=None):
     """Whole reference call (utils_unwrap2d.py:1082-1766) with the solver in `dtype`."""
     if phase.ndim != 2:
         raise ValueError(f"irls_unwrap_2d expects 2D phase array, got shape {phase.shape}")
@@ -213,7 +214,7 @@ def irls_unwrap_2d(phase, weight=None, max_iter=50, tol=1e-3, cg_max_iter=20, cg
         out = (np.full_like(phase, np.nan), np.zeros(phase.shape, np.uint16))
         return out + (None,) if return_raw else out
     crop, ph_c, nm, pf, wf = pro
-    u = solve(pf, wf, ~nm, max_iter, tol, cg_max_iter, cg_tol, epsilon, dtype)
+    u = solve(pf, wf, ~nm, max_iter, tol, cg_max_iter, cg_tol, epsilon, dtype, inject=inject)
     ep = post.epilogue(u, ph_c, nm, conncomp_size)
     if ep is None:
         out = (np.full(phase.shape, np.nan, np.float32), np.zeros(phase.shape, np.uint16))

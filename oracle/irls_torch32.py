@@ -58,12 +58,14 @@ def idct(X: torch.Tensor) -> torch.Tensor:
 
 
 def solve(phi_np, w_np, valid_np, max_iter=50, tol=1e-3, cg_max_iter=20, cg_tol=1e-4, epsilon=1e-2,
-          stats=None, dtype=torch.float32, device='cpu'):
+          stats=None, dtype=torch.float32, device='cpu', inject=None):
     """utils_unwrap2d.py:1247-1632 on a filled, cropped grid, float32 on the CPU device.
 
     `dtype` / `device` run the same torch operations in another precision or on a torch device
     (the reference itself runs them on whatever device torch is given): float64 gives an FP64
-    trajectory of the same algorithm for grids too large for the numpy oracle."""
+    trajectory of the same algorithm for grids too large for the numpy oracle.  `inject` = (IRLS
+    iteration, PCG iteration): p[0, 0] = +inf once alpha is formed there (the library's test hook
+    irls2d_debug_inject)."""
     phi = torch.from_numpy(np.ascontiguousarray(phi_np, np.float32)).to(device=device, dtype=dtype)
     w = torch.from_numpy(np.ascontiguousarray(w_np, np.float32)).to(device=device, dtype=dtype)
     valid = torch.from_numpy(np.ascontiguousarray(valid_np, bool)).to(device)
@@ -119,7 +121,7 @@ def solve(phi_np, w_np, valid_np, max_iter=50, tol=1e-3, cg_max_iter=20, cg_tol=
         X[0, 0] = 0.0
         return idct2(X)
 
-    def cg(b, wxi, wyi, x):
+    def cg(b, wxi, wyi, x, inject_cg=-1):
         n = 0
         r = b.clone()
         r.sub_(lap(x, wxi, wyi))
@@ -135,6 +137,8 @@ def solve(phi_np, w_np, valid_np, max_iter=50, tol=1e-3, cg_max_iter=20, cg_tol=
             if pAp.abs() < 1e-15 or not torch.isfinite(pAp):
                 break
             alpha = torch.clamp(rz / pAp, -1e6, 1e6).item()
+            if i == inject_cg:
+                p[0, 0] = float('inf')
             x.add_(p, alpha=alpha)
             r.sub_(Ap, alpha=alpha)
             if not torch.isfinite(x).all():
@@ -190,7 +194,7 @@ def solve(phi_np, w_np, valid_np, max_iter=50, tol=1e-3, cg_max_iter=20, cg_tol=
         b[1:, :].addcmul_(wyi[:-1, :], dy_t[:-1, :])
         b[:-1, :].addcmul_(wyi[:-1, :], dy_t[:-1, :], value=-1)
         del rx, ry
-        u, _ = cg(b, wxi, wyi, u)
+        u, _ = cg(b, wxi, wyi, u, inject[1] if inject is not None and inject[0] == it else -1)
         if not torch.isfinite(u).all():
             u.copy_(u_best)
             break
@@ -207,7 +211,8 @@ def solve(phi_np, w_np, valid_np, max_iter=50, tol=1e-3, cg_max_iter=20, cg_tol=
 
 
 def irls_unwrap_2d(phase, weight=None, max_iter=50, tol=1e-3, cg_max_iter=20, cg_tol=1e-4,
-                   epsilon=1e-2, conncomp_size=30, return_raw=False, dtype=torch.float32, device='cpu'):
+                   epsilon=1e-2, conncomp_size=30, return_raw=False, dtype=torch.float32, device='cpu',
+                   inject=None):
     """The reference call on the CPU device (utils_unwrap2d.py:1082-1766); `dtype` / `device` as
     in `solve` (the epilogue always runs on the host, as the reference's does)."""
     if phase.ndim != 2:
@@ -217,7 +222,7 @@ def irls_unwrap_2d(phase, weight=None, max_iter=50, tol=1e-3, cg_max_iter=20, cg
         out = (np.full_like(phase, np.nan), np.zeros(phase.shape, np.uint16))
         return out + (None,) if return_raw else out
     crop, ph_c, nm, pf, wf = pro
-    u = solve(pf, wf, ~nm, max_iter, tol, cg_max_iter, cg_tol, epsilon, dtype=dtype, device=device)
+    u = solve(pf, wf, ~nm, max_iter, tol, cg_max_iter, cg_tol, epsilon, dtype=dtype, device=device, inject=inject)
     ep = post.epilogue(u, ph_c, nm, conncomp_size)
     if ep is None:
         out = (np.full(phase.shape, np.nan, np.float32), np.zeros(phase.shape, np.uint16))

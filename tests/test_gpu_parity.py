@@ -75,6 +75,38 @@ def test_batched_equals_plain_bitwise(hip):
         assert np.array_equal(cb, cp)
 
 
+def test_result_independent_of_batch_mates(hip):
+    """Every strategy choice is per image (column pass, slab split, elements per thread, long
+    lines), so an image's solve is bitwise the same alone and in a batch mixing all of them: two
+    slab splits, direct (Bluestein) and transposed columns, rows above 8192 points (12 values per
+    thread) and below, and a long column."""
+    import torch
+    from core_amd import _lib, solve_device
+    from core_amd.unwrap2d import Workspace, image_view, pack
+    shapes = [(1024, 600), (2048, 130), (70, 90), (200, 333), (40, 10000), (13122, 15)]
+    print("strategies", _lib.describe_columns(shapes), flush=True)
+    ims = [_case(h, w, 100 + i, i % 2 == 0, 0.05 if i == 1 else 0.0) for i, (h, w) in enumerate(shapes)]
+
+    def solve(idx):
+        sh = [shapes[k] for k in idx]
+        ws = Workspace(sh, hip)
+        u, info = solve_device(pack([ims[k][0] for k in idx], ws, hip), pack([ims[k][1] for k in idx], ws, hip), sh,
+                               workspace=ws, **FIXED)
+        out = [image_view(u, ws, j).cpu().numpy().copy() for j in range(len(idx))]
+        ws.release()
+        torch.cuda.empty_cache()
+        return out
+
+    together = solve(list(range(len(shapes))))
+    for k in range(len(shapes)):
+        alone = solve([k])[0]
+        assert np.array_equal(together[k], alone, equal_nan=True), shapes[k]
+        assert np.isfinite(alone[~np.isnan(ims[k][0])]).all()
+    reordered = solve([5, 2, 0, 4, 1, 3])
+    for j, k in enumerate([5, 2, 0, 4, 1, 3]):
+        assert np.array_equal(reordered[j], together[k], equal_nan=True), shapes[k]
+
+
 def test_graph_replay_equals_eager(hip):
     import torch
     from core_amd import solve_device

@@ -32,7 +32,7 @@ def test_unwrap_stack_equals_per_pair_calls(hip):
 def test_unwrap_chunked_equals_per_tile_calls(hip):
     """Tiled unwrapping with overlap (all tiles of all pairs in one batched solve) equals the
     drop-in call on each extended tile, trimmed to its chunk, including an all-NaN tile."""
-    from core_amd import _lib, irls_unwrap_2d
+    from core_amd import irls_unwrap_2d
     from core_amd.stack import chunk_windows, unwrap_chunked
     pairs = [S.generate(S.Workload(100, 90, 80 + i, ramp_max=30.0, n_bumps=3, fault=i == 0, mask_frac=0.05))[:2]
              for i in range(2)]
@@ -40,16 +40,12 @@ def test_unwrap_chunked_equals_per_tile_calls(hip):
     co = np.stack([p[1] for p in pairs])
     ph[1, :48, :48] = np.nan                       # the first extended tile of pair 1 is all NaN
     kw = dict(max_iter=50, tol=1e-2, cg_max_iter=10, cg_tol=1e-3, epsilon=1e-2, conncomp_size=30)
-    try:
-        _lib.set_column_mode(_lib.COLUMN_TRANSPOSE)   # one column strategy for batch and single calls
-        print("unwrap_chunked 2 x 100 x 90, chunks (40, 40), overlap 8", flush=True)
-        out = unwrap_chunked(ph, co, chunks=(40, 40), overlap=8, device='cuda', **kw)
-        wins = chunk_windows(100, 90, (40, 40), 8)
-        for p in range(2):
-            for (y0, y1, x0, x1), (ey0, ey1, ex0, ex1) in wins:
-                u1, _ = irls_unwrap_2d(ph[p, ey0:ey1, ex0:ex1], co[p, ey0:ey1, ex0:ex1], device='cuda', **kw)
-                core = u1[y0 - ey0:y1 - ey0, x0 - ex0:x1 - ex0]
-                assert np.array_equal(out[p, y0:y1, x0:x1], core, equal_nan=True), (p, y0, x0)
-        assert np.isnan(out[1, :40, :40]).all()
-    finally:
-        _lib.set_column_mode(_lib.COLUMN_AUTO)
+    print("unwrap_chunked 2 x 100 x 90, chunks (40, 40), overlap 8", flush=True)
+    out = unwrap_chunked(ph, co, chunks=(40, 40), overlap=8, device='cuda', **kw)
+    wins = chunk_windows(100, 90, (40, 40), 8)
+    for p in range(2):
+        for (y0, y1, x0, x1), (ey0, ey1, ex0, ex1) in wins:
+            u1, _ = irls_unwrap_2d(ph[p, ey0:ey1, ex0:ex1], co[p, ey0:ey1, ex0:ex1], device='cuda', **kw)
+            core = u1[y0 - ey0:y1 - ey0, x0 - ex0:x1 - ex0]
+            assert np.array_equal(out[p, y0:y1, x0:x1], core, equal_nan=True), (p, y0, x0)
+    assert np.isnan(out[1, :40, :40]).all()

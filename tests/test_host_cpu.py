@@ -34,6 +34,17 @@ def test_library_exports_every_header_symbol(lib):
     assert L.irls2d_version() >= 100
 
 
+def test_column_strategy_is_per_image(lib):
+    """Each image's column strategy follows from its own shape, the same alone and in any batch:
+    power-of-two-split heights take the register slab passes (one launch per split), Bluestein
+    heights the direct column pass, the rest the transposes."""
+    shapes = [(1024, 4000), (1030, 5000), (1000, 4500), (2048, 300), (96, 130), (103, 64)]
+    alone = [lib.describe_columns([sh])[0] for sh in shapes]
+    assert alone == ["slab_register", "direct", "transpose", "slab_register", "transpose", "direct"]
+    assert lib.describe_columns(shapes) == alone
+    assert lib.describe_columns(shapes[::-1]) == alone[::-1]
+
+
 def test_arena_layout_pitch_and_alignment(lib):
     shapes = [(10, 20), (3, 65), (1, 1), (7, 128)]
     off, pitch, total = lib.arena_layout(shapes)
@@ -200,11 +211,12 @@ def test_autotune_candidates_are_valid(lib):
 
 
 def test_column_strategies(lib):
-    """Which column pass each image gets.  Automatic mode: the register-stage slab kernels for
-    heights from 256 on whose split is a power-of-two pair, for the split of the first such image
-    in the batch; the transposed pass for other heights (the batched-LDS slab kernels are
-    slower), the column kernel on the row-major field for Bluestein heights, the four-step long
-    path above 12288.  Slab mode: every 5-smooth height within the block limits."""
+    """Which column pass each image gets, from its own shape.  Automatic mode: the register-stage
+    slab kernels for heights from 256 on whose split is a power-of-two pair (one launch per split);
+    the transposed pass for other heights (the batched-LDS slab kernels are slower), the column
+    kernel on the row-major field for Bluestein heights, the four-step long path above 12288.
+    Slab mode: every 5-smooth height within the block limits (register kernels where the split
+    allows them)."""
     d = lib.describe_columns
     try:
         lib.set_column_mode(lib.COLUMN_AUTO)
@@ -212,17 +224,17 @@ def test_column_strategies(lib):
         assert d([(8192, 70), (8192, 3)]) == ["slab_register"] * 2
         assert d([(256, 256)]) == ["slab_register"]
         assert d([(8192, 70), (1000, 7)]) == ["slab_register", "transpose"]  # 1000: no power-of-two split
-        assert d([(8192, 70), (2048, 9), (8192, 5)]) == ["slab_register", "transpose", "slab_register"]
-        assert d([(2048, 9), (8192, 70)]) == ["slab_register", "transpose"]  # the first split wins
+        assert d([(8192, 70), (2048, 9), (8192, 5)]) == ["slab_register"] * 3  # two splits, two launches
+        assert d([(2048, 9), (8192, 70)]) == ["slab_register"] * 2
         assert d([(6000, 6000)]) == ["transpose"]
         assert d([(96, 130)]) == ["transpose"]                     # below the slab threshold
         assert d([(127, 211)]) == ["direct"]                       # Bluestein columns
         assert d([(12800, 8)]) == ["long"]                         # long sides: transposed two-pass path
         assert d([(1000, 12500)]) == ["transpose"]
-        assert d([(8192, 70), (127, 211), (12800, 8)]) == ["slab_register", "transpose", "long"]
+        assert d([(8192, 70), (127, 211), (12800, 8)]) == ["slab_register", "direct", "long"]
         lib.set_column_mode(lib.COLUMN_SLAB)
         assert d([(64, 48), (96, 130)]) == ["slab", "slab"]
-        assert d([(8192, 70), (1000, 7)]) == ["slab", "slab"]     # two splits: batched LDS FFTs
+        assert d([(8192, 70), (1000, 7)]) == ["slab_register", "slab"]  # 1000 = 40 x 25: batched LDS
         assert d([(12800, 8)]) == ["slab"]                         # 12800 = 160 x 80
         assert d([(40000, 64)]) == ["slab"]                        # 40000 = 250 x 160
         assert d([(177147, 8)]) == ["long"]                        # 3^11: no split within the block limits

@@ -181,6 +181,20 @@ def test_fp32_restatement_tracks_fp64():
     assert np.array_equal(cc32, cc64)
 
 
+def test_torch_restatement_in_float64_equals_fp64_oracle():
+    """The torch restatement run in float64 is the FP64 oracle's trajectory (same operations,
+    different libraries): it stands in for the numpy oracle where that is too slow (large grids,
+    run on the HIP device by the GPU tests)."""
+    for H, W, seed, fault in [(64, 80, 11, True), (37, 101, 12, False)]:
+        psi, coh, _ = S.generate(S.Workload(H, W, seed, ramp_max=30.0, fault=fault, mask_frac=0.05))
+        kw = dict(max_iter=10, tol=0, cg_max_iter=50, cg_tol=0)
+        _, _, r64 = O.irls_unwrap_2d(psi, coh, return_raw=True, **kw)
+        _, _, rt = T32.irls_unwrap_2d(psi, coh, return_raw=True, dtype=torch.float64, **kw)
+        assert rt.dtype == np.float64
+        v = ~np.isnan(psi[~np.all(np.isnan(psi), axis=1)][:, ~np.all(np.isnan(psi), axis=0)])
+        assert np.abs(rt - r64)[v].max() < 1e-9 * max(1.0, np.abs(r64).max())
+
+
 def test_epilogue_invariants():
     r = np.random.default_rng(12)
     H, W = 30, 40
@@ -278,3 +292,27 @@ def test_planted_rhs_sign_is_caught(monkeypatch):
     monkeypatch.setattr(O, "divergence_rhs", lambda tx, ty, wx, wy: -orig(tx, ty, wx, wy))
     with pytest.raises(AssertionError):
         test_rhs_equals_sparse_DtWt(4, 6)
+
+
+def test_injected_nonfinite_x_breaks_pcg_and_restores_best_iterate():
+    """The oracle side of the library's test hook: +inf planted in p makes x non-finite; PCG breaks
+    there ('x_nonfinite'), the IRLS loop restores u_best and stops, and u_best is the iterate of
+    the lowest residual so far (utils_unwrap2d.py:1444-1449, :1551-1558, :1599-1604)."""
+    psi, coh, _ = S.generate(S.Workload(40, 52, 21, ramp_max=20.0, fault=True, mask_frac=0.05))
+    pro = post.prologue(psi, coh)
+    _, _, nm, pf, wf = pro
+    kw = dict(max_iter=6, tol=0, cg_max_iter=12, cg_tol=0, epsilon=1e-2)
+    for inj in [(0, 0), (3, 5)]:
+        tr = []
+        u = O.solve(pf, wf, ~nm, trace=tr, inject=inj, **kw)
+        assert [t['reason'] for t in tr][-1] == 'x_nonfinite' and tr[-1]['stop'] == 'nonfinite'
+        assert len(tr) == inj[0] + 1 and np.isfinite(u).all()
+        # u_best: the centred iterate with the lowest residual among iterations 0 .. inj[0]
+        ref = []
+        O.solve(pf, wf, ~nm, trace=ref, **{**kw, 'max_iter': inj[0]})
+        best_it = int(np.argmin([t['residual'] for t in tr]))
+        assert best_it <= inj[0]
+        u_prefix = O.solve(pf, wf, ~nm, **{**kw, 'max_iter': best_it})
+        vf = (~nm).astype(np.float64)
+        expect = u_prefix - (u_prefix * vf).sum() / vf.sum()
+        np.testing.assert_allclose(u, expect, rtol=0, atol=1e-12)
