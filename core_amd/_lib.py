@@ -18,6 +18,8 @@ LIB_PATH = os.path.join(HERE, "libirls2d.so")
 OK, EINVAL, EUNSUPPORTED, EWORKSPACE, EHIP = 0, 1, 2, 3, 4
 FLAG_GRAPH = 1
 FLAG_TIME_KERNELS = 2
+# IRLS2D_STATIC_SCRATCH_BYTES of include/irls2d.h: the device scratch of irls2d_solve_static
+STATIC_SCRATCH_BYTES = 96 << 20
 
 EXPORTS = (
     "irls2d_version",
@@ -26,6 +28,7 @@ EXPORTS = (
     "irls2d_arena_layout",
     "irls2d_workspace_size",
     "irls2d_solve",
+    "irls2d_solve_static",
     "irls2d_adjoint",
     "irls2d_precondition",
     "irls2d_plan_override",
@@ -82,6 +85,13 @@ def load():
         ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint,
     ]
     L.irls2d_solve.restype = ctypes.c_int
+    L.irls2d_solve_static.argtypes = [
+        ctypes.c_int, i64p, i64p,
+        ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+        ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+        ctypes.c_void_p, ctypes.c_void_p,
+    ]
+    L.irls2d_solve_static.restype = ctypes.c_int
     L.irls2d_adjoint.argtypes = [
         ctypes.c_int, i64p, i64p,
         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,

@@ -1300,6 +1300,68 @@ int solve(int n, const int64_t *H, const int64_t *W, const float *phase, const f
   return IRLS2D_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Solve without a caller workspace (irls2d_solve_static): the workspace is the library's static
+// device array `scratch`, one per GPU the library runs on, held until the process ends.  Host code
+// never passes the array's name to the device: the address comes from hipGetSymbolAddress, per
+// device, on that device's first call.
+// ---------------------------------------------------------------------------------------------
+
+static __device__ __align__(256) unsigned char scratch[IRLS2D_STATIC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static std::map<int, void *> g_scratch_ptr;  // device -> address of `scratch` on that device
+
+static int scratch_address(void **out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(IRLS2D_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    it = g_scratch_ptr.emplace(dev, p).first;
+  }
+  *out = it->second;
+  return IRLS2D_OK;
+}
+
+int solve_static(int n, const int64_t *H, const int64_t *W, const float *phase, const float *weight, float *out,
+                 int max_iter, double tol, int cg_max_iter, double cg_tol, double epsilon, int32_t *info,
+                 void *stream) {
+  g_err.clear();
+  if (!phase || !out) return fail(IRLS2D_EINVAL, "null phase or output pointer");
+  if (max_iter < 0 || cg_max_iter < 0) return fail(IRLS2D_EINVAL, "iteration counts must be >= 0");
+  if (!(epsilon >= 0) || !std::isfinite(epsilon)) return fail(IRLS2D_EINVAL, "epsilon must be finite and >= 0");
+  if (std::isnan(tol) || std::isnan(cg_tol)) return fail(IRLS2D_EINVAL, "tol and cg_tol must not be NaN");
+  std::shared_ptr<Layout> Lo;
+  int rc = get_layout(n, H, W, Lo);
+  if (rc) return rc;
+  if (Lo->bytes > (size_t)IRLS2D_STATIC_SCRATCH_BYTES)
+    return fail(IRLS2D_EWORKSPACE, "these shapes need a workspace of " + std::to_string(Lo->bytes) +
+                                       " bytes; the static scratch holds " +
+                                       std::to_string((size_t)IRLS2D_STATIC_SCRATCH_BYTES));
+  void *ws = nullptr;
+  rc = scratch_address(&ws);
+  if (rc) return rc;
+  rc = set_attrs();
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  // metadata straight from the layout's host image (pageable; the stream is synchronised after
+  // the copy, so nothing is staged in memory the library would have to allocate)
+  if (!Lo->meta.empty()) {
+    HIP_TRY(hipMemcpyAsync((unsigned char *)ws + Lo->o_meta, Lo->meta.data(), Lo->meta.size(), hipMemcpyHostToDevice,
+                           s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  Params p{max_iter, cg_max_iter, tol, cg_tol, epsilon};
+  Ctx c = make_ctx(*Lo, (unsigned char *)ws, out, p);
+  enqueue(*Lo, c, phase, weight, info, p, s);
+  HIP_TRY(hipGetLastError());
+  return IRLS2D_OK;
+}
+
 static void enqueue_adjoint(const Layout &Lo, const Ctx &c, const float *phase, const float *weight,
                             const float *u, const float *g, float *gphase, float *gweight, int32_t *info,
                             int cg_max_iter, hipStream_t s) {
@@ -1755,6 +1817,13 @@ int irls2d_debug_inject(int irls_iter, int cg_iter) {
   irls2d::g_inject_irls = irls_iter;
   irls2d::g_inject_cg = cg_iter;
   return IRLS2D_OK;
+}
+
+int irls2d_solve_static(int n, const int64_t *H, const int64_t *W, const float *phase, const float *weight,
+                        float *unwrapped, int max_iter, double tol, int cg_max_iter, double cg_tol, double epsilon,
+                        int32_t *info, void *stream) {
+  return irls2d::solve_static(n, H, W, phase, weight, unwrapped, max_iter, tol, cg_max_iter, cg_tol, epsilon, info,
+                              stream);
 }
 
 int64_t irls2d_ep_blocks(int64_t H, int64_t P) { return irls2d::ep_blocks(H, P); }

@@ -99,6 +99,25 @@ int irls2d_solve(int n, const int64_t *H, const int64_t *W,
                  void *stream, unsigned flags);
 
 /*
+ * irls2d_solve without a caller workspace: the same launch sequence on the library's static device
+ * scratch array of IRLS2D_STATIC_SCRATCH_BYTES per GPU (allocated with the library's code object on
+ * each GPU it runs on and held until the process ends).  No counterpart in the reference: for
+ * callers that cannot allocate device memory (serving loops, harnesses that forbid hipMalloc).  The
+ * call allocates no device memory (no hipMalloc / hipFree / allocating library routine); it
+ * uploads the transform tables from pageable host memory and synchronises `stream` once before
+ * launching, then runs eagerly (no hipGraph).  Shapes whose irls2d_workspace_size exceeds
+ * IRLS2D_STATIC_SCRATCH_BYTES (about 1.5 Mpx in total, e.g. one 1024 x 1024 image) are rejected
+ * with IRLS2D_EWORKSPACE.  Every call on a GPU uses that GPU's one array: calls that overlap on a
+ * GPU (other threads or streams) race and must be serialised by the caller.  Nothing in the array
+ * needs to start at zero (every value is written before it is read within a call).
+ */
+#define IRLS2D_STATIC_SCRATCH_BYTES (96ull << 20)
+int irls2d_solve_static(int n, const int64_t *H, const int64_t *W,
+                        const float *phase, const float *weight, float *unwrapped,
+                        int max_iter, double tol, int cg_max_iter, double cg_tol, double epsilon,
+                        int32_t *info, void *stream);
+
+/*
  * Implicit adjoint of the solve (no reference counterpart: the reference is not differentiable).
  * Given the inputs of irls2d_solve, its output `unwrapped` (the IRLS fixed point u*) and
  * grad_unwrapped = dL/d(C u*) with C the centring u - mean_valid(u), writes

@@ -34,6 +34,15 @@ def test_library_exports_every_header_symbol(lib):
     assert L.irls2d_version() >= 100
 
 
+def test_static_scratch_size_matches_header(lib):
+    """The Python constant of irls2d_solve_static's scratch is the header's, and it holds a
+    1024 x 1024 solve but not a 2048 x 2048 one."""
+    text = open(os.path.join(ROOT, "include", "irls2d.h")).read()
+    m = re.search(r"#define IRLS2D_STATIC_SCRATCH_BYTES \((\d+)ull << (\d+)\)", text)
+    assert m and int(m.group(1)) << int(m.group(2)) == lib.STATIC_SCRATCH_BYTES
+    assert lib.workspace_size([(1024, 1024)]) <= lib.STATIC_SCRATCH_BYTES < lib.workspace_size([(2048, 2048)])
+
+
 def test_column_strategy_is_per_image(lib):
     """Each image's column strategy follows from its own shape, the same alone and in any batch:
     power-of-two-split heights take the register slab passes (one launch per split), Bluestein
