@@ -105,17 +105,28 @@ struct DctPlan {
 // sits at row k1, column k2 of the scratch line.  Pass 1 blocks take a set of columns closed
 // under the reflection that pairs their inputs, pass 2 blocks a set of rows closed under the one
 // that pairs their outputs, so every global element is read and written once, coalesced.
+//
+// blue = 1: N has no such split (a prime factor above 5, or factors above 1024).  The length-N FFT
+// of the packed sequence then runs as Bluestein's chirp convolution of length M = N1 N2 >= 2N - 1
+// (5-smooth, both factors <= 1024): a pack pass writes the sequence to the scratch line Sa, three
+// four-step passes (k_lb_fft1 / k_lb_fft2 / k_lb_fft3) turn it into its spectrum in place through
+// the scratch Sb of M values, and a post pass forms the DCT coefficients or samples.
 struct LongPlan {
-  int N, N1, N2;
+  int N, N1, N2;   // line length; the FFT factors (of N, or of M for Bluestein)
   int f1, f2;      // DctPlan indices whose FftPlans transform lengths N1 and N2
-  int tw_off;      // split twiddle table of length N: exp(-2 pi i b / N), b < 64, then exp(-2 pi i 64 a / N)
+  int tw_off;      // split twiddle table of length N (M for Bluestein): exp(-2 pi i b / L), b < 64, then exp(-2 pi i 64 a / L)
   int mk_off;      // N entries exp(-i pi k / 2N)
   int lb1, lb2;    // log2 of the FFTs per block (slots) in pass 1 and pass 2
-  int nb1, nb2;    // blocks per line pair in pass 1 and pass 2
+  int nb1, nb2;    // blocks per line pair in pass 1 and pass 2 (Bluestein: fft1 / fft3, and fft2)
   int spitch;      // float2 elements per line pair in the scratch arena
   float invN;
   float inv_n2;    // 1 / N2 for float quotients
-  int pad;
+  int blue;        // 1: Bluestein (below)
+  int M;           // Bluestein convolution length N1 N2
+  int chirp_off;   // Bluestein: N entries exp(-i pi n^2 / N)
+  int bhat_off;    // Bluestein: M entries FFT_M(conj chirp, wrapped) / M
+  int sb_off;      // Bluestein: float2 offset of Sb inside a line pair's scratch (Sa first)
+  int nbp;         // Bluestein: pack / post blocks per line pair
 };
 
 // Column DCT pair of length N = N1 N2 on slabs of the row-major field (k_slab1/2/3).
@@ -147,6 +158,8 @@ struct ImgDesc {
   int nslab;       // slabs of the image: ceil(W / (2 SLAB_CP))
   int rowpart0;    // first block-partial slot of the image's row-pair blocks (short rows)
   int coldirect;   // 1: the column DCTs read column pairs straight from the row-major field
+  int lbpart;      // Bluestein long rows: first slot of the pack / post block partials (partials2)
+  int pad;
 };
 
 // Per-image solver state.  Written by k_state_init and k_finalize (one block per image), read by
@@ -186,6 +199,8 @@ struct Geo {
   const int *tile, *rowpair, *colpair, *trans;
   const int *rowL1, *rowL2, *colL1, *colL2;  // long-line passes (images with W or H > MAX_FFT_LEN)
   const int *slab1, *slab2;                  // slab column passes: K1 / K3, and K2
+  // Bluestein long lines: pack / post, fft1 / fft3 and fft2 blocks, rows then columns
+  const int *browP, *brow1, *brow2, *bcolP, *bcol1, *bcol2;
 };
 
 struct Ctx {

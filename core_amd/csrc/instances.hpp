@@ -20,7 +20,16 @@
   X template __global__ void k_long_p1<LONG_DCT3, LONG_E>(Ctx, int, float *, const float *, int); \
   X template __global__ void k_long_p1<LONG_DCT3_SCALE, LONG_E>(Ctx, int, float *, const float *, int); \
   X template __global__ void k_long_p2<LONG_DCT2, LONG_E>(Ctx, int, float *, int, int);           \
-  X template __global__ void k_long_p2<LONG_DCT3, LONG_E>(Ctx, int, float *, int, int);
+  X template __global__ void k_long_p2<LONG_DCT3, LONG_E>(Ctx, int, float *, int, int);           \
+  X template __global__ void k_lb_pack<LONG_DCT2>(Ctx, int, float *, const float *, int);         \
+  X template __global__ void k_lb_pack<LONG_DCT2_UPDATE>(Ctx, int, float *, const float *, int);  \
+  X template __global__ void k_lb_pack<LONG_DCT3>(Ctx, int, float *, const float *, int);         \
+  X template __global__ void k_lb_pack<LONG_DCT3_SCALE>(Ctx, int, float *, const float *, int);   \
+  X template __global__ void k_lb_fft13<1, LONG_E>(Ctx, int, int);                                \
+  X template __global__ void k_lb_fft13<3, LONG_E>(Ctx, int, int);                                \
+  X template __global__ void k_lb_fft2<LONG_E>(Ctx, int, int);                                    \
+  X template __global__ void k_lb_post<LONG_DCT2>(Ctx, int, float *, int, int);                   \
+  X template __global__ void k_lb_post<LONG_DCT3>(Ctx, int, float *, int, int);
 
 #define IRLS2D_SLAB_KERNELS(X)                                                                    \
   X template __global__ void k_slab1<SLAB_E>(Ctx, const float *, int);                            \

@@ -170,11 +170,48 @@ static bool long_split(int64_t N, int &N1, int &N2) {
   return true;
 }
 
+// The most balanced split M = M1 M2 (M1 >= M2, both <= LONG_MAX_FACTOR) of a 5-smooth M, or false.
+static bool factor_split(int64_t M, int &M1, int &M2) {
+  if (!smooth235(M)) return false;
+  int64_t best = -1;
+  for (int64_t b = 2; b <= LONG_MAX_FACTOR; ++b) {
+    if (M % b) continue;
+    const int64_t a = M / b;
+    if (a > LONG_MAX_FACTOR || a < b) continue;
+    if (best < 0 || a < best) best = a;
+  }
+  if (best < 0) return false;
+  M1 = (int)best;
+  M2 = (int)(M / best);
+  return true;
+}
+
+// Bluestein length of a line longer than MAX_FFT_LEN without a smooth split: the 5-smooth
+// M >= 2N - 1 (within 30%) that splits into two factors <= LONG_MAX_FACTOR, lowest
+// M * (passes of M1 + passes of M2); 0 when there is none (N above about 2^19).
+static int64_t long_blue_len(int64_t N, int &M1, int &M2) {
+  const int64_t lo = 2 * N - 1;
+  int64_t best = 0;
+  double best_cost = 0;
+  for (int64_t M = lo; M <= lo + lo * 3 / 10 + 8 && M <= (int64_t)LONG_MAX_FACTOR * LONG_MAX_FACTOR; ++M) {
+    int a, b;
+    if (!factor_split(M, a, b)) continue;
+    const double cost = (double)M * (double)(radices(a).size() + radices(b).size());
+    if (!best || cost < best_cost) {
+      best = M;
+      best_cost = cost;
+      M1 = a;
+      M2 = b;
+    }
+  }
+  return best;
+}
+
 int supported(int64_t N) {
   if (N < 1) return 0;
   if (N > MAX_FFT_LEN) {
     int a, b;
-    return long_split(N, a, b) ? 1 : 0;
+    return (long_split(N, a, b) || long_blue_len(N, a, b) > 0) ? 1 : 0;
   }
   if (smooth235(N)) return 1;
   return bluestein_len(N) <= MAX_FFT_LEN;
@@ -284,31 +321,49 @@ static std::shared_ptr<HostLong> get_long_plan(int N) {
   auto h = std::make_shared<HostLong>();
   LongPlan &P = h->plan;
   std::memset(&P, 0, sizeof(P));
-  long_split(N, P.N1, P.N2);
   P.N = N;
   P.invN = (float)(1.0 / (double)N);
+  // the FFT length of the four-step passes: N itself, or the Bluestein convolution length
+  int64_t L = N;
+  if (!long_split(N, P.N1, P.N2)) {
+    L = long_blue_len(N, P.N1, P.N2);
+    P.blue = 1;
+    P.M = (int)L;
+  }
   P.inv_n2 = (float)(1.0 / (double)P.N2);
+  auto push = [&](double ang) { h->tab.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang))); };
   P.tw_off = 0;
-  for (int b = 0; b < TW_LO; ++b) {
-    const double ang = -2.0 * M_PI * (double)b / (double)N;
-    h->tab.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
-  }
-  for (int a = 0; a < (N + 63) / 64; ++a) {
-    const double ang = -2.0 * M_PI * (double)(64 * (int64_t)a) / (double)N;
-    h->tab.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
-  }
+  for (int b = 0; b < TW_LO; ++b) push(-2.0 * M_PI * (double)b / (double)L);
+  for (int64_t a = 0; a < (L + 63) / 64; ++a) push(-2.0 * M_PI * (double)(64 * a) / (double)L);
   P.mk_off = (int)h->tab.size();
-  for (int k = 0; k < N; ++k) {
-    const double ang = -M_PI * (double)k / (2.0 * (double)N);
-    h->tab.push_back(make_float2((float)std::cos(ang), (float)std::sin(ang)));
-  }
+  for (int k = 0; k < N; ++k) push(-M_PI * (double)k / (2.0 * (double)N));
   P.lb1 = batch_log2(P.N1);
   P.lb2 = batch_log2(P.N2);
-  // representatives per block = half the slots; the reflection k -> N - k has the most of them
-  const int C1 = (1 << P.lb1) / 2, C2 = (1 << P.lb2) / 2;
-  P.nb1 = (P.N2 / 2 + 1 + C1 - 1) / C1;
-  P.nb2 = (P.N1 / 2 + 1 + C2 - 1) / C2;
-  P.spitch = (N + 31) / 32 * 32;
+  if (P.blue) {
+    // chirp w[n] = exp(-i pi n^2 / N) and H = FFT_M(conj w wrapped) / M, from float64
+    std::vector<cd> ch(N);
+    for (int64_t n = 0; n < N; ++n) ch[n] = std::polar(1.0, -M_PI * (double)((n * n) % (2 * (int64_t)N)) / (double)N);
+    P.chirp_off = (int)h->tab.size();
+    for (const cd &z : ch) h->tab.push_back(make_float2((float)z.real(), (float)z.imag()));
+    std::vector<cd> b(L, cd(0, 0));
+    for (int64_t n = 0; n < N; ++n) b[n] = std::conj(ch[n]);
+    for (int64_t n = 1; n < N; ++n) b[L - n] = std::conj(ch[n]);
+    host_fft(b);
+    P.bhat_off = (int)h->tab.size();
+    for (const cd &z : b) h->tab.push_back(make_float2((float)(z.real() / (double)L), (float)(z.imag() / (double)L)));
+    // fft1 / fft3: 2^lb1 columns per block; fft2: 2^lb2 rows per block; pack / post: LB_CHUNK
+    P.nb1 = (P.N2 + (1 << P.lb1) - 1) >> P.lb1;
+    P.nb2 = (P.N1 + (1 << P.lb2) - 1) >> P.lb2;
+    P.nbp = (N + LB_CHUNK - 1) / LB_CHUNK;
+    P.sb_off = (N + 31) / 32 * 32;
+    P.spitch = P.sb_off + (int)((L + 31) / 32 * 32);
+  } else {
+    // representatives per block = half the slots; the reflection k -> N - k has the most of them
+    const int C1 = (1 << P.lb1) / 2, C2 = (1 << P.lb2) / 2;
+    P.nb1 = (P.N2 / 2 + 1 + C1 - 1) / C1;
+    P.nb2 = (P.N1 / 2 + 1 + C2 - 1) / C2;
+    P.spitch = (N + 31) / 32 * 32;
+  }
   g_lplans[N] = h;
   return h;
 }
@@ -408,7 +463,8 @@ constexpr int64_t FIELD_PAD = 149 * 64;
 static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Fixed block-start tables of a layout; the class tables (LaunchClass::geo / geo2) follow them.
-enum { G_TILE = 0, G_TRANS, G_ROWL1, G_ROWL2, G_COLL1, G_COLL2, G_FIXED };
+enum { G_TILE = 0, G_TRANS, G_ROWL1, G_ROWL2, G_COLL1, G_COLL2, G_BROWP, G_BROW1, G_BROW2, G_BCOLP, G_BCOL1,
+       G_BCOL2, G_FIXED };
 
 // Images whose transform pass shares one kernel instantiation: one launch per class and pass.
 // Every choice is a function of the image's own shape (and the global mode / plan overrides), so an
@@ -488,9 +544,7 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     if (H[i] > (int64_t)1 << 30 || W[i] > (int64_t)1 << 30) return fail(IRLS2D_EINVAL, "dimension too large");
     if (!supported(H[i]) || !supported(W[i]))
       return fail(IRLS2D_EUNSUPPORTED, "grid " + std::to_string(H[i]) + "x" + std::to_string(W[i]) +
-                                           ": sides above " + std::to_string(MAX_FFT_LEN) +
-                                           " must be 2^a 3^b 5^c with two factors <= " +
-                                           std::to_string(LONG_MAX_FACTOR));
+                                           ": a side is longer than the two-pass transforms reach (about 2^19)");
   }
   Lo.sig = shape_sig(n, H, W);
   std::map<int, int> plan_idx, long_idx, cos_off;
@@ -518,6 +572,10 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     Lo.tab.insert(Lo.tab.end(), h->tab.begin(), h->tab.end());
     P.tw_off += base;
     P.mk_off += base;
+    if (P.blue) {
+      P.chirp_off += base;
+      P.bhat_off += base;
+    }
     P.f1 = add_plan(P.N1);
     P.f2 = add_plan(P.N2);
     Lo.lplans.push_back(P);
@@ -642,15 +700,29 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     int64_t sz = 0;
     if (lw) {
       const LongPlan &P = Lo.lplans[d.rowlong];
-      sz = std::max<int64_t>(sz, (int64_t)((h + 1) / 2) * P.spitch);
-      adds[i].push_back({G_ROWL1, (int64_t)((h + 1) / 2) * P.nb1});
-      adds[i].push_back({G_ROWL2, (int64_t)((h + 1) / 2) * P.nb2});
+      const int64_t lp = (h + 1) / 2;
+      sz = std::max<int64_t>(sz, lp * P.spitch);
+      if (P.blue) {
+        adds[i].push_back({G_BROWP, lp * P.nbp});
+        adds[i].push_back({G_BROW1, lp * P.nb1});
+        adds[i].push_back({G_BROW2, lp * P.nb2});
+      } else {
+        adds[i].push_back({G_ROWL1, lp * P.nb1});
+        adds[i].push_back({G_ROWL2, lp * P.nb2});
+      }
     }
     if (d.collong >= 0) {
       const LongPlan &P = Lo.lplans[d.collong];
-      sz = std::max<int64_t>(sz, (int64_t)((w + 1) / 2) * P.spitch);
-      adds[i].push_back({G_COLL1, (int64_t)((w + 1) / 2) * P.nb1});
-      adds[i].push_back({G_COLL2, (int64_t)((w + 1) / 2) * P.nb2});
+      const int64_t lp = (w + 1) / 2;
+      sz = std::max<int64_t>(sz, lp * P.spitch);
+      if (P.blue) {
+        adds[i].push_back({G_BCOLP, lp * P.nbp});
+        adds[i].push_back({G_BCOL1, lp * P.nb1});
+        adds[i].push_back({G_BCOL2, lp * P.nb2});
+      } else {
+        adds[i].push_back({G_COLL1, lp * P.nb1});
+        adds[i].push_back({G_COLL2, lp * P.nb2});
+      }
     }
     d.offS = Lo.totalS;
     Lo.totalS += (sz + 31) / 32 * 32;
@@ -695,8 +767,12 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     }
   }
   Lo.max_blocks = std::max(std::max(Lo.geo[G_TILE].back(), rowparts), 1);
+  // partials2: the two-pass long kernels by block index, then the Bluestein long rows' pack / post
+  // blocks from each image's lbpart
   Lo.max_blocks2 = std::max(std::max(Lo.geo[G_ROWL1].back(), Lo.geo[G_ROWL2].back()),
                             std::max(Lo.geo[G_COLL1].back(), Lo.geo[G_COLL2].back()));
+  for (int i = 0; i < n; ++i) Lo.desc[i].lbpart = Lo.max_blocks2 + Lo.geo[G_BROWP][i];
+  Lo.max_blocks2 += Lo.geo[G_BROWP].back();
   // launch shapes per class: the workgroup covers the class's longest transform
   auto shape_of = [&](LaunchClass &C) {
     int t = (C.maxM + C.E - 1) / C.E;
@@ -836,6 +912,12 @@ static Ctx make_ctx(const Layout &Lo, unsigned char *ws, float *u, const Params 
   c.geo.rowL2 = (const int *)(ws + Lo.o_geo[G_ROWL2]);
   c.geo.colL1 = (const int *)(ws + Lo.o_geo[G_COLL1]);
   c.geo.colL2 = (const int *)(ws + Lo.o_geo[G_COLL2]);
+  c.geo.browP = (const int *)(ws + Lo.o_geo[G_BROWP]);
+  c.geo.brow1 = (const int *)(ws + Lo.o_geo[G_BROW1]);
+  c.geo.brow2 = (const int *)(ws + Lo.o_geo[G_BROW2]);
+  c.geo.bcolP = (const int *)(ws + Lo.o_geo[G_BCOLP]);
+  c.geo.bcol1 = (const int *)(ws + Lo.o_geo[G_BCOL1]);
+  c.geo.bcol2 = (const int *)(ws + Lo.o_geo[G_BCOL2]);
   // class tables: set per launch (class_ctx)
   c.geo.rowpair = c.geo.colpair = c.geo.slab1 = c.geo.slab2 = nullptr;
   c.ws_base = ws;
@@ -896,6 +978,39 @@ static void long_pass2(const Layout &Lo, const Ctx &c, int dir, int mode, float 
            s, c, dir, dst, dot, guard);
 }
 
+// Bluestein long lines: the three four-step passes of the chirp convolution (dir 0 rows, 1 columns).
+static void lb_ffts(const Layout &Lo, const Ctx &c, int dir, int guard, hipStream_t s) {
+  const dim3 b(LONG_THREADS), g1(Lo.geo[dir ? G_BCOL1 : G_BROW1].back()), g2(Lo.geo[dir ? G_BCOL2 : G_BROW2].back());
+  LAUNCH("k_lb_fft1", (k_lb_fft13<1, LONG_E>), g1, b, Lo.long_lds, s, c, dir, guard);
+  LAUNCH("k_lb_fft2", (k_lb_fft2<LONG_E>), g2, b, Lo.long_lds, s, c, dir, guard);
+  LAUNCH("k_lb_fft3", (k_lb_fft13<3, LONG_E>), g1, b, Lo.long_lds, s, c, dir, guard);
+}
+
+// Bluestein long DCT-II of rows (dir 0: src field rows -> dst T rows, with the PCG update when
+// `fuse`) or columns (dir 1: Tt -> Tt).
+static void lb_dct2(const Layout &Lo, const Ctx &c, int dir, bool fuse, float *src, float *dst, const float *p,
+                    int guard, hipStream_t s) {
+  const dim3 gp(Lo.geo[dir ? G_BCOLP : G_BROWP].back()), b(LONG_THREADS);
+  if (gp.x == 0) return;
+  if (fuse) LAUNCH("k_lb_pack_update", (k_lb_pack<LONG_DCT2_UPDATE>), gp, b, 0, s, c, dir, src, p, guard);
+  else LAUNCH("k_lb_pack_dct2", (k_lb_pack<LONG_DCT2>), gp, b, 0, s, c, dir, src, p, guard);
+  lb_ffts(Lo, c, dir, guard, s);
+  LAUNCH("k_lb_post_dct2", (k_lb_post<LONG_DCT2>), gp, b, 0, s, c, dir, dst, 0, guard);
+}
+
+// Bluestein long DCT-III (dir 0: T rows -> dst field rows, r.dst partials when `dot`; dir 1: Tt ->
+// Tt with the -1/lambda scaling of the preconditioner first).
+static void lb_dct3(const Layout &Lo, const Ctx &c, int dir, const float *src, float *dst, int dot, int guard,
+                    hipStream_t s) {
+  const dim3 gp(Lo.geo[dir ? G_BCOLP : G_BROWP].back()), b(LONG_THREADS);
+  if (gp.x == 0) return;
+  float *sp = const_cast<float *>(src);
+  if (dir) LAUNCH("k_lb_pack_scale", (k_lb_pack<LONG_DCT3_SCALE>), gp, b, 0, s, c, dir, sp, nullptr, guard);
+  else LAUNCH("k_lb_pack_dct3", (k_lb_pack<LONG_DCT3>), gp, b, 0, s, c, dir, sp, nullptr, guard);
+  lb_ffts(Lo, c, dir, guard, s);
+  LAUNCH("k_lb_post_dct3", (k_lb_post<LONG_DCT3>), gp, b, 0, s, c, dir, dst, dot, guard);
+}
+
 // Context of one class launch: the class's block-start tables in place of rowpair / colpair /
 // slab1 / slab2.
 static Ctx class_ctx(const Layout &Lo, const Ctx &c, const LaunchClass &C) {
@@ -912,6 +1027,7 @@ static void row_dct2(const Layout &Lo, const Ctx &c, bool fuse, float *src, floa
     long_pass1(Lo, c, 0, fuse ? LONG_DCT2_UPDATE : LONG_DCT2, src, p, guard, s);
     long_pass2(Lo, c, 0, LONG_DCT2, dst, 0, guard, s);
   }
+  lb_dct2(Lo, c, 0, fuse, src, dst, p, guard, s);
   for (const LaunchClass &C : Lo.rowcls) {
     const int nb = Lo.geo[C.geo].back();
     if (nb == 0) continue;
@@ -934,6 +1050,7 @@ static void row_dct3(const Layout &Lo, const Ctx &c, const float *src, float *ds
     long_pass1(Lo, c, 0, LONG_DCT3, const_cast<float *>(src), nullptr, guard, s);
     long_pass2(Lo, c, 0, LONG_DCT3, dst, dot, guard, s);
   }
+  lb_dct3(Lo, c, 0, src, dst, dot, guard, s);
   for (const LaunchClass &C : Lo.rowcls) {
     const int nb = Lo.geo[C.geo].back();
     if (nb == 0) continue;
@@ -985,6 +1102,8 @@ static void col_pass(const Layout &Lo, const Ctx &c, int guard, hipStream_t s) {
     long_pass1(Lo, c, 1, LONG_DCT3_SCALE, c.f.Tt, nullptr, guard, s);
     long_pass2(Lo, c, 1, LONG_DCT3, c.f.Tt, 0, guard, s);
   }
+  lb_dct2(Lo, c, 1, false, c.f.Tt, c.f.Tt, nullptr, guard, s);
+  lb_dct3(Lo, c, 1, c.f.Tt, c.f.Tt, 0, guard, s);
   for (const LaunchClass &C : Lo.colcls) {
     const int nb = Lo.geo[C.geo].back();
     if (nb == 0) continue;
@@ -1166,7 +1285,9 @@ static int set_attrs() {
   const int llds = LONG_LDS * (int)sizeof(float2);
   const void *lfns[] = {(const void *)k_long_p1<LONG_DCT2, LONG_E>, (const void *)k_long_p1<LONG_DCT2_UPDATE, LONG_E>,
                         (const void *)k_long_p1<LONG_DCT3, LONG_E>, (const void *)k_long_p1<LONG_DCT3_SCALE, LONG_E>,
-                        (const void *)k_long_p2<LONG_DCT2, LONG_E>, (const void *)k_long_p2<LONG_DCT3, LONG_E>};
+                        (const void *)k_long_p2<LONG_DCT2, LONG_E>, (const void *)k_long_p2<LONG_DCT3, LONG_E>,
+                        (const void *)k_lb_fft13<1, LONG_E>, (const void *)k_lb_fft13<3, LONG_E>,
+                        (const void *)k_lb_fft2<LONG_E>};
   for (const void *f : lfns) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, llds));
   const int slds = (TW_LDS + SLAB_E * SLAB_MAX_THREADS + 2 * SLAB_MAX_THREADS) * (int)sizeof(float2);
   const void *sfns[] = {(const void *)k_slab1<SLAB_E>, (const void *)k_slab2<SLAB_E>, (const void *)k_slab3<SLAB_E>};

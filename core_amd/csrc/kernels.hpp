@@ -1003,6 +1003,10 @@ __global__ __launch_bounds__(256) void k_finalize(Ctx c, int kind, int it) {
   if (rowgeo && !lng) {
     b0 = dsc.rowpart0;
     b1 = b0 + (dsc.H + 1) / 2;
+  } else if (lng && c.lplans[dsc.rowlong].blue) {
+    // Bluestein long rows: the pack (update) and post (r.z) passes share one block numbering
+    b0 = dsc.lbpart;
+    b1 = b0 + ((dsc.H + 1) / 2) * c.lplans[dsc.rowlong].nbp;
   } else {
     const int *geo = rowgeo ? (kind == FIN_UPDATE ? c.geo.rowL1 : c.geo.rowL2) : c.geo.tile;
     b0 = geo[img];

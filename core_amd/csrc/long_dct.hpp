@@ -153,14 +153,18 @@ struct LongLine {
   float2 *S;         // scratch line of this pair
 };
 
-__device__ __forceinline__ bool long_line(const Ctx &c, int dir, int pass, int guard, LongLine &ln) {
-  const int *geo = dir ? (pass == 1 ? c.geo.colL1 : c.geo.colL2) : (pass == 1 ? c.geo.rowL1 : c.geo.rowL2);
+// The line pair and block of this workgroup in a launch over `geo`, whose images have `which`
+// blocks per line pair: 1 = nb1, 2 = nb2, 0 = nbp (Bluestein pack / post).  `local` (optional)
+// receives the block's index among its image's blocks.
+__device__ __forceinline__ bool long_line_geo(const Ctx &c, int dir, const int *geo, int which, int guard,
+                                              LongLine &ln, int *local_out = nullptr) {
   ln.img = find_img(geo, c.n, blockIdx.x);
   const ImgDesc &d = c.img[ln.img];
   if (guard_off(&c.st[ln.img], guard)) return false;
   ln.L = c.lplans + (dir ? d.collong : d.rowlong);
-  const int nbp = pass == 1 ? ln.L->nb1 : ln.L->nb2;
+  const int nbp = which == 1 ? ln.L->nb1 : (which == 2 ? ln.L->nb2 : ln.L->nbp);
   const int local = blockIdx.x - geo[ln.img];
+  if (local_out) *local_out = local;
   ln.lp = local / nbp;
   ln.blk = local - ln.lp * nbp;
   const int nlines = dir ? d.W : d.H;
@@ -176,6 +180,11 @@ __device__ __forceinline__ bool long_line(const Ctx &c, int dir, int pass, int g
   }
   ln.S = c.S + d.offS + (long long)ln.lp * ln.L->spitch;
   return true;
+}
+
+__device__ __forceinline__ bool long_line(const Ctx &c, int dir, int pass, int guard, LongLine &ln) {
+  const int *geo = dir ? (pass == 1 ? c.geo.colL1 : c.geo.colL2) : (pass == 1 ? c.geo.rowL1 : c.geo.rowL2);
+  return long_line_geo(c, dir, geo, pass, guard, ln);
 }
 
 __device__ __forceinline__ void load_tws(float2 *tws, const float2 *__restrict__ tab, const FftPlan *F) {
@@ -471,6 +480,247 @@ __global__ __launch_bounds__(LONG_THREADS) void k_long_p2(Ctx c, int dir, float 
     }
   }
   if (!dct2 && dot) block_partial<1>(acc, c.partials2);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Long lines without a smooth two-factor split (LongPlan::blue): the length-N FFT of the packed
+// sequence u as Bluestein's chirp convolution, U[m] = w[m] conj(sum_k conj(Y[k] H[k]) e^{-2 pi i k m / M})
+// with w[n] = exp(-i pi n^2 / N), Y = FFT_M(u w zero-padded), H = FFT_M(conj w wrapped) / M, computed
+// four-step with n = N2 n1 + n2 (forward) and k = k1 + N1 k2, m = m2 + N2 m1 (inverse):
+//   k_lb_fft1  per column n2: FFT over n1 of u[n] w[n], times exp(-2 pi i n2 k1 / M) -> Sb[k1 N2 + n2]
+//   k_lb_fft2  per row k1: FFT over n2 -> Y[k1 + N1 k2]; conj(Y H); FFT over k2 -> m2, times
+//              exp(-2 pi i k1 m2 / M) -> Sb[k1 N2 + m2]
+//   k_lb_fft3  per column m2: FFT over k1 -> m1; U[m] = w[m] conj(.) for m < N -> Sa[m]
+// around them k_lb_pack writes u to Sa (the Makhoul packing of both lines, or the DCT-III
+// pre-step) and k_lb_post forms the DCT-II coefficients (from U[k], U[N - k]) or the DCT-III
+// samples (from U at each sample's Makhoul position).  Sa holds N values per line pair, Sb M.
+// ---------------------------------------------------------------------------------------------
+
+constexpr int LB_CHUNK = 4096;  // sequence elements (or coefficient pairs) per pack / post block
+
+template <int MODE>
+__global__ __launch_bounds__(LONG_THREADS) void k_lb_pack(Ctx c, int dir, float *src, const float *__restrict__ p,
+                                                          int guard) {
+  LongLine ln;
+  int local = 0;
+  if (!long_line_geo(c, dir, dir ? c.geo.bcolP : c.geo.browP, 0, guard, ln, &local)) return;
+  const LongPlan &L = *ln.L;
+  const int N = L.N, T = blockDim.x;
+  float2 *__restrict__ Sa = ln.S;
+  constexpr bool dct2 = (MODE == LONG_DCT2 || MODE == LONG_DCT2_UPDATE);
+  constexpr int U = LONG_U;
+  double acc[2] = {0.0, 0.0};
+  if (dct2) {
+    // element m of both lines -> Makhoul slot makhoul_pos(m); the update first
+    const float alpha = (MODE == LONG_DCT2_UPDATE) ? (float)c.st[ln.img].alpha : 0.f;
+    float *x = c.f.u;
+    const float *Ap = c.f.Ap;
+    const int e0 = ln.blk * LB_CHUNK, e1 = min(e0 + LB_CHUNK, N);
+    for (int m0 = e0 + (int)threadIdx.x; m0 < e1; m0 += U * T) {
+      float v[U][2], xv[U][2], pv[U][2], av[U][2];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int m = m0 + u * T;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          v[u][h] = xv[u][h] = pv[u][h] = av[u][h] = 0.f;
+          if (m < e1 && (h == 0 || ln.hasb)) {
+            const long long e = ln.la + (long long)h * ln.pitch + m;
+            v[u][h] = src[e];
+            if (MODE == LONG_DCT2_UPDATE) {
+              xv[u][h] = x[e];
+              pv[u][h] = p[e];
+              av[u][h] = Ap[e];
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int m = m0 + u * T;
+        if (m >= e1) continue;
+        if (MODE == LONG_DCT2_UPDATE) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            if (h == 1 && !ln.hasb) break;
+            const long long e = ln.la + (long long)h * ln.pitch + m;
+            const float xn = xv[u][h] + alpha * pv[u][h];
+            const float rn = v[u][h] - alpha * av[u][h];
+            x[e] = xn;
+            src[e] = rn;
+            acc[0] += (double)rn * rn;
+            acc[1] += isfinite(xn) ? 0.0 : 1.0;
+            v[u][h] = rn;
+          }
+        }
+        Sa[makhoul_pos(m, N)] = make_float2(v[u][0], ln.hasb ? v[u][1] : 0.f);
+      }
+    }
+  } else {
+    // DCT-III pre-step of the pairs (k, N - k), k = 0 .. N / 2; _SCALE first divides by -lambda
+    const float *a = src + ln.ta;
+    const float *b = src + ln.tb;
+    const float2 *mk = c.tab + L.mk_off;
+    const ImgDesc &d = c.img[ln.img];
+    const double *cosH = c.dtab + d.cosH_off;
+    const int ja = 2 * ln.lp;
+    const double cwa = (MODE == LONG_DCT3_SCALE) ? 2.0 * c.dtab[d.cosW_off + ja] : 0.0;
+    const double cwb = (MODE == LONG_DCT3_SCALE && ln.hasb) ? 2.0 * c.dtab[d.cosW_off + ja + 1] : 0.0;
+    const int e0 = ln.blk * LB_CHUNK, e1 = min(e0 + LB_CHUNK, N / 2 + 1);
+    for (int k = e0 + (int)threadIdx.x; k < e1; k += T) {
+      const int nk = k ? N - k : 0;
+      float Xak = a[k], Xbk = ln.hasb ? b[k] : 0.f;
+      float Xan = k ? a[nk] : 0.f, Xbn = (k && ln.hasb) ? b[nk] : 0.f;
+      if (MODE == LONG_DCT3_SCALE) {
+        const double ck = 2.0 * cosH[k], cn = 2.0 * cosH[nk];
+        Xak = (k == 0 && ja == 0) ? 0.f : -(Xak * __builtin_amdgcn_rcpf((float)(ck + cwa - 4.0)));
+        Xbk = ln.hasb ? -(Xbk * __builtin_amdgcn_rcpf((float)(ck + cwb - 4.0))) : 0.f;
+        if (k) {
+          Xan = -(Xan * __builtin_amdgcn_rcpf((float)(cn + cwa - 4.0)));
+          Xbn = ln.hasb ? -(Xbn * __builtin_amdgcn_rcpf((float)(cn + cwb - 4.0))) : 0.f;
+        }
+      }
+      Sa[k] = dct3_pre_w(mk[k], Xak, Xbk, Xan, Xbn);
+      if (nk != k) Sa[nk] = dct3_pre_w(mk[nk], Xan, Xbn, Xak, Xbk);
+    }
+  }
+  if (MODE == LONG_DCT2_UPDATE) block_partial<2>(acc, c.partials2, c.img[ln.img].lbpart + local);
+}
+
+// k_lb_fft1 (STAGE 1) and k_lb_fft3 (STAGE 3): a block transforms B = 2^lb1 columns of the
+// N1 x N2 matrix over its N1 rows; a thread keeps one column slot and walks the rows.
+template <int STAGE, int E>
+__global__ __launch_bounds__(LONG_THREADS) void k_lb_fft13(Ctx c, int dir, int guard) {
+  extern __shared__ float2 lds[];
+  LongLine ln;
+  if (!long_line_geo(c, dir, dir ? c.geo.bcol1 : c.geo.brow1, 1, guard, ln)) return;
+  const LongPlan &L = *ln.L;
+  const int N = L.N, N1 = L.N1, N2 = L.N2, lb = L.lb1, B = 1 << lb, BS = B + 1;
+  const FftPlan *F = &c.plans[L.f1].fft;
+  float2 *tws = lds, *buf = lds + TW_LDS;
+  load_tws(tws, c.tab, F);
+  float2 *__restrict__ Sa = ln.S;
+  float2 *__restrict__ Sb = ln.S + L.sb_off;
+  const float2 *__restrict__ ch = c.tab + L.chirp_off;
+  const int s = threadIdx.x & (B - 1), rstep = blockDim.x >> lb;
+  const int col = ln.blk * B + s;  // n2 (stage 1) or m2 (stage 3)
+  const bool live = col < N2;
+  for (int r = threadIdx.x >> lb; r < N1; r += rstep) {
+    float2 v = make_float2(0.f, 0.f);
+    if (live) {
+      if (STAGE == 1) {
+        const int n = N2 * r + col;
+        if (n < N) v = cmul(Sa[n], ch[n]);
+      } else {
+        v = Sb[(long long)r * N2 + col];
+      }
+    }
+    buf[r * BS + s] = v;
+  }
+  __syncthreads();
+  batch_fft<E>(buf, BS, lb, tws, F);
+  const float2 *tw = c.tab + L.tw_off;
+  for (int r = threadIdx.x >> lb; r < N1; r += rstep) {
+    if (!live) continue;
+    const float2 v = buf[r * BS + s];
+    if (STAGE == 1) {
+      Sb[(long long)r * N2 + col] = (r && col) ? cmul(v, long_twiddle(tw, r * col)) : v;  // r = k1
+    } else {
+      const int m = col + N2 * r;  // r = m1
+      if (m < N) Sa[m] = cmul(ch[m], cconj(v));
+    }
+  }
+}
+
+// k_lb_fft2: a block takes B = 2^lb2 rows k1 of the N1 x N2 matrix: FFT over the row, product with
+// the transformed chirp, conjugate, FFT over the row again, twiddle, in place.
+template <int E>
+__global__ __launch_bounds__(LONG_THREADS) void k_lb_fft2(Ctx c, int dir, int guard) {
+  extern __shared__ float2 lds[];
+  LongLine ln;
+  if (!long_line_geo(c, dir, dir ? c.geo.bcol2 : c.geo.brow2, 2, guard, ln)) return;
+  const LongPlan &L = *ln.L;
+  const int N1 = L.N1, N2 = L.N2, lb = L.lb2, B = 1 << lb, BS = B + 1;
+  const FftPlan *F = &c.plans[L.f2].fft;
+  float2 *tws = lds, *buf = lds + TW_LDS;
+  load_tws(tws, c.tab, F);
+  float2 *__restrict__ Sb = ln.S + L.sb_off;
+  const float2 *__restrict__ bh = c.tab + L.bhat_off;
+  const int T = blockDim.x, nel = N2 << lb;
+  // element idx = q N2 + i: row k1 = blk B + q, column i (contiguous runs along the rows)
+  auto rowcol = [&](int idx, int &q, int &i) {
+    q = (int)((float)idx * L.inv_n2);
+    i = idx - q * N2;
+    if (i < 0) { --q; i += N2; }
+    if (i >= N2) { ++q; i -= N2; }
+  };
+  for (int idx = threadIdx.x; idx < nel; idx += T) {
+    int q, i;
+    rowcol(idx, q, i);
+    const int k1 = ln.blk * B + q;
+    buf[i * BS + q] = k1 < N1 ? Sb[(long long)k1 * N2 + i] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  batch_fft<E>(buf, BS, lb, tws, F);
+  for (int idx = threadIdx.x; idx < nel; idx += T) {
+    const int q = idx & (B - 1), k2 = idx >> lb;
+    const int k1 = ln.blk * B + q;
+    if (k1 < N1) buf[k2 * BS + q] = cconj(cmul(buf[k2 * BS + q], bh[k1 + N1 * k2]));
+  }
+  __syncthreads();
+  batch_fft<E>(buf, BS, lb, tws, F);
+  const float2 *tw = c.tab + L.tw_off;
+  for (int idx = threadIdx.x; idx < nel; idx += T) {
+    int q, i;
+    rowcol(idx, q, i);  // i = m2
+    const int k1 = ln.blk * B + q;
+    if (k1 >= N1) continue;
+    const float2 v = buf[i * BS + q];
+    Sb[(long long)k1 * N2 + i] = (k1 && i) ? cmul(v, long_twiddle(tw, k1 * i)) : v;
+  }
+}
+
+// Post pass.  LONG_DCT2: coefficients at k and N - k from U[k], U[N - k] -> dst lines (T / Tt
+// order).  LONG_DCT3: sample m of both lines from U at its Makhoul position -> dst lines (row
+// order), with the partial sums of r.dst when `dot`.
+template <int MODE>
+__global__ __launch_bounds__(LONG_THREADS) void k_lb_post(Ctx c, int dir, float *__restrict__ dst, int dot,
+                                                          int guard) {
+  LongLine ln;
+  int local = 0;
+  if (!long_line_geo(c, dir, dir ? c.geo.bcolP : c.geo.browP, 0, guard, ln, &local)) return;
+  const LongPlan &L = *ln.L;
+  const int N = L.N, T = blockDim.x;
+  const float2 *__restrict__ Sa = ln.S;
+  double acc[1] = {0.0};
+  if (MODE == LONG_DCT2) {
+    float *da = dst + ln.ta, *db = dst + ln.tb;
+    const float2 *mk = c.tab + L.mk_off;
+    const int e0 = ln.blk * LB_CHUNK, e1 = min(e0 + LB_CHUNK, N / 2 + 1);
+    for (int k = e0 + (int)threadIdx.x; k < e1; k += T) {
+      const int nk = k ? N - k : 0;
+      float Xak, Xbk, Xan, Xbn;
+      dct2_coef_w(Sa[k], Sa[nk], mk[k], mk[nk], Xak, Xbk, Xan, Xbn);
+      da[k] = Xak;
+      if (ln.hasb) db[k] = Xbk;
+      if (nk != k) {
+        da[nk] = Xan;
+        if (ln.hasb) db[nk] = Xbn;
+      }
+    }
+  } else {
+    float *da = dst + ln.la, *db = dst + ln.la + ln.pitch;
+    const float *r = c.f.r + ln.la;
+    const int e0 = ln.blk * LB_CHUNK, e1 = min(e0 + LB_CHUNK, N);
+    for (int m = e0 + (int)threadIdx.x; m < e1; m += T) {
+      const float2 w = Sa[makhoul_pos(m, N)];
+      const float va = w.x * L.invN, vb = -w.y * L.invN;
+      da[m] = va;
+      if (ln.hasb) db[m] = vb;
+      if (dot) acc[0] += (double)r[m] * va + (ln.hasb ? (double)r[ln.pitch + m] * vb : 0.0);
+    }
+  }
+  if (MODE == LONG_DCT3 && dot) block_partial<1>(acc, c.partials2, c.img[ln.img].lbpart + local);
 }
 
 }  // namespace irls2d

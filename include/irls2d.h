@@ -41,7 +41,7 @@ extern "C" {
 
 #define IRLS2D_OK 0
 #define IRLS2D_EINVAL 1       /* bad argument (shape, pointer, iteration count) */
-#define IRLS2D_EUNSUPPORTED 2 /* grid dimension beyond what the LDS transforms handle */
+#define IRLS2D_EUNSUPPORTED 2 /* grid dimension beyond what the transforms handle (a side above ~2^19) */
 #define IRLS2D_EWORKSPACE 3   /* workspace too small */
 #define IRLS2D_EHIP 4         /* HIP runtime error */
 
@@ -63,8 +63,9 @@ int irls2d_version(void);
 const char *irls2d_last_error(void);
 
 /* 1 if a grid side of length N is supported: N <= 12288 (LDS-resident DCT, Bluestein for prime
-   factors above 5), or a larger N = 2^a 3^b 5^c that splits into two factors <= 1024 (two-pass
-   four-step DCT). */
+   factors above 5); a larger N = 2^a 3^b 5^c that splits into two factors <= 1024 (two-pass
+   four-step DCT); any other larger N whose Bluestein length (5-smooth, >= 2N - 1, within 30%)
+   splits into two factors <= 1024, i.e. N up to about 2^19 (four-step Bluestein passes). */
 int irls2d_supported_length(int64_t n);
 
 /*

@@ -1,6 +1,7 @@
 """GPU parity of the four-step (two-pass) DCT used for sides longer than the LDS-resident limit
-(12288): rows, columns, odd factors, odd line counts (a last line without a partner), mixed
-batches of long and short images, graph replay.
+(12288), and of its Bluestein form for long sides without a smooth split (primes): rows, columns,
+odd factors, odd line counts (a last line without a partner), mixed batches of long and short
+images, graph replay.
 
 Bars: cycle counts agree with the float32 reference-CPU path on >= 99.99% of valid pixels, and
 RMS(phi_gpu - phi_fp64) is within the float32 envelope: <= max(1e-3, 2 x the largest
@@ -50,6 +51,8 @@ LONG_CASES = [
     (9, 15625, 31, False, 0.0),    # rows 125 x 125 (both factors odd), odd row count
     (12800, 20, 32, False, 0.0),   # columns 128 x 100
     (13122, 15, 33, True, 0.0),    # columns 162 x 81 (odd factor), odd column count
+    (9, 20011, 34, True, 0.05),    # prime rows: Bluestein, M = 40960 = 256 x 160
+    (13001, 12, 35, False, 0.0),   # prime columns: Bluestein, M = 32768 = 256 x 128
 ]
 
 
@@ -89,11 +92,25 @@ def test_long_mixed_batch_equals_plain(hip):
         assert np.array_equal(cb, cp)
 
 
+def test_long_bluestein_mixed_batch_equals_plain(hip):
+    """Prime long rows and columns (Bluestein four-step passes) in one batch with a smooth long
+    side and a short image: each equals its own plain call bitwise."""
+    from core_amd import irls_unwrap_2d, irls_unwrap_2d_batch
+    cases = [_case(10, 12289, 43), _case(13001, 7, 44, True, 0.0), _case(12, 12500, 45), _case(40, 61, 46)]
+    print("bluestein mixed batch 10x12289, 13001x7, 12x12500, 40x61", flush=True)
+    batch = irls_unwrap_2d_batch([c[0] for c in cases], [c[1] for c in cases], device='cuda', **ITERS)
+    for (psi, coh), (ub, cb) in zip(cases, batch):
+        up, cp = irls_unwrap_2d(psi, coh, device='cuda', **ITERS)
+        assert np.array_equal(np.isnan(ub), np.isnan(psi))
+        assert np.array_equal(ub, up, equal_nan=True)
+        assert np.array_equal(cb, cp)
+
+
 def test_long_graph_equals_eager(hip):
     import torch
     from core_amd import solve_device
     from core_amd.unwrap2d import Workspace, image_view, pack
-    shapes = [(16, 13500), (12960, 12)]
+    shapes = [(16, 13500), (12960, 12), (8, 12289)]
     print(f"graph replay {shapes}", flush=True)
     ims = [_case(h, w, 50 + i) for i, (h, w) in enumerate(shapes)]
     ws = Workspace.get(shapes, hip)
@@ -117,11 +134,13 @@ def test_long_graph_equals_eager(hip):
     ws.release()
 
 
-PRECOND_SHAPES = [(64, 48), (127, 211), (1, 1000), (16, 12500), (12800, 8), (9, 15625), (12, 40000), (40000, 6)]
+PRECOND_SHAPES = [(64, 48), (127, 211), (1, 1000), (16, 12500), (12800, 8), (9, 15625), (12, 40000), (40000, 6),
+                  (12, 20011), (13001, 9), (16, 24989), (3, 12289), (12289, 1), (20011, 13001)]
 
 
 def test_preconditioner_matches_scipy_fp64(hip):
-    """The DCT engine alone (LDS-resident, Bluestein and four-step lengths) against scipy in FP64:
+    """The DCT engine alone (LDS-resident, Bluestein and four-step lengths, and four-step
+    Bluestein for long prime sides, 20011 x 13001 both ways) against scipy in FP64:
     z = idctn(-dctn(r) / lambda), DC = 0, lambda = 2cos(pi i/H) + 2cos(pi j/W) - 4."""
     import scipy.fft as sf
     import torch
@@ -139,9 +158,9 @@ def test_preconditioner_matches_scipy_fp64(hip):
     for k, ((h, w), r) in enumerate(zip(PRECOND_SHAPES, rs)):
         lam = (2 * np.cos(np.pi * np.arange(h) / h)[:, None] + 2 * np.cos(np.pi * np.arange(w) / w)[None, :] - 4)
         lam[0, 0] = 1.0
-        X = sf.dctn(r.astype(np.float64), type=2) / -lam
+        X = sf.dctn(r.astype(np.float64), type=2, workers=-1) / -lam
         X[0, 0] = 0.0
-        z64 = sf.idctn(X, type=2)
+        z64 = sf.idctn(X, type=2, workers=-1)
         zg = image_view(zh, ws, k).astype(np.float64)
         # max error relative to max |z|; scipy's own float32 dctn/idctn reaches 2e-7 .. 6e-7 here
         err = np.abs(zg - z64).max() / np.abs(z64).max()

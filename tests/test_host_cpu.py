@@ -76,10 +76,12 @@ def test_workspace_size_grows_with_pixels(lib):
 def test_supported_lengths(lib):
     assert lib.supported_length(8192) and lib.supported_length(12288)
     assert lib.supported_length(4999)            # Bluestein
-    assert not lib.supported_length(12289)       # prime beyond the LDS-resident lengths
+    assert lib.supported_length(12289)           # beyond the LDS-resident lengths: four-step Bluestein
+    assert lib.supported_length(20011)           # prime
     assert lib.supported_length(40000)           # four-step 200 x 200
+    assert not lib.supported_length(600011)      # the Bluestein length would exceed 1024 x 1024
     assert lib.supported_length(12500) and lib.supported_length(15625) and lib.supported_length(13122)
-    assert not lib.supported_length(2 * 12289)   # long and not 5-smooth
+    assert lib.supported_length(2 * 12289)       # long and not 5-smooth: Bluestein
     assert not lib.supported_length(2 ** 21)     # no split into two factors <= 1024
     assert lib.supported_length(2 ** 20)
     assert not lib.supported_length(0)
@@ -105,8 +107,10 @@ def test_argument_errors_without_gpu(lib):
     nb, Hb, Wb = lib.shapes_arrays([(0, 16)])
     b = ctypes.c_size_t(0)
     assert L.irls2d_workspace_size(nb, Hb, Wb, ctypes.byref(b)) == lib.EINVAL
-    nu, Hu, Wu = lib.shapes_arrays([(16, 20011)])
+    nu, Hu, Wu = lib.shapes_arrays([(16, 600011)])   # beyond the four-step Bluestein lengths
     assert L.irls2d_workspace_size(nu, Hu, Wu, ctypes.byref(b)) == lib.EUNSUPPORTED
+    nb2, Hb2, Wb2 = lib.shapes_arrays([(16, 20011)])  # a long prime side is supported
+    assert L.irls2d_workspace_size(nb2, Hb2, Wb2, ctypes.byref(b)) == lib.OK
     assert L.irls2d_arena_layout(0, H, W, None, None) == -1
 
 
