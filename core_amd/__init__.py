@@ -10,11 +10,13 @@ Drop-in for insardev's 2-D unwrap hot path (insardev/insardev/utils_unwrap2d.py:
 reference's Stack_unwrap2d tasks (which look the function up on that module at call time,
 Stack_unwrap2d.py:12, :65-69, :124-128) run on this solver unchanged.  `core_amd.stack` has the
 stack-level entry points (unwrap2d_irls / unwrap2d / unwrap2d_dataset) that send all pairs of a
-burst through one batched device call; `core_amd.autograd.irls_unwrap_2d_diff` is the
+burst through one batched device call (with `link_components`, the default component linking);
+`core_amd.autograd.irls_unwrap_2d_diff` is the
 differentiable form and `core_amd.autotune` the transform-plan tuner.
 """
 from __future__ import annotations
 
+from .linking import link_components  # noqa: F401
 from .unwrap2d import (irls_unwrap_2d, irls_unwrap_2d_batch, precondition_device, resolve_device,  # noqa: F401
                        solve_device)
 

@@ -19,17 +19,16 @@ goes through ONE batched device solve (`irls_unwrap_2d_batch`), with the same pe
   in one batched device solve.
 * `filter_small_components`: the first step of the reference's component linking
   (Stack_unwrap2d.py:343-375), which NaNs out labelled components below max(conncomp_size, 4).
-  The integer-program linking of the remaining components (:377-397, utils_unwrap2d.py:195-444)
-  is not part of this build: `unwrap2d(conncomp=False)` warns and returns them unlinked when two
-  or more remain.
+  `unwrap2d(conncomp=False)` then links the remaining components by integer 2 pi offsets
+  (core_amd.linking, Stack_unwrap2d.py:377-397, utils_unwrap2d.py:195-444, :886-1040).
 """
 from __future__ import annotations
 
 import importlib
-import warnings
 
 import numpy as np
 
+from .linking import link_components
 from .unwrap2d import irls_unwrap_2d_batch
 
 __all__ = ["unwrap_stack", "filter_small_components", "unwrap2d_irls", "unwrap2d", "unwrap2d_dataset",
@@ -160,10 +159,10 @@ def unwrap2d(phase, weight=None, conncomp=False, conncomp_size=1_000, conncomp_g
              conncomp_linkcount=30, device='auto', debug=False, xr=None, **kwargs):
     """Stack_unwrap2d.unwrap2d (:645-759) over a burst mapping {key: xr.Dataset}.
 
-    conncomp=True returns (unwrapped, conncomp).  conncomp=False applies the small-component
-    filter of the reference's linking step and returns the unwrapped mapping; linking two or more
-    remaining components (the reference's integer program) is not part of this build, so they are
-    returned unlinked with a RuntimeWarning.
+    conncomp=True returns (unwrapped, conncomp).  conncomp=False (the default) links each grid's
+    components as the reference's `_link_components` does (Stack_unwrap2d.py:293-454): components
+    below max(conncomp_size, 4) pixels become NaN, the others are shifted by the integer 2 pi
+    offsets of core_amd.linking.link_components, and the unwrapped mapping is returned.
     """
     if not conncomp and conncomp_linksize > conncomp_size:
         raise ValueError(
@@ -175,7 +174,6 @@ def unwrap2d(phase, weight=None, conncomp=False, conncomp_size=1_000, conncomp_g
     if conncomp:
         return unwrapped, labels
     result = {}
-    unlinked = 0
     for key, ds in unwrapped.items():
         lds = labels[key]
         vars_ = {}
@@ -184,19 +182,16 @@ def unwrap2d(phase, weight=None, conncomp=False, conncomp_size=1_000, conncomp_g
             u = np.asarray(uda.values)
             cc = np.asarray(lds[var].values)
             u3, c3 = (u[None], cc[None]) if u.ndim == 2 else (u, cc)
-            out = np.empty_like(u3)
+            out = np.empty_like(u3, dtype=np.float32)
             for p in range(u3.shape[0]):
-                out[p], kept = filter_small_components(u3[p], c3[p], conncomp_size)
-                unlinked += kept >= 2
+                out[p] = link_components(u3[p], c3[p], conncomp_size=conncomp_size, conncomp_gap=conncomp_gap,
+                                         conncomp_linksize=conncomp_linksize, conncomp_linkcount=conncomp_linkcount,
+                                         debug=debug)
             out = out[0] if u.ndim == 2 else out
             nda = xr.DataArray(out, dims=uda.dims, coords=uda.coords)
             nda.attrs.update(dict(uda.attrs))
             vars_[var] = nda
         result[key] = xr.Dataset(vars_, attrs=dict(ds.attrs))
-    if unlinked:
-        warnings.warn(f"unwrap2d: {unlinked} grid(s) keep two or more components; linking them by 2 pi offsets "
-                      "(the reference's integer program) is not part of this build, they are returned unlinked "
-                      "(use conncomp=True to get their labels)", RuntimeWarning, stacklevel=2)
     return result
 
 

@@ -2,7 +2,6 @@
 brute-force restatement of Stack_unwrap2d.py:343-375, and the structure of the Dataset adapters
 with a minimal xarray stand-in (xarray is not installed here) and the device solve replaced by a
 recording stub."""
-import warnings
 
 import numpy as np
 import pytest
@@ -151,11 +150,14 @@ def test_unwrap2d_errors_and_modes(stub_solver):
         stack.unwrap2d({"b": ph}, conncomp_size=4, conncomp_linksize=5, xr=FakeXr)
     unw, cc = stack.unwrap2d({"b": ph}, {"b": co}, conncomp=True, conncomp_size=4, xr=FakeXr)
     assert stub_solver[-1][2]["conncomp_size"] == 4
-    with warnings.catch_warnings(record=True) as w:
-        warnings.simplefilter("always")
-        out = stack.unwrap2d({"b": ph}, conncomp_size=10, conncomp_linksize=5, xr=FakeXr)
-    # label 2 covers 12 px >= 10: both components survive, linking is not built -> warning
-    assert any(issubclass(x.category, RuntimeWarning) for x in w)
+    out = stack.unwrap2d({"b": ph}, conncomp_size=10, conncomp_linksize=5, xr=FakeXr)
+    # label 2 covers 12 px >= 10: both components survive and each pair is linked
+    from core_amd.linking import link_components
+    u_stub = ph["VV"].values.astype(np.float32) + 1.0
+    cc_stub = np.ones(u_stub.shape, np.uint16)
+    cc_stub[..., :, -2:] = 2
+    for p in range(u_stub.shape[0]):
+        assert np.array_equal(out["b"]["VV"].values[p], link_components(u_stub[p], cc_stub[p], 10, None, 5, 30))
     assert not np.isnan(out["b"]["VV"].values).any()
     out = stack.unwrap2d({"b": ph}, conncomp_size=13, conncomp_linksize=5, xr=FakeXr)
     assert np.isnan(out["b"]["VV"].values[..., -2:]).all()      # the 12 px component is dropped
