@@ -249,13 +249,31 @@ def adjoint_batch(args, dev):
 
     ms_a, _ = time_steps(adj, 1, 1)
     ok = bool(torch.isfinite(gph).all() and torch.isfinite(gw).all())
+    # accuracy of the 50-iteration adjoint: dL/dphase = valid * C^T g holds exactly at the solution
+    # of H lam = C^T g (its error is the adjoint CG residual); dL/dweight against a 1000-iteration
+    # adjoint of the same fixed point
+    from core_amd.unwrap2d import image_view
+    ident = 0.0
+    for k in range(len(shapes)):
+        v = ~torch.isnan(image_view(ph, ws, k))
+        gk = torch.where(v, image_view(g, ws, k), torch.zeros_like(image_view(g, ws, k)))
+        ct = torch.where(v, gk - gk.sum() / v.sum(), torch.zeros_like(gk))
+        ident = max(ident, float((image_view(gph, ws, k) - ct).abs().max() / ct.abs().max()))
+    gw50 = gw.clone()
+    _lib.check(L.irls2d_adjoint(n, H, W, ph.data_ptr(), co.data_ptr(), out.data_ptr(), g.data_ptr(),
+                                gph.data_ptr(), gw.data_ptr(), 1e-2, 1000, 0.0, ws.buf.data_ptr(), ws.nbytes,
+                                ws.info.data_ptr(), stream, 0), "irls2d_adjoint")
+    wrel = max(float((image_view(gw50, ws, k) - image_view(gw, ws, k)).abs().max()
+                     / image_view(gw, ws, k).abs().max().clamp_min(1e-30)) for k in range(len(shapes)))
     ws.release()
-    del ws, ph, co, out, g, gph, gw
+    del ws, ph, co, out, g, gph, gw, gw50
     torch.cuda.empty_cache()
     return {
         "workload": "16 x 2048x2048, 1024-case generator seeds 3+i; forward 10 IRLS x 50 PCG; adjoint 50 PCG",
         "pixels": npx, "forward_ms": round(ms_f, 2), "forward_mpx_s": round(npx / ms_f / 1e3, 2),
         "adjoint_ms": round(ms_a, 2), "adjoint_mpx_s": round(npx / ms_a / 1e3, 2), "grads_finite": ok,
+        "phase_grad_identity_rel_err": float(f"{ident:.3g}"),
+        "weight_grad_rel_err_vs_1000_pcg": float(f"{wrel:.3g}"),
     }
 
 

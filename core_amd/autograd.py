@@ -29,6 +29,27 @@ def _from_arena(a, ws, B):
     return torch.stack([image_view(a, ws, k) for k in range(B)])
 
 
+def adjoint_device(phase, weight, u, grad, epsilon=1e-2, cg_iters=200, cg_tol=0.0):
+    """The implicit adjoint at a given fixed point: [B, H, W] HIP tensors phase, weight (or None),
+    u (the solver output u*, uncentred) and grad = dL/d(C u) in; (dL/dphase, dL/dweight or None)
+    out (include/irls2d.h: irls2d_adjoint)."""
+    B, H, W = phase.shape
+    ws = Workspace.get([(H, W)] * B, phase.device)
+    ph = _to_arena(phase.float(), ws)
+    wt = _to_arena(weight.float(), ws) if weight is not None else None
+    ua = _to_arena(u.float(), ws)
+    ga = _to_arena(grad.contiguous().float(), ws)
+    gph = torch.empty(ws.total, dtype=torch.float32, device=phase.device)
+    gw = torch.empty(ws.total, dtype=torch.float32, device=phase.device) if weight is not None else None
+    n, Hs, Ws = _lib.shapes_arrays(ws.shapes)
+    rc = _lib.load().irls2d_adjoint(
+        n, Hs, Ws, ph.data_ptr(), wt.data_ptr() if wt is not None else None, ua.data_ptr(), ga.data_ptr(),
+        gph.data_ptr(), gw.data_ptr() if gw is not None else None, float(epsilon), int(cg_iters), float(cg_tol),
+        ws.buf.data_ptr(), ws.nbytes, ws.info.data_ptr(), torch.cuda.current_stream(phase.device).cuda_stream, 0)
+    _lib.check(rc, "irls2d_adjoint")
+    return _from_arena(gph, ws, B), (_from_arena(gw, ws, B) if gw is not None else None)
+
+
 class IrlsUnwrap2d(torch.autograd.Function):
     @staticmethod
     def forward(ctx, phase, weight, max_iter=10, cg_max_iter=50, epsilon=1e-2, adj_cg_iter=200, adj_cg_tol=0.0,
@@ -55,22 +76,8 @@ class IrlsUnwrap2d(torch.autograd.Function):
     def backward(ctx, grad_out):
         phase, weight, u = ctx.saved_tensors
         epsilon, iters, tol = ctx.params
-        B, H, W = phase.shape
-        ws = Workspace.get([(H, W)] * B, phase.device)
-        ph = _to_arena(phase.float(), ws)
-        wt = _to_arena(weight.float(), ws) if weight is not None else None
-        ua = _to_arena(u, ws)
-        ga = _to_arena(grad_out.contiguous().float(), ws)
-        gph = torch.empty(ws.total, dtype=torch.float32, device=phase.device)
-        gw = torch.empty(ws.total, dtype=torch.float32, device=phase.device) if ctx.has_weight else None
-        n, Hs, Ws = _lib.shapes_arrays(ws.shapes)
-        rc = _lib.load().irls2d_adjoint(
-            n, Hs, Ws, ph.data_ptr(), wt.data_ptr() if wt is not None else None, ua.data_ptr(), ga.data_ptr(),
-            gph.data_ptr(), gw.data_ptr() if gw is not None else None, float(epsilon), int(iters), float(tol),
-            ws.buf.data_ptr(), ws.nbytes, ws.info.data_ptr(), torch.cuda.current_stream(phase.device).cuda_stream, 0)
-        _lib.check(rc, "irls2d_adjoint")
-        g_phase = _from_arena(gph, ws, B)
-        g_weight = _from_arena(gw, ws, B) if gw is not None else None
+        g_phase, g_weight = adjoint_device(phase, weight if ctx.has_weight else None, u, grad_out, epsilon, iters,
+                                           tol)
         return g_phase, g_weight, None, None, None, None, None, None
 
 

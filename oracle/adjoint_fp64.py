@@ -168,3 +168,62 @@ def fixed_point(phase, weight, u0, epsilon=1e-2, tol=1e-12, max_newton=200):
             t *= 0.5
         u = u - t * d
     return u
+
+
+def chord_solver(phase, weight, ustar, epsilon=1e-2):
+    """Fixed points of perturbed inputs near u*: returns F(phase', weight') -> u solving
+    G(u; phase', weight') = 0 by chord iterations u <- u - H0^-1 G(u) from u*, with H0 = D^T W' D at
+    (phase, weight, u*) factorised once (grounded at pixel 0).  For perturbations of 1e-6 the
+    contraction factor is O(1e-6): the iteration reaches the double-precision floor of G in a few
+    steps, as the damped-Newton `fixed_point` does, at the cost of triangular solves only."""
+    nan = np.isnan(phase)
+    valid = ~nan
+    Hh, Ww = phase.shape
+    Dx, Dy = diff_ops(Hh, Ww)
+    keep = np.arange(1, Hh * Ww)
+    ex = np.zeros((Hh, Ww)); ex[:, :-1] = 1.0
+    ey = np.zeros((Hh, Ww)); ey[:-1, :] = 1.0
+    eps2 = epsilon * epsilon
+
+    def setup(ph, wt):
+        phi = np.where(nan, 0.0, ph).astype(np.float64)
+        if wt is not None:
+            w = np.clip(np.where(np.isnan(wt) | nan, 0.0, wt).astype(np.float64), 0.01, 1.0)
+        else:
+            w = (~nan).astype(np.float64)
+        tx, ty = O.targets(phi)
+        wx, wy = O.edge_weights(w, valid)
+        return tx, ty, wx, wy
+
+    def grad(u, tx, ty, wx, wy):
+        rx = np.zeros_like(u); ry = np.zeros_like(u)
+        rx[:, :-1] = u[:, 1:] - u[:, :-1]
+        ry[:-1, :] = u[1:, :] - u[:-1, :]
+        rx -= tx
+        ry -= ty
+        gx = np.where(wx > 0, wx * rx / np.sqrt(rx * rx + eps2), 1e-6 * rx) * ex
+        gy = np.where(wy > 0, wy * ry / np.sqrt(ry * ry + eps2), 1e-6 * ry) * ey
+        return Dx.T @ gx.ravel() + Dy.T @ gy.ravel()
+
+    tx, ty, wx, wy = setup(phase, weight)
+    wpx, _, wpy, _ = edge_terms(ustar, tx, ty, wx, wy, epsilon)
+    A = (Dx.T @ sp.diags(wpx.ravel()) @ Dx + Dy.T @ sp.diags(wpy.ravel()) @ Dy).tocsc()
+    lu = spla.splu(A[keep][:, keep].tocsc())
+
+    def F(ph, wt, max_iter=30):
+        # iterate to the rounding floor of G: finite differences with steps of 1e-6 over many
+        # pixels need the fixed point to ~1e-15, not to a looser tolerance
+        t = setup(ph, wt)
+        u = ustar.astype(np.float64).copy()
+        best = np.inf
+        for _ in range(max_iter):
+            G = grad(u, *t)
+            g = float(np.abs(G[keep]).max())
+            if g >= best or g == 0.0:
+                break
+            best = g
+            d = np.zeros(Hh * Ww)
+            d[keep] = lu.solve(G[keep])
+            u = u - d.reshape(Hh, Ww)
+        return u
+    return F

@@ -70,3 +70,25 @@ def test_phase_gradient_is_centred_incoming_gradient():
     d[v] = 1e-3 * np.random.default_rng(4).normal(size=v.sum())
     u2 = A.fixed_point(psi + d, coh, ustar)
     np.testing.assert_allclose(A.center(u2, v), A.center(ustar + d, v), atol=1e-8)
+
+
+@pytest.mark.parametrize("noise", [0.3, 1.0])
+def test_chord_solver_equals_newton_fixed_point(noise):
+    """The chord iterations used for the 256 x 256 finite differences land on the same perturbed
+    fixed point as the damped-Newton solve (an independent method) to the rounding floor."""
+    wl = S.Workload(32, 36, 5, ramp_max=10.0, n_bumps=2, noise=noise)
+    psi, coh, _ = S.generate(wl)
+    psi = psi.astype(np.float64)
+    coh = coh.astype(np.float64)
+    _, _, nm, pf, wf = post.prologue(psi.astype(np.float32), coh.astype(np.float32))
+    u0 = O.solve(pf, wf, ~nm, max_iter=10, tol=0, cg_max_iter=50, cg_tol=0)
+    ustar = A.fixed_point(psi, coh, u0)
+    F = A.chord_solver(psi, coh, ustar)
+    rng = np.random.default_rng(7)
+    d = rng.normal(size=psi.shape)
+    for h in (1e-6, 1e-4):
+        for ps, wt in ((psi + h * d, coh), (psi, coh + h * d)):
+            a = A.center(F(ps, wt), ~nm)
+            b = A.center(A.fixed_point(ps, wt, ustar), ~nm)
+            assert np.abs(a - b).max() < 1e-11
+    assert np.abs(A.center(F(psi, coh), ~nm) - A.center(ustar, ~nm)).max() < 1e-12

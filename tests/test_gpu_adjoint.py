@@ -87,3 +87,77 @@ def test_gpu_adjoint_matches_unrolled_autograd(hip, noise, mask):
     print(f"  dL/dphase rel err {ephi:.2e}; dL/dweight err {ew:.2e} (of the largest gradient entry)", flush=True)
     assert ephi < 1e-4
     assert ew < 2e-3
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("noise", [0.3, 1.0])
+def test_adjoint_256_against_fp64_finite_differences(hip, noise):
+    """The differentiable-unwrap check at 256 x 256 (the 1024-case generator scaled: ramp 0-10 rad,
+    6 bumps, coherence U[0.3, 1]; noise 0.3 as specified, and 1.0, which adds residues so that the
+    weights matter): 64 random unit probe directions, FP64 central differences with step 1e-6 of
+    the IRLS fixed-point map (oracle/adjoint_fp64.py: damped Newton to the fixed point, chord
+    iterations for the perturbed inputs), loss sum((C u - target)^2) over valid pixels with the
+    noiseless phase as the (centred) target.
+
+    Each directional derivative is compared with its finite difference relative to
+    max(|difference|, RMS directional derivative) -- the RMS over random unit directions is
+    |gradient|_2 / sqrt(valid pixels); the differences carry an absolute rounding floor near 5e-7.
+    FP64 oracle adjoint: within 1e-5.  GPU adjoint (float32, evaluated at the same fixed point):
+    within 1e-4 for dL/dphase and 1e-3 for dL/dweight.  Without residues the fixed point has r = 0
+    on every edge and dL/dweight vanishes; then both adjoints and the differences must be 0 to
+    1e-5 of the phase scale."""
+    import torch
+    from core_amd.autograd import adjoint_device
+    from core_amd.unwrap2d import Workspace, image_view, pack, solve_device
+    from oracle import adjoint_fp64 as A
+    wl0 = S.scaled(S.single_1024(), 256, 256)
+    wl = S.Workload(256, 256, wl0.seed, wl0.ramp_max, wl0.n_bumps, coh_lo=wl0.coh_lo, coh_hi=wl0.coh_hi, noise=noise)
+    psi, coh, true = S.generate(wl)
+    print(f"adjoint FD 256x256 noise={noise}", flush=True)
+    v = ~np.isnan(psi)
+    ws = Workspace.get([psi.shape], hip)
+    u0, _ = solve_device(pack([psi], ws, hip), pack([coh], ws, hip), [psi.shape], max_iter=10, tol=0.0,
+                         cg_max_iter=50, cg_tol=0.0, workspace=ws)
+    u0 = image_view(u0, ws, 0).cpu().numpy().astype(np.float64)
+    p64, c64 = psi.astype(np.float64), coh.astype(np.float64)
+    ustar = A.fixed_point(p64, c64, u0)
+    F0 = A.chord_solver(p64, c64, ustar)
+    F = lambda ps, wt: A.center(F0(ps, wt), v)
+    target = A.center(true.astype(np.float64), v)
+    u = A.center(ustar, v)
+    g = 2 * (u - target) * v
+    rphi, rw, _ = A.adjoint(p64, c64, ustar, g)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32))[None].to(hip)
+    gph, gw = adjoint_device(t(psi), t(coh), t(ustar), t(g), cg_iters=3000)
+    gph = gph[0].cpu().numpy().astype(np.float64)
+    gw = gw[0].cpu().numpy().astype(np.float64)
+    nv = np.sqrt(v.sum())
+    rms_phi, rms_w = np.linalg.norm(rphi) / nv, np.linalg.norm(rw) / nv
+    weights_matter = rms_w > 1e-3 * rms_phi
+    print(f"  RMS directional derivative: phase {rms_phi:.2e} weight {rms_w:.2e}; GPU-vs-oracle max abs "
+          f"phase {np.abs(gph - rphi).max():.1e} weight {np.abs(gw - rw).max():.1e}", flush=True)
+    dL = lambda up, um: float(np.sum((up - um) * (up + um - 2 * target) * v))
+    rng = np.random.default_rng(64)
+    h = 1e-6
+    worst = np.zeros(4)
+    for probe in range(64):
+        if probe % 16 == 0:
+            print(f"  probe {probe}", flush=True)
+        d = rng.normal(size=psi.shape)
+        d[~v] = 0
+        d /= np.linalg.norm(d)
+        fd = dL(F(p64 + h * d, c64), F(p64 - h * d, c64)) / (2 * h)
+        dw = rng.normal(size=psi.shape)
+        dw[~v] = 0
+        dw /= np.linalg.norm(dw)
+        fdw = dL(F(p64, c64 + h * dw), F(p64, c64 - h * dw)) / (2 * h)
+        den = max(abs(fd), rms_phi)
+        denw = max(abs(fdw), rms_w) if weights_matter else rms_phi
+        worst = np.maximum(worst, [abs(fd - np.sum(rphi * d)) / den, abs(fdw - np.sum(rw * dw)) / denw,
+                                   abs(fd - np.sum(gph * d)) / den, abs(fdw - np.sum(gw * dw)) / denw])
+    print(f"  64 probes, worst relative error vs FD: oracle phase {worst[0]:.1e} weight {worst[1]:.1e}; "
+          f"GPU phase {worst[2]:.1e} weight {worst[3]:.1e} (weights matter: {weights_matter})", flush=True)
+    assert worst[0] < 1e-5 and worst[1] < 1e-5
+    assert worst[2] < 1e-4 and worst[3] < (1e-3 if weights_matter else 1e-5)
+    if noise >= 1.0:
+        assert weights_matter
