@@ -168,18 +168,22 @@ def batched_stack(args, dev, kernels=False):
     ims = [S.generate(wl)[:2] for wl in wls]
     shapes = [(wl.H, wl.W) for wl in wls]
     npx = sum(h * w for h, w in shapes)
-    kw = dict(max_iter=N_IRLS, tol=0.0, cg_max_iter=N_CG, cg_tol=0.0)
+    n_irls = getattr(args, "stack_irls", None) or N_IRLS
+    kw = dict(max_iter=n_irls, tol=0.0, cg_max_iter=N_CG, cg_tol=0.0)
     ws = Workspace(shapes, dev)
     ph = upload(ws, [im[0] for im in ims], dev)
     co = upload(ws, [im[1] for im in ims], dev)
     out = torch.empty(ws.total, dtype=torch.float32, device=dev)
+    print(f"batched stack: {npx} px generated and uploaded", file=sys.stderr, flush=True)
     ms_b, _ = time_steps(lambda: solve_device(ph, co, shapes, graph=True, out=out, workspace=ws, **kw),
                          args.batched_steps, 1)
+    print(f"batched stack: graph steps done ({ms_b:.1f} ms)", file=sys.stderr, flush=True)
     kstats = None
     if kernels:
         # per-kernel durations of one event-timed eager batched solve
         _lib.kernel_stats_reset()
         solve_device(ph, co, shapes, graph=False, time_kernels=True, out=out, workspace=ws, **kw)
+        print("batched stack: event-timed eager solve done", file=sys.stderr, flush=True)
         kstats = {}
         for k, (tot, n) in sorted(_lib.kernel_stats().items(), key=lambda kv: -kv[1][0]):
             e = {"total_ms": round(tot, 1), "launches": n, "mean_us": round(tot / n * 1e3, 1)}
@@ -189,6 +193,10 @@ def batched_stack(args, dev, kernels=False):
     ws.release()
     del ws, ph, co, out
     torch.cuda.empty_cache()
+    if getattr(args, "no_sequential", False):
+        return {"workload": f"48 SBAS bursts H~U{{1000..1500}} W~U{{4000..6000}}, seeds 1000+i, {n_irls} IRLS x 50 PCG",
+                "pixels": npx, "batched_ms": round(ms_b, 2), "batched_mpx_s": round(npx / ms_b / 1e3, 2),
+                **({"kernels": kstats} if kstats is not None else {})}
     # 48 sequential plain calls (one image per call, eager launches), inputs already in HBM
     singles = []
     for (psi, coh), sh in zip(ims, shapes):
@@ -549,6 +557,10 @@ def main():
     ap.add_argument("--only-max-scale", action="store_true", help="run only the 40000 x 40000 workload")
     ap.add_argument("--only-batched", action="store_true",
                     help="run only the 48-burst batched stack, with per-kernel durations")
+    ap.add_argument("--stack-irls", type=int, default=None,
+                    help="IRLS steps of the batched stack (default 10; fewer for counter runs under a profiler)")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="skip the 48 sequential plain calls the batched stack is compared with")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

@@ -220,10 +220,10 @@ def plan_reset():
     check(load().irls2d_plan_reset(), "irls2d_plan_reset")
 
 
-COLUMN_AUTO, COLUMN_TRANSPOSE, COLUMN_DIRECT, COLUMN_SLAB = 0, 1, 2, 3
+COLUMN_AUTO, COLUMN_TRANSPOSE, COLUMN_DIRECT, COLUMN_SLAB, COLUMN_TRIDIAG = 0, 1, 2, 3, 4
 
 
-COLUMN_STRATEGIES = ("transpose", "direct", "long", "slab", "slab_register")
+COLUMN_STRATEGIES = ("transpose", "direct", "long", "slab", "slab_register", "tridiag")
 
 
 def describe_columns(shapes):
@@ -236,5 +236,6 @@ def describe_columns(shapes):
 
 def set_column_mode(mode: int):
     """0 = automatic, 1 = transpose + column kernel, 2 = column kernel on the row-major field,
-    3 = four-step slab passes on the row-major field wherever the column length allows them."""
+    3 = four-step slab passes on the row-major field wherever the column length allows them,
+    4 = tridiagonal column solves wherever H <= 12288."""
     check(load().irls2d_set_column_mode(int(mode)), "irls2d_set_column_mode")

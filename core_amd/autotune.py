@@ -238,7 +238,7 @@ def tune(shapes, device="cuda", reps=5, path=DEFAULT_FILE, save=True, verbose=Fa
         # passes): measured for the record; the library chooses it per image
         column_ms = {}
         for name, mode in (("transpose", _lib.COLUMN_TRANSPOSE), ("direct", _lib.COLUMN_DIRECT),
-                           ("slab", _lib.COLUMN_SLAB), ("auto", _lib.COLUMN_AUTO)):
+                           ("slab", _lib.COLUMN_SLAB), ("tridiag", _lib.COLUMN_TRIDIAG), ("auto", _lib.COLUMN_AUTO)):
             _lib.set_column_mode(mode)
             column_ms[name] = round(batch.time(reps), 4)
         after = batch.time(reps)

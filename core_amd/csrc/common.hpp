@@ -159,7 +159,7 @@ struct ImgDesc {
   int rowpart0;    // first block-partial slot of the image's row-pair blocks (short rows)
   int coldirect;   // 1: the column DCTs read column pairs straight from the row-major field
   int lbpart;      // Bluestein long rows: first slot of the pack / post block partials (partials2)
-  int pad;
+  int coltri;      // 1: the column half of the preconditioner runs as tridiagonal solves (k_col_tri)
 };
 
 // Per-image solver state.  Written by k_state_init and k_finalize (one block per image), read by
@@ -201,6 +201,7 @@ struct Geo {
   const int *slab1, *slab2;                  // slab column passes: K1 / K3, and K2
   // Bluestein long lines: pack / post, fft1 / fft3 and fft2 blocks, rows then columns
   const int *browP, *brow1, *brow2, *bcolP, *bcol1, *bcol2;
+  const int *coltri;                         // tridiagonal column solves: 256 columns per block
 };
 
 struct Ctx {

@@ -207,14 +207,19 @@ static int64_t long_blue_len(int64_t N, int &M1, int &M2) {
   return best;
 }
 
+// Lines whose DCTs run as two four-step passes: longer than MAX_FFT_LEN, or with a Bluestein
+// length beyond it (prime factors above 5 from 6145 on).
+static bool long_line(int64_t N) {
+  return N > MAX_FFT_LEN || (!smooth235(N) && bluestein_len(N) > MAX_FFT_LEN);
+}
+
 int supported(int64_t N) {
   if (N < 1) return 0;
-  if (N > MAX_FFT_LEN) {
+  if (long_line(N)) {
     int a, b;
     return (long_split(N, a, b) || long_blue_len(N, a, b) > 0) ? 1 : 0;
   }
-  if (smooth235(N)) return 1;
-  return bluestein_len(N) <= MAX_FFT_LEN;
+  return 1;
 }
 
 // 16 values per thread up to 8192 points (two workgroups of 8 waves per CU); above, 12 on up to
@@ -464,7 +469,7 @@ static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Fixed block-start tables of a layout; the class tables (LaunchClass::geo / geo2) follow them.
 enum { G_TILE = 0, G_TRANS, G_ROWL1, G_ROWL2, G_COLL1, G_COLL2, G_BROWP, G_BROW1, G_BROW2, G_BCOLP, G_BCOL1,
-       G_BCOL2, G_FIXED };
+       G_BCOL2, G_TRI, G_FIXED };
 
 // Images whose transform pass shares one kernel instantiation: one launch per class and pass.
 // Every choice is a function of the image's own shape (and the global mode / plan overrides), so an
@@ -674,7 +679,7 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     d.W = w;
     d.P = (int)Lo.pitch[i];
     d.PT = (int)pitch_of(h);
-    const bool lw = w > MAX_FFT_LEN, lh = h > MAX_FFT_LEN;
+    const bool lw = long_line(w), lh = long_line(h);
     d.rowplan = lw ? -1 : add_plan(w);
     d.colplan = lh ? -1 : add_plan(h);
     d.rowlong = lw ? add_long(w) : -1;
@@ -687,12 +692,17 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     d.colslab = want_slab ? add_slab(h) : -1;
     d.nslab = (w + 2 * SLAB_CP - 1) / (2 * SLAB_CP);
     const bool slab = d.colslab >= 0;
-    d.collong = (lh && !slab) ? add_long(h) : -1;
-    // direct column DCTs when this image's own column transform is Bluestein (its arithmetic hides
-    // the strided access), transposes otherwise (measured on MI355X: -16% per PCG iteration for the
-    // 48-burst stack with direct, +16% at 8192^2)
-    const bool colk = !slab && !lh;
-    d.coldirect = colk && (g_col_mode == 2 || ((g_col_mode == 0 || g_col_mode == 3) && Lo.plans[d.colplan].blue));
+    // tridiagonal column solves when this image's own column transform would be Bluestein (four
+    // length-M >= 2H - 1 FFTs per column pair, against 16 B/px of memory traffic), short or long,
+    // or for every image in mode 4; column DCTs otherwise: LDS-resident direct on the row-major
+    // field in mode 2, on the transposed copy in the others (measured on MI355X: direct is +16% per
+    // PCG iteration at 8192^2), two-pass long transforms above the LDS limit
+    int lf1 = 0, lf2 = 0;
+    const bool hblue = lh ? !long_split(h, lf1, lf2) : Lo.plans[d.colplan].blue;
+    d.coltri = !slab && (g_col_mode == 4 || ((g_col_mode == 0 || g_col_mode == 3) && hblue));
+    d.collong = (lh && !slab && !d.coltri) ? add_long(h) : -1;
+    const bool colk = !slab && !lh && !d.coltri;
+    d.coldirect = colk && g_col_mode == 2;
     // transposed copy (W x PT), or the slab scratch (H rows of nslab x SLAB_CP float2 = P floats)
     Lo.offT[i] = Lo.totalT;
     d.offT = Lo.totalT;
@@ -727,8 +737,9 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     d.offS = Lo.totalS;
     Lo.totalS += (sz + 31) / 32 * 32;
     adds[i].push_back({G_TILE, ((int64_t)(h + TILE_H - 1) / TILE_H) * ((w + TILE_W - 1) / TILE_W)});
-    if (!slab && !d.coldirect)
+    if (!slab && !d.coldirect && !d.coltri)
       adds[i].push_back({G_TRANS, ((int64_t)(h + TT - 1) / TT) * ((w + TT - 1) / TT)});
+    if (d.coltri) adds[i].push_back({G_TRI, (w + TRI_THREADS - 1) / TRI_THREADS});
     if (!lw) {
       const DctPlan &R = Lo.plans[d.rowplan];
       LaunchClass &C = Lo.rowcls[row_class(R.ept)];
@@ -918,6 +929,7 @@ static Ctx make_ctx(const Layout &Lo, unsigned char *ws, float *u, const Params 
   c.geo.bcolP = (const int *)(ws + Lo.o_geo[G_BCOLP]);
   c.geo.bcol1 = (const int *)(ws + Lo.o_geo[G_BCOL1]);
   c.geo.bcol2 = (const int *)(ws + Lo.o_geo[G_BCOL2]);
+  c.geo.coltri = (const int *)(ws + Lo.o_geo[G_TRI]);
   // class tables: set per launch (class_ctx)
   c.geo.rowpair = c.geo.colpair = c.geo.slab1 = c.geo.slab2 = nullptr;
   c.ws_base = ws;
@@ -1104,6 +1116,8 @@ static void col_pass(const Layout &Lo, const Ctx &c, int guard, hipStream_t s) {
   }
   lb_dct2(Lo, c, 1, false, c.f.Tt, c.f.Tt, nullptr, guard, s);
   lb_dct3(Lo, c, 1, c.f.Tt, c.f.Tt, 0, guard, s);
+  if (Lo.geo[G_TRI].back() > 0)
+    LAUNCH("k_col_tri", k_col_tri, dim3(Lo.geo[G_TRI].back()), dim3(TRI_THREADS), 0, s, c, c.f.T, guard);
   for (const LaunchClass &C : Lo.colcls) {
     const int nb = Lo.geo[C.geo].back();
     if (nb == 0) continue;
@@ -1615,8 +1629,9 @@ int plan_describe(int64_t N, int64_t *fft_len, int *rad, int *nrad, int *ept) {
 
 int set_column_mode(int mode) {
   g_err.clear();
-  if (mode < 0 || mode > 3)
-    return fail(IRLS2D_EINVAL, "column mode must be 0 (auto), 1 (transpose), 2 (direct) or 3 (slab)");
+  if (mode < 0 || mode > 4)
+    return fail(IRLS2D_EINVAL,
+                "column mode must be 0 (auto), 1 (transpose), 2 (direct), 3 (slab) or 4 (tridiagonal)");
   {
     std::lock_guard<std::mutex> lk(g_plan_mu);
     g_col_mode = mode;
@@ -2025,6 +2040,8 @@ int irls2d_describe_columns(int n, const int64_t *H, const int64_t *W, int32_t *
       strategy[i] = irls2d::slab_fast(P.N1, P.N2) ? IRLS2D_COLUMNS_SLAB_REGISTER : IRLS2D_COLUMNS_SLAB;
     } else if (d.collong >= 0) {
       strategy[i] = IRLS2D_COLUMNS_LONG;
+    } else if (d.coltri) {
+      strategy[i] = IRLS2D_COLUMNS_TRIDIAG;
     } else {
       strategy[i] = d.coldirect ? IRLS2D_COLUMNS_DIRECT : IRLS2D_COLUMNS_TRANSPOSE;
     }

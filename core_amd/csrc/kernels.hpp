@@ -953,6 +953,147 @@ __global__ __launch_bounds__(fft_threads_max<E>()) __attribute__((amdgpu_waves_p
 
 #ifndef IRLS2D_TRANSFORMS_ONLY
 // ---------------------------------------------------------------------------------------------
+// Column half of the preconditioner as tridiagonal solves (utils_unwrap2d.py:1390-1403 restated)
+// ---------------------------------------------------------------------------------------------
+// The preconditioner z = IDCT2(-DCT2(r) / lambda) with DC zeroed is the pseudo-inverse of the
+// Neumann five-point Laplacian L_W (+) L_H (lambda_kl = -(mu_k + nu_l), mu_k = 4 sin^2(pi k / 2W),
+// nu_l = 4 sin^2(pi l / 2H): the DCT-II diagonalises the second-difference matrix with
+// reflecting ends).  After the row DCT-II, column k of the row spectrum holds c_k and the column
+// DCT pair with the division by -lambda equals the solve (L_H + mu_k I) x = c_k, L_H =
+// tridiag(-1, 2, -1) with corner entries 1: the same operator, applied without transforms.  k_col_tri
+// solves it by LU in float64, one thread per column, in place on the row-major spectrum field
+// (coalesced rows of 256 columns); column 0 (mu_0 = 0, singular) takes the pseudo-inverse with the
+// mean removed from input and output, which is the DC zeroing.
+//
+// The LU pivots of L_H + mu I have a closed form: with cosh(theta) = 1 + mu / 2,
+//   1 / d_i = cosh((i + 1/2) theta) / cosh((i + 3/2) theta) = e^-theta (1 + t_i) / (1 + e^-2theta t_i)
+//   for i < H - 1, t_i = e^-(2i+1)theta, and the last (reflecting) row
+//   1 / d_{H-1} = (1 + t_{H-1}) / ((e^theta - 1)(1 - e^-2H theta))
+// (from d_0 = 1 + mu and d_i = 2 + mu - 1 / d_{i-1} by the identity 2 cosh a cosh b = cosh(a + b) +
+// cosh(a - b)); no cancellation, unlike the recurrence, whose pivots lose digits for small mu.
+// Forward: w_i = (c_i + w_{i-1}) / d_i (the float32 w overwrites c); backward: x_i = w_i + x_{i+1} / d_i.
+// t_i is re-anchored with exp() at the start of every batch of TRI_U rows.
+
+constexpr int TRI_THREADS = 256;
+constexpr int TRI_U = 32;
+
+struct TriPivots {
+  double em, q, eq2, theta, inv_last;
+  __device__ __forceinline__ double t_at(int i) const { return exp(-(2.0 * i + 1.0) * theta); }
+  __device__ __forceinline__ double inv(double t, bool last) const {
+    if (last) return inv_last;
+    return t > 0x1p-60 ? em * (1.0 + t) / (1.0 + q * t) : em;
+  }
+};
+
+// x = L_H^+ (c - mean c) with mean(x) = 0, for column 0.  With g_i = sum_{j<=i} (c_j - m) and
+// x_{i+1} = x_i - g_i, x_i = X_i + m i (i + 1) / 2 where X is the same recursion on c itself, and
+// sum x = sum X + m (H - 1) H (H + 1) / 6: one pass for m and sum X, one to write x - mean x.
+__device__ void tri_singular(float *__restrict__ col, unsigned P, int H) {
+  double sc = 0.0, sx = 0.0;
+  double G = 0.0, X = 0.0;
+  for (int i0 = 0; i0 < H; i0 += TRI_U) {
+    float v[TRI_U];
+#pragma unroll
+    for (int u = 0; u < TRI_U; ++u) v[u] = col[(size_t)min(i0 + u, H - 1) * P];
+#pragma unroll
+    for (int u = 0; u < TRI_U; ++u)
+      if (i0 + u < H) {
+        sc += v[u];
+        sx += X;
+        G += v[u];
+        X -= G;
+      }
+  }
+  const double m = sc / H;
+  const double mean = (sx + m * ((double)(H - 1) * H * (H + 1) / 6.0)) / H;
+  G = X = 0.0;
+  for (int i0 = 0; i0 < H; i0 += TRI_U) {
+    float v[TRI_U];
+#pragma unroll
+    for (int u = 0; u < TRI_U; ++u) v[u] = col[(size_t)min(i0 + u, H - 1) * P];
+#pragma unroll
+    for (int u = 0; u < TRI_U; ++u) {
+      const int i = i0 + u;
+      if (i < H) {
+        col[(size_t)i * P] = (float)(X + m * (0.5 * (double)i * (i + 1)) - mean);
+        G += v[u];
+        X -= G;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(TRI_THREADS) void k_col_tri(Ctx c, float *__restrict__ T, int guard) {
+  const int img = find_img(c.geo.coltri, c.n, blockIdx.x);
+  const ImgDesc d = c.img[img];
+  if (guard_off(&c.st[img], guard)) return;
+  const int k = (int)(blockIdx.x - c.geo.coltri[img]) * TRI_THREADS + (int)threadIdx.x;
+  const int H = d.H;
+  if (k >= d.W) return;
+  float *__restrict__ col = T + d.off + k;
+  const unsigned P = d.P;
+  if (k == 0) {
+    tri_singular(col, P, H);
+    return;
+  }
+  const double sn = sin(M_PI * (double)k / (2.0 * d.W));
+  const double mu = 4.0 * sn * sn;
+  const double ch = 1.0 + 0.5 * mu, sh = sqrt(mu * (1.0 + 0.25 * mu));
+  TriPivots tp;
+  tp.em = 1.0 / (ch + sh);
+  tp.q = tp.em * tp.em;
+  tp.eq2 = (ch + sh) * (ch + sh);
+  const double ep1 = 0.5 * mu + sh;  // e^theta - 1
+  tp.theta = log1p(ep1);
+  // 1 - e^-theta t_{H-1} = 1 - e^-2H theta
+  tp.inv_last = (1.0 + tp.t_at(H - 1)) / (ep1 * -expm1(-2.0 * H * tp.theta));
+  // forward sweep; the next batch's loads are issued before the current batch's chain
+  double w = 0.0;
+  float v[TRI_U], nv[TRI_U];
+#pragma unroll
+  for (int u = 0; u < TRI_U; ++u) v[u] = col[(size_t)min(u, H - 1) * P];
+  for (int i0 = 0; i0 < H; i0 += TRI_U) {
+    const int n0 = i0 + TRI_U;
+#pragma unroll
+    for (int u = 0; u < TRI_U; ++u) nv[u] = col[(size_t)min(n0 + u, H - 1) * P];
+    double t = tp.t_at(i0);
+#pragma unroll
+    for (int u = 0; u < TRI_U; ++u) {
+      const int i = i0 + u;
+      if (i < H) {
+        w = tp.inv(t, i == H - 1) * ((double)v[u] + w);
+        col[(size_t)i * P] = (float)w;
+      }
+      t *= tp.q;
+    }
+#pragma unroll
+    for (int u = 0; u < TRI_U; ++u) v[u] = nv[u];
+  }
+  // backward sweep over batches [i1 - TRI_U, i1), from the bottom
+  double x = 0.0;
+#pragma unroll
+  for (int u = 0; u < TRI_U; ++u) v[u] = col[(size_t)max(H - 1 - u, 0) * P];
+  for (int i1 = H; i1 > 0; i1 -= TRI_U) {
+    const int n1 = i1 - TRI_U;
+#pragma unroll
+    for (int u = 0; u < TRI_U; ++u) nv[u] = col[(size_t)max(n1 - 1 - u, 0) * P];
+    double t = tp.t_at(i1 - 1);
+#pragma unroll
+    for (int u = 0; u < TRI_U; ++u) {
+      const int i = i1 - 1 - u;
+      if (i >= 0) {
+        x = (double)v[u] + tp.inv(t, i == H - 1) * x;
+        col[(size_t)i * P] = (float)x;
+      }
+      t *= tp.eq2;
+    }
+#pragma unroll
+    for (int u = 0; u < TRI_U; ++u) v[u] = nv[u];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
 // End of an IRLS step (utils_unwrap2d.py:1599-1626)
 // ---------------------------------------------------------------------------------------------
 

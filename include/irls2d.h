@@ -169,15 +169,16 @@ int irls2d_plan_reset(void);
  * 0 = automatic (default), decided per image from its own shape: images with both sides <= 12288
  * whose column length, at least 256, splits into a power-of-two pair N1 N2 (N1 16 .. 128, N2
  * 16 .. 64: heights 256 .. 8192) run the three four-step slab passes with register-stage kernels on
- * the row-major field (no transposes; one launch per split in a batch); the other images read
- * their column pairs straight from the row-major field when their column transform is Bluestein
- * (whose arithmetic hides the strided access), otherwise their rows are transposed, transformed
- * and transposed back;
- * 1 = always transpose; 2 = always direct; 3 = slab passes for every image whose column length
- * allows them (2^a 3^b 5^c with such a split, from 4), the automatic choice for the rest.  Other
- * columns longer than 12288 transpose and run the two-pass long transform.  Since every choice is
- * per image, an image's result is the same alone and in any batch (under one mode and one plan
- * set).  Drops cached layouts and hipGraphs.  core_amd.autotune measures the strategies.
+ * the row-major field (no transposes; one launch per split in a batch); images whose column
+ * transform would be Bluestein (a column length with a prime factor above 5) replace the column
+ * DCT pair by the equivalent tridiagonal solves (L_H + mu_k I) x = c_k, in float64, on the
+ * row-major field; the other images transpose their rows, transform and transpose back;
+ * 1 = always transpose; 2 = column DCTs on the row-major field; 3 = slab passes for every image
+ * whose column length allows them (2^a 3^b 5^c with such a split, from 4), the automatic choice
+ * for the rest; 4 = tridiagonal solves for every image with H <= 12288.  Other columns longer than
+ * 12288 transpose and run the two-pass long transform.  Since every choice is per image, an
+ * image's result is the same alone and in any batch (under one mode and one plan set).  Drops
+ * cached layouts and hipGraphs.  core_amd.autotune measures the strategies.
  */
 int irls2d_set_column_mode(int mode);
 
@@ -222,6 +223,7 @@ int irls2d_goldstein(int64_t ny, int64_t nx, const void *phase, const float *cor
 #define IRLS2D_COLUMNS_LONG 2           /* transpose, two-pass four-step DCTs (H > 12288, no slab split) */
 #define IRLS2D_COLUMNS_SLAB 3           /* three four-step slab passes, batched LDS FFTs */
 #define IRLS2D_COLUMNS_SLAB_REGISTER 4  /* three four-step slab passes, register-stage FFTs */
+#define IRLS2D_COLUMNS_TRIDIAG 5        /* float64 tridiagonal solves on the row-major field */
 int irls2d_describe_columns(int n, const int64_t *H, const int64_t *W, int32_t *strategy);
 
 /*

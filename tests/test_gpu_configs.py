@@ -113,7 +113,7 @@ def test_plain_8192_workload_in_full(hip):
 def test_stack_48_batched(hip):
     """The 48-burst stack (H ~ U{1000..1500}, W ~ U{4000..6000}, seeds 1000 + i, ramp 0-80 rad,
     coherence U[0.2, 1]) in one batched call: every image finite with psi's NaN mask and bitwise
-    equal to its own plain call (a slab, a direct and a transposed column pass among them); the
+    equal to its own plain call (a slab, a tridiagonal and a transposed column pass among them); the
     three smallest bursts against the FP64 and float32 trajectories."""
     from core_amd import irls_unwrap_2d, irls_unwrap_2d_batch
     wls = S.stack_48()
@@ -126,7 +126,7 @@ def test_stack_48_batched(hip):
     torch.cuda.empty_cache()
     from core_amd import _lib
     strat = _lib.describe_columns([(wl.H, wl.W) for wl in wls])
-    picks = sorted({strat.index(k) for k in ("slab_register", "direct", "transpose") if k in strat} | {0, 47})
+    picks = sorted({strat.index(k) for k in ("slab_register", "tridiag", "direct", "transpose") if k in strat} | {0, 47})
     for i in picks:  # plain calls of bursts of every column strategy present: bitwise the same solve
         psi, coh = ims[i]
         up, cp = irls_unwrap_2d(psi, coh, device='cuda', **FIXED)

@@ -46,10 +46,10 @@ def test_static_scratch_size_matches_header(lib):
 def test_column_strategy_is_per_image(lib):
     """Each image's column strategy follows from its own shape, the same alone and in any batch:
     power-of-two-split heights take the register slab passes (one launch per split), Bluestein
-    heights the direct column pass, the rest the transposes."""
+    heights the tridiagonal column solves, the rest the transposes."""
     shapes = [(1024, 4000), (1030, 5000), (1000, 4500), (2048, 300), (96, 130), (103, 64)]
     alone = [lib.describe_columns([sh])[0] for sh in shapes]
-    assert alone == ["slab_register", "direct", "transpose", "slab_register", "transpose", "direct"]
+    assert alone == ["slab_register", "tridiag", "transpose", "slab_register", "transpose", "tridiag"]
     assert lib.describe_columns(shapes) == alone
     assert lib.describe_columns(shapes[::-1]) == alone[::-1]
 
@@ -76,6 +76,9 @@ def test_workspace_size_grows_with_pixels(lib):
 def test_supported_lengths(lib):
     assert lib.supported_length(8192) and lib.supported_length(12288)
     assert lib.supported_length(4999)            # Bluestein
+    # prime factors above 5 between 6145 and 12288 (Bluestein length beyond the LDS limit): the
+    # four-step Bluestein passes
+    assert all(lib.supported_length(n) for n in (6145, 6151, 7001, 10007, 12007, 12287))
     assert lib.supported_length(12289)           # beyond the LDS-resident lengths: four-step Bluestein
     assert lib.supported_length(20011)           # prime
     assert lib.supported_length(40000)           # four-step 200 x 200
@@ -241,10 +244,10 @@ def test_column_strategies(lib):
         assert d([(2048, 9), (8192, 70)]) == ["slab_register"] * 2
         assert d([(6000, 6000)]) == ["transpose"]
         assert d([(96, 130)]) == ["transpose"]                     # below the slab threshold
-        assert d([(127, 211)]) == ["direct"]                       # Bluestein columns
+        assert d([(127, 211)]) == ["tridiag"]                      # Bluestein columns
         assert d([(12800, 8)]) == ["long"]                         # long sides: transposed two-pass path
         assert d([(1000, 12500)]) == ["transpose"]
-        assert d([(8192, 70), (127, 211), (12800, 8)]) == ["slab_register", "direct", "long"]
+        assert d([(8192, 70), (127, 211), (12800, 8)]) == ["slab_register", "tridiag", "long"]
         lib.set_column_mode(lib.COLUMN_SLAB)
         assert d([(64, 48), (96, 130)]) == ["slab", "slab"]
         assert d([(8192, 70), (1000, 7)]) == ["slab_register", "slab"]  # 1000 = 40 x 25: batched LDS
@@ -252,18 +255,27 @@ def test_column_strategies(lib):
         assert d([(40000, 64)]) == ["slab"]                        # 40000 = 250 x 160
         assert d([(177147, 8)]) == ["long"]                        # 3^11: no split within the block limits
         assert d([(1000, 12500)]) == ["slab"]                      # long rows write T in slab order too
-        assert d([(127, 211)]) == ["direct"]                       # no slab split: automatic choice
+        assert d([(127, 211)]) == ["tridiag"]                      # no slab split: automatic choice
         lib.set_column_mode(lib.COLUMN_TRANSPOSE)
         assert d([(8192, 8192)]) == ["transpose"]
+        lib.set_column_mode(lib.COLUMN_DIRECT)
+        assert d([(127, 211), (8192, 70), (12800, 8)]) == ["direct", "direct", "long"]
+        lib.set_column_mode(lib.COLUMN_TRIDIAG)
+        assert d([(127, 211), (8192, 70), (96, 130), (12800, 8)]) == ["tridiag"] * 4
+        lib.set_column_mode(lib.COLUMN_AUTO)
+        # columns with a prime factor above 5: tridiagonal solves at any height; 5-smooth long
+        # columns keep the two-pass transforms
+        assert d([(13001, 9), (7001, 9), (12800, 8), (9, 7001)]) == ["tridiag", "tridiag", "long", "transpose"]
     finally:
         lib.set_column_mode(lib.COLUMN_AUTO)
 
 
 def test_column_mode_validation(lib):
     with pytest.raises(ValueError):
-        lib.set_column_mode(4)
+        lib.set_column_mode(5)
     with pytest.raises(ValueError):
         lib.set_column_mode(-1)
+    lib.set_column_mode(lib.COLUMN_TRIDIAG)
     lib.set_column_mode(lib.COLUMN_SLAB)
     lib.set_column_mode(lib.COLUMN_DIRECT)
     lib.set_column_mode(lib.COLUMN_AUTO)
