@@ -561,6 +561,8 @@ def main():
                     help="IRLS steps of the batched stack (default 10; fewer for counter runs under a profiler)")
     ap.add_argument("--no-sequential", action="store_true",
                     help="skip the 48 sequential plain calls the batched stack is compared with")
+    ap.add_argument("--column-mode", choices=["auto", "transpose", "direct", "slab", "tridiag"], default="auto",
+                    help="column-pass strategy for every solve (irls2d_set_column_mode; default automatic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -575,6 +577,8 @@ def main():
     dev = torch.device("cuda", 0)
     from core_amd import autotune
     n_tuned = autotune.apply()  # plans measured by `python -m core_amd.autotune` on this GPU type
+    L.set_column_mode({"auto": L.COLUMN_AUTO, "transpose": L.COLUMN_TRANSPOSE, "direct": L.COLUMN_DIRECT,
+                       "slab": L.COLUMN_SLAB, "tridiag": L.COLUMN_TRIDIAG}[args.column_mode])
     if args.only_max_scale:
         print(json.dumps({"max_scale": max_scale(args, dev)}), flush=True)
         return

@@ -237,5 +237,5 @@ def describe_columns(shapes):
 def set_column_mode(mode: int):
     """0 = automatic, 1 = transpose + column kernel, 2 = column kernel on the row-major field,
     3 = four-step slab passes on the row-major field wherever the column length allows them,
-    4 = tridiagonal column solves wherever H <= 12288."""
+    4 = tridiagonal column solves for every image."""
     check(load().irls2d_set_column_mode(int(mode)), "irls2d_set_column_mode")

@@ -692,21 +692,21 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     d.colslab = want_slab ? add_slab(h) : -1;
     d.nslab = (w + 2 * SLAB_CP - 1) / (2 * SLAB_CP);
     const bool slab = d.colslab >= 0;
-    // tridiagonal column solves when this image's own column transform would be Bluestein (four
-    // length-M >= 2H - 1 FFTs per column pair, against 16 B/px of memory traffic), short or long,
-    // or for every image in mode 4; column DCTs otherwise: LDS-resident direct on the row-major
-    // field in mode 2, on the transposed copy in the others (measured on MI355X: direct is +16% per
-    // PCG iteration at 8192^2), two-pass long transforms above the LDS limit
-    int lf1 = 0, lf2 = 0;
-    const bool hblue = lh ? !long_split(h, lf1, lf2) : Lo.plans[d.colplan].blue;
-    d.coltri = !slab && (g_col_mode == 4 || ((g_col_mode == 0 || g_col_mode == 3) && hblue));
+    // tridiagonal column solves (16 B/px of memory traffic) when this image's own column transform
+    // would be Bluestein (four length-M >= 2H - 1 FFTs per column pair) or a long two-pass
+    // transform with its transposes, or for every image in mode 4 (measured on MI355X: 48-burst
+    // stack +24%, 40000^2 +32% per solve); column DCTs otherwise: LDS-resident direct on the
+    // row-major field in mode 2, on the transposed copy in the others (direct is +16% per PCG
+    // iteration at 8192^2; one thread per column has too few threads in flight for square images)
+    const bool hblue = !lh && Lo.plans[d.colplan].blue;
+    d.coltri = !slab && (g_col_mode == 4 || ((g_col_mode == 0 || g_col_mode == 3) && (hblue || lh)));
     d.collong = (lh && !slab && !d.coltri) ? add_long(h) : -1;
     const bool colk = !slab && !lh && !d.coltri;
     d.coldirect = colk && g_col_mode == 2;
     // transposed copy (W x PT), or the slab scratch (H rows of nslab x SLAB_CP float2 = P floats)
     Lo.offT[i] = Lo.totalT;
     d.offT = Lo.totalT;
-    Lo.totalT += slab ? (int64_t)h * d.P : (int64_t)w * d.PT;
+    Lo.totalT += slab ? (int64_t)h * d.P : (d.coltri || d.coldirect) ? 0 : (int64_t)w * d.PT;
     int64_t sz = 0;
     if (lw) {
       const LongPlan &P = Lo.lplans[d.rowlong];
@@ -739,7 +739,7 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     adds[i].push_back({G_TILE, ((int64_t)(h + TILE_H - 1) / TILE_H) * ((w + TILE_W - 1) / TILE_W)});
     if (!slab && !d.coldirect && !d.coltri)
       adds[i].push_back({G_TRANS, ((int64_t)(h + TT - 1) / TT) * ((w + TT - 1) / TT)});
-    if (d.coltri) adds[i].push_back({G_TRI, (w + TRI_THREADS - 1) / TRI_THREADS});
+    if (d.coltri) adds[i].push_back({G_TRI, (w - 1 + TRI_THREADS - 1) / TRI_THREADS + 1});
     if (!lw) {
       const DctPlan &R = Lo.plans[d.rowplan];
       LaunchClass &C = Lo.rowcls[row_class(R.ept)];

@@ -969,54 +969,98 @@ __global__ __launch_bounds__(fft_threads_max<E>()) __attribute__((amdgpu_waves_p
 //   1 / d_i = cosh((i + 1/2) theta) / cosh((i + 3/2) theta) = e^-theta (1 + t_i) / (1 + e^-2theta t_i)
 //   for i < H - 1, t_i = e^-(2i+1)theta, and the last (reflecting) row
 //   1 / d_{H-1} = (1 + t_{H-1}) / ((e^theta - 1)(1 - e^-2H theta))
+// (for H = 1 this is 1 / mu)
 // (from d_0 = 1 + mu and d_i = 2 + mu - 1 / d_{i-1} by the identity 2 cosh a cosh b = cosh(a + b) +
 // cosh(a - b)); no cancellation, unlike the recurrence, whose pivots lose digits for small mu.
 // Forward: w_i = (c_i + w_{i-1}) / d_i (the float32 w overwrites c); backward: x_i = w_i + x_{i+1} / d_i.
-// t_i is re-anchored with exp() at the start of every batch of TRI_U rows.
+// t_i is re-anchored with exp() at the start of every batch of TRI_U rows; a batch whose t starts
+// below 2^-60 in every lane takes 1 / d_i = e^-theta (the relative difference is below 2^-60).
 
 constexpr int TRI_THREADS = 256;
-constexpr int TRI_U = 32;
+constexpr int TRI_U = 16;
 
 struct TriPivots {
   double em, q, eq2, theta, inv_last;
   __device__ __forceinline__ double t_at(int i) const { return exp(-(2.0 * i + 1.0) * theta); }
-  __device__ __forceinline__ double inv(double t, bool last) const {
-    if (last) return inv_last;
-    return t > 0x1p-60 ? em * (1.0 + t) / (1.0 + q * t) : em;
+  // 1 / d_i for i < H - 1: e^-theta (1 + t) / (1 + e^-2theta t), 1 + e^-2theta t in [1, 2), by the
+  // hardware reciprocal and two Newton steps
+  __device__ __forceinline__ double inv(double t) const {
+    const double x = fma(q, t, 1.0);
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return em * (1.0 + t) * r;
   }
 };
 
-// x = L_H^+ (c - mean c) with mean(x) = 0, for column 0.  With g_i = sum_{j<=i} (c_j - m) and
-// x_{i+1} = x_i - g_i, x_i = X_i + m i (i + 1) / 2 where X is the same recursion on c itself, and
-// sum x = sum X + m (H - 1) H (H + 1) / 6: one pass for m and sum X, one to write x - mean x.
-__device__ void tri_singular(float *__restrict__ col, unsigned P, int H) {
-  double sc = 0.0, sx = 0.0;
-  double G = 0.0, X = 0.0;
-  for (int i0 = 0; i0 < H; i0 += TRI_U) {
-    float v[TRI_U];
+// Rows [i0, i0 + TRI_U) of column k (clamped to the first and last row) into registers.  Row bases
+// are uniform (scalar registers), the column is a per-lane 32-bit offset.
+template <typename F>
+__device__ __forceinline__ F *uniform_ptr(F *p) {
+  const unsigned long long a = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a), hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+  return (F *)(((unsigned long long)hi << 32) | lo);
+}
+
+__device__ __forceinline__ void tri_load(float (&v)[TRI_U], const float *__restrict__ base, unsigned k, unsigned P, int i0,
+                                         int H) {
 #pragma unroll
-    for (int u = 0; u < TRI_U; ++u) v[u] = col[(size_t)min(i0 + u, H - 1) * P];
+  for (int u = 0; u < TRI_U; ++u) v[u] = uniform_ptr(base + (size_t)min(max(i0 + u, 0), H - 1) * P)[k];
+}
+
+// x = L_H^+ (c - mean c) with mean(x) = 0, for column 0, by one wave.  With m = mean c,
+// G_i = sum_{j<=i} c_j and X_i = -sum_{j<i} G_j (X_0 = 0), the solution of x_{i+1} = x_i - g_i,
+// g_i = sum_{j<=i} (c_j - m), is x_i = X_i + m i (i + 1) / 2 - mean, where
+// sum_i X_i = -sum_j (H - 1 - j) G_j.  Lane l owns rows [l L, l L + L): one pass sums its rows
+// locally (G and the weighted sums from G = 0), wave scans give each lane G and X at its first row
+// and the totals, a second pass writes its rows.
+__device__ __forceinline__ double wave_scan_excl(double v, int lane) {
+  double inc = v;
 #pragma unroll
-    for (int u = 0; u < TRI_U; ++u)
-      if (i0 + u < H) {
-        sc += v[u];
-        sx += X;
-        G += v[u];
-        X -= G;
-      }
+  for (int o = 1; o < 64; o <<= 1) {
+    const double y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
   }
-  const double m = sc / H;
-  const double mean = (sx + m * ((double)(H - 1) * H * (H + 1) / 6.0)) / H;
-  G = X = 0.0;
-  for (int i0 = 0; i0 < H; i0 += TRI_U) {
+  return inc - v;
+}
+
+__device__ void tri_singular(float *__restrict__ base, unsigned P, int H, int lane) {
+  const int L = (H + 63) / 64;
+  const int r0 = min(lane * L, H), r1 = min(r0 + L, H);
+  const int len = r1 - r0;
+  double sc = 0.0, sg = 0.0, sw = 0.0, g = 0.0;
+  for (int i0 = r0; i0 < r1; i0 += TRI_U) {
     float v[TRI_U];
 #pragma unroll
-    for (int u = 0; u < TRI_U; ++u) v[u] = col[(size_t)min(i0 + u, H - 1) * P];
+    for (int u = 0; u < TRI_U; ++u) v[u] = base[(size_t)min(i0 + u, H - 1) * P];
 #pragma unroll
     for (int u = 0; u < TRI_U; ++u) {
       const int i = i0 + u;
-      if (i < H) {
-        col[(size_t)i * P] = (float)(X + m * (0.5 * (double)i * (i + 1)) - mean);
+      if (i < r1) {
+        sc += v[u];
+        g += v[u];
+        sg += g;
+        sw += (double)(H - 1 - i) * g;
+      }
+    }
+  }
+  const double gstart = wave_scan_excl(sc, lane);
+  const double m = wave_sum(sc) / H;
+  const double xstart = -wave_scan_excl(len * gstart + sg, lane);
+  // sum_{j in rows} (H - 1 - j) = len (H - 1) - (r0 + r1 - 1) len / 2
+  const double wsum = (double)len * (H - 1) - 0.5 * (double)(r0 + r1 - 1) * len;
+  const double sumx = -wave_sum(gstart * wsum + sw);
+  const double mean = (sumx + m * ((double)(H - 1) * H * (H + 1) / 6.0)) / H;
+  double G = gstart, X = xstart;
+  for (int i0 = r0; i0 < r1; i0 += TRI_U) {
+    float v[TRI_U];
+#pragma unroll
+    for (int u = 0; u < TRI_U; ++u) v[u] = base[(size_t)min(i0 + u, H - 1) * P];
+#pragma unroll
+    for (int u = 0; u < TRI_U; ++u) {
+      const int i = i0 + u;
+      if (i < r1) {
+        base[(size_t)i * P] = (float)(X + m * (0.5 * (double)i * (i + 1)) - mean);
         G += v[u];
         X -= G;
       }
@@ -1024,20 +1068,26 @@ __device__ void tri_singular(float *__restrict__ col, unsigned P, int H) {
   }
 }
 
-__global__ __launch_bounds__(TRI_THREADS) void k_col_tri(Ctx c, float *__restrict__ T, int guard) {
+// An image's blocks: ceil((W - 1) / 256) blocks of one column k = 1 .. W - 1 per thread, then one
+// block whose first wave solves column 0.  Batches of TRI_U rows end before row H - 1 except the
+// last; in a batch where every lane's t has fallen below 2^-60, 1 / d_i = e^-theta exactly in
+// double and the division is skipped.
+__global__ __launch_bounds__(TRI_THREADS, 4) void k_col_tri(Ctx c, float *__restrict__ T, int guard) {
   const int img = find_img(c.geo.coltri, c.n, blockIdx.x);
   const ImgDesc d = c.img[img];
   if (guard_off(&c.st[img], guard)) return;
-  const int k = (int)(blockIdx.x - c.geo.coltri[img]) * TRI_THREADS + (int)threadIdx.x;
-  const int H = d.H;
-  if (k >= d.W) return;
-  float *__restrict__ col = T + d.off + k;
+  const int bi = (int)(blockIdx.x - c.geo.coltri[img]);
+  const int H = d.H, W = d.W;
+  float *__restrict__ base = T + d.off;
   const unsigned P = d.P;
-  if (k == 0) {
-    tri_singular(col, P, H);
+  if (bi == (W - 1 + TRI_THREADS - 1) / TRI_THREADS) {
+    if (threadIdx.x < 64) tri_singular(base, P, H, (int)threadIdx.x);
     return;
   }
-  const double sn = sin(M_PI * (double)k / (2.0 * d.W));
+  const int j = bi * TRI_THREADS + (int)threadIdx.x;
+  if (j >= W - 1) return;
+  const unsigned k = (unsigned)(j + 1);
+  const double sn = sin(M_PI * (double)k / (2.0 * W));
   const double mu = 4.0 * sn * sn;
   const double ch = 1.0 + 0.5 * mu, sh = sqrt(mu * (1.0 + 0.25 * mu));
   TriPivots tp;
@@ -1048,48 +1098,98 @@ __global__ __launch_bounds__(TRI_THREADS) void k_col_tri(Ctx c, float *__restric
   tp.theta = log1p(ep1);
   // 1 - e^-theta t_{H-1} = 1 - e^-2H theta
   tp.inv_last = (1.0 + tp.t_at(H - 1)) / (ep1 * -expm1(-2.0 * H * tp.theta));
-  // forward sweep; the next batch's loads are issued before the current batch's chain
+  const double em = tp.em;
+  constexpr double TSMALL = 0x1p-60;
+  const int nfull = (H - 1) / TRI_U;  // full batches, all rows before H - 1
+  const int il = nfull * TRI_U;        // the last batch: rows il .. H - 1
+  // three batch buffers: the loads of the batch two ahead are in flight during each batch
+  float A[TRI_U], B[TRI_U], C[TRI_U];
+  // forward: w_i = (c_i + w_{i-1}) / d_i
   double w = 0.0;
-  float v[TRI_U], nv[TRI_U];
-#pragma unroll
-  for (int u = 0; u < TRI_U; ++u) v[u] = col[(size_t)min(u, H - 1) * P];
-  for (int i0 = 0; i0 < H; i0 += TRI_U) {
-    const int n0 = i0 + TRI_U;
-#pragma unroll
-    for (int u = 0; u < TRI_U; ++u) nv[u] = col[(size_t)min(n0 + u, H - 1) * P];
+  auto fwd = [&](int b, float (&cur)[TRI_U], float (&pre)[TRI_U]) {
+    const int i0 = b * TRI_U;
+    tri_load(pre, base, k, P, i0 + 2 * TRI_U, H);
     double t = tp.t_at(i0);
+    float *__restrict__ row = base + (size_t)i0 * P;
+    if (__ballot(t > TSMALL)) {
+#pragma unroll
+      for (int u = 0; u < TRI_U; ++u) {
+        const double iv = tp.inv(t);
+        w = fma(iv, w, iv * (double)cur[u]);
+        uniform_ptr(row + (size_t)u * P)[k] = (float)w;
+        t *= tp.q;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < TRI_U; ++u) {
+        w = fma(em, w, em * (double)cur[u]);
+        uniform_ptr(row + (size_t)u * P)[k] = (float)w;
+      }
+    }
+  };
+  tri_load(A, base, k, P, 0, H);
+  tri_load(B, base, k, P, TRI_U, H);
+  for (int b = 0; b < nfull; b += 3) {
+    fwd(b, A, C);
+    if (b + 1 < nfull) fwd(b + 1, B, A);
+    if (b + 2 < nfull) fwd(b + 2, C, B);
+  }
+  tri_load(A, base, k, P, il, H);
+  {
+    double t = tp.t_at(il);
 #pragma unroll
     for (int u = 0; u < TRI_U; ++u) {
-      const int i = i0 + u;
+      const int i = il + u;
       if (i < H) {
-        w = tp.inv(t, i == H - 1) * ((double)v[u] + w);
-        col[(size_t)i * P] = (float)w;
+        const double iv = i == H - 1 ? tp.inv_last : tp.inv(t);
+        w = fma(iv, w, iv * (double)A[u]);
+        uniform_ptr(base + (size_t)i * P)[k] = (float)w;
       }
       t *= tp.q;
     }
-#pragma unroll
-    for (int u = 0; u < TRI_U; ++u) v[u] = nv[u];
   }
-  // backward sweep over batches [i1 - TRI_U, i1), from the bottom
+  // backward: x_i = w_i + x_{i+1} / d_i, the last batch first
   double x = 0.0;
+  tri_load(A, base, k, P, il, H);
+  tri_load(B, base, k, P, il - TRI_U, H);
+  tri_load(C, base, k, P, il - 2 * TRI_U, H);
+  {
+    double t = tp.t_at(il + TRI_U - 1);
 #pragma unroll
-  for (int u = 0; u < TRI_U; ++u) v[u] = col[(size_t)max(H - 1 - u, 0) * P];
-  for (int i1 = H; i1 > 0; i1 -= TRI_U) {
-    const int n1 = i1 - TRI_U;
-#pragma unroll
-    for (int u = 0; u < TRI_U; ++u) nv[u] = col[(size_t)max(n1 - 1 - u, 0) * P];
-    double t = tp.t_at(i1 - 1);
-#pragma unroll
-    for (int u = 0; u < TRI_U; ++u) {
-      const int i = i1 - 1 - u;
-      if (i >= 0) {
-        x = (double)v[u] + tp.inv(t, i == H - 1) * x;
-        col[(size_t)i * P] = (float)x;
+    for (int u = TRI_U - 1; u >= 0; --u) {
+      const int i = il + u;
+      if (i < H) {
+        const double iv = i == H - 1 ? tp.inv_last : tp.inv(t);
+        x = fma(iv, x, (double)A[u]);
+        uniform_ptr(base + (size_t)i * P)[k] = (float)x;
       }
       t *= tp.eq2;
     }
+  }
+  auto bwd = [&](int b, float (&cur)[TRI_U], float (&pre)[TRI_U]) {
+    const int i0 = b * TRI_U;
+    tri_load(pre, base, k, P, i0 - 2 * TRI_U, H);
+    double t = tp.t_at(i0 + TRI_U - 1);
+    float *__restrict__ row = base + (size_t)i0 * P;
+    if (__ballot(tp.t_at(i0) > TSMALL)) {  // t grows towards row i0
 #pragma unroll
-    for (int u = 0; u < TRI_U; ++u) v[u] = nv[u];
+      for (int u = TRI_U - 1; u >= 0; --u) {
+        x = fma(tp.inv(t), x, (double)cur[u]);
+        uniform_ptr(row + (size_t)u * P)[k] = (float)x;
+        t *= tp.eq2;
+      }
+    } else {
+#pragma unroll
+      for (int u = TRI_U - 1; u >= 0; --u) {
+        x = fma(em, x, (double)cur[u]);
+        uniform_ptr(row + (size_t)u * P)[k] = (float)x;
+      }
+    }
+  };
+  for (int b = nfull - 1; b >= 0; b -= 3) {
+    bwd(b, B, A);
+    if (b - 1 >= 0) bwd(b - 1, C, B);
+    if (b - 2 >= 0) bwd(b - 2, A, C);
   }
 }
 

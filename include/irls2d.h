@@ -170,15 +170,16 @@ int irls2d_plan_reset(void);
  * whose column length, at least 256, splits into a power-of-two pair N1 N2 (N1 16 .. 128, N2
  * 16 .. 64: heights 256 .. 8192) run the three four-step slab passes with register-stage kernels on
  * the row-major field (no transposes; one launch per split in a batch); images whose column
- * transform would be Bluestein (a column length with a prime factor above 5) replace the column
- * DCT pair by the equivalent tridiagonal solves (L_H + mu_k I) x = c_k, in float64, on the
- * row-major field; the other images transpose their rows, transform and transpose back;
- * 1 = always transpose; 2 = column DCTs on the row-major field; 3 = slab passes for every image
- * whose column length allows them (2^a 3^b 5^c with such a split, from 4), the automatic choice
- * for the rest; 4 = tridiagonal solves for every image with H <= 12288.  Other columns longer than
- * 12288 transpose and run the two-pass long transform.  Since every choice is per image, an
- * image's result is the same alone and in any batch (under one mode and one plan set).  Drops
- * cached layouts and hipGraphs.  core_amd.autotune measures the strategies.
+ * transform would be Bluestein (a column length with a prime factor above 5) or a two-pass long
+ * transform (H above 12288) replace the column DCT pair by the equivalent tridiagonal solves
+ * (L_H + mu_k I) x = c_k, in float64, on the row-major field; the other images transpose their
+ * rows, transform and transpose back;
+ * 1 = transpose (two-pass long transforms above 12288); 2 = column DCTs on the row-major field
+ * (long ones transposed); 3 = slab passes for every image whose column length allows them
+ * (2^a 3^b 5^c with such a split, from 4), the automatic choice for the rest; 4 = tridiagonal
+ * solves for every image.  Since every choice is per image, an image's result is the same alone
+ * and in any batch (under one mode and one plan set).  Drops cached layouts and hipGraphs.
+ * core_amd.autotune measures the strategies.
  */
 int irls2d_set_column_mode(int mode);
 

@@ -138,12 +138,27 @@ PRECOND_SHAPES = [(64, 48), (127, 211), (1, 1000), (16, 12500), (12800, 8), (9, 
                   (12, 20011), (13001, 9), (16, 24989), (3, 12289), (12289, 1), (20011, 13001)]
 
 
-def test_preconditioner_matches_scipy_fp64(hip):
+@pytest.mark.parametrize("mode", ["auto", "transpose"])
+def test_preconditioner_matches_scipy_fp64(hip, mode):
     """The DCT engine alone (LDS-resident, Bluestein and four-step lengths, and four-step
     Bluestein for long prime sides, 20011 x 13001 both ways) against scipy in FP64:
-    z = idctn(-dctn(r) / lambda), DC = 0, lambda = 2cos(pi i/H) + 2cos(pi j/W) - 4."""
+    z = idctn(-dctn(r) / lambda), DC = 0, lambda = 2cos(pi i/H) + 2cos(pi j/W) - 4.  Automatic
+    mode solves long and Bluestein columns as tridiagonal systems; transpose mode runs them as
+    column DCTs on the transposed field (two-pass four-step above 12288)."""
     import scipy.fft as sf
     import torch
+    from core_amd import _lib
+    from core_amd.unwrap2d import Workspace, image_view, pack, precondition_device
+    _lib.set_column_mode(_lib.COLUMN_TRANSPOSE if mode == "transpose" else _lib.COLUMN_AUTO)
+    try:
+        _precondition_check(hip)
+    finally:
+        _lib.set_column_mode(_lib.COLUMN_AUTO)
+
+
+def _precondition_check(hip):
+    import scipy.fft as sf
+    from core_amd import _lib
     from core_amd.unwrap2d import Workspace, image_view, pack, precondition_device
     rng = np.random.default_rng(5)
     rs = []
@@ -151,8 +166,8 @@ def test_preconditioner_matches_scipy_fp64(hip):
         y, x = np.mgrid[0:h, 0:w]
         smooth = np.sin(2 * np.pi * x / max(w, 2) * 3) + np.cos(2 * np.pi * y / max(h, 2) * 2)
         rs.append((rng.normal(size=(h, w)) + 5 * smooth).astype(np.float32))
-    print(f"preconditioner {PRECOND_SHAPES}", flush=True)
-    ws = Workspace.get(PRECOND_SHAPES, hip)
+    print(f"preconditioner {PRECOND_SHAPES}: {_lib.describe_columns(PRECOND_SHAPES)}", flush=True)
+    ws = Workspace(PRECOND_SHAPES, hip)
     z = precondition_device(pack(rs, ws, hip), PRECOND_SHAPES, workspace=ws)
     zh = z.cpu().numpy()
     for k, ((h, w), r) in enumerate(zip(PRECOND_SHAPES, rs)):
@@ -167,6 +182,36 @@ def test_preconditioner_matches_scipy_fp64(hip):
         print(f"  {h}x{w}: max rel err {err:.2e}", flush=True)
         assert err < 2e-5, (h, w, err)
     ws.release()
+
+
+@pytest.mark.parametrize("H,W,seed", [(12800, 20, 32), (13001, 12, 35)])
+def test_long_columns_transposed_parity(hip, H, W, seed):
+    """Long columns as two-pass column DCTs on the transposed field (mode 1; the automatic mode
+    solves them as tridiagonal systems) against the oracles, with the bar of the long-side parity
+    test (these tall grids are ill-conditioned: float32 paths spread by up to ~2e-3 rad)."""
+    from core_amd import _lib, irls_unwrap_2d
+    from oracle import irls_fp64, irls_torch32
+    psi, coh = _case(H, W, seed)
+    print(f"long columns {H}x{W}: transposed two-pass DCTs", flush=True)
+    try:
+        _lib.set_column_mode(_lib.COLUMN_TRANSPOSE)
+        assert _lib.describe_columns([(H, W)]) == ["long"]
+        u_l, cc_l = irls_unwrap_2d(psi, coh, device='cuda', **ITERS)
+    finally:
+        _lib.set_column_mode(_lib.COLUMN_AUTO)
+    u64, cc64 = irls_fp64.irls_unwrap_2d(psi, coh, **ITERS)
+    u32, _ = irls_torch32.irls_unwrap_2d(psi, coh, **ITERS)
+    envelope = _rms(u32, u64, psi)
+    rng = np.random.default_rng(seed)
+    for _ in range(6):
+        psi_p = (psi + rng.uniform(-1e-6, 1e-6, size=psi.shape)).astype(np.float32)
+        up, _ = irls_torch32.irls_unwrap_2d(psi_p, coh, **ITERS)
+        envelope = max(envelope, _rms(up, u64, psi))
+    rms = _rms(u_l, u64, psi)
+    print(f"  rms_vs_fp64={rms:.3e} fp32_envelope={envelope:.3e} k_agree={_kagree(u_l, u32, psi):.6f}", flush=True)
+    assert rms <= max(RMS_TOL, 2 * envelope)
+    assert _kagree(u_l, u32, psi) >= K_AGREE
+    assert np.array_equal(cc_l, cc64)
 
 
 def test_long_rows_and_slab_columns_preconditioner(hip):

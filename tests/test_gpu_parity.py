@@ -222,11 +222,12 @@ def test_column_modes_agree(hip, H, W, seed):
         ref = outs[_lib.COLUMN_TRANSPOSE]
         # direct: the same arithmetic reordered; tridiagonal: the same operator with other rounding
         # (float64 solves in place of the float32 column DCTs), amplified along the IRLS trajectory
-        for mode, max_bar in ((_lib.COLUMN_DIRECT, 1e-4), (_lib.COLUMN_TRIDIAG, 2e-3)):
+        # of these fault-line grids (both within the FP64 bar above)
+        for mode, max_bar, rms_bar in ((_lib.COLUMN_DIRECT, 1e-4, 1e-5), (_lib.COLUMN_TRIDIAG, 1e-2, 5e-4)):
             d = np.abs(ref[0][v] - outs[mode][0][v])
             print(f"  mode {mode} vs transpose: max {d.max():.1e} RMS {_metrics(ref[0], outs[mode][0], psi):.1e}",
                   flush=True)
-            assert d.max() < max_bar and _metrics(ref[0], outs[mode][0], psi) < 2e-4
+            assert d.max() < max_bar and _metrics(ref[0], outs[mode][0], psi) < rms_bar
             assert np.array_equal(ref[1], outs[mode][1])
     finally:
         _lib.set_column_mode(_lib.COLUMN_AUTO)

@@ -95,8 +95,17 @@ def test_workspace_size_long_sides(lib):
     short = lib.workspace_size([(24, 12000)])
     long_ = lib.workspace_size([(24, 12500)])
     assert long_ > short * 12500 / 12000 + 12 * 12500 * 8 * 0.99
+    # long rows and columns: 15 float32 fields and the long-row scratch (4 B/px); the columns run as
+    # tridiagonal solves in place (no transposed copy); with transposed two-pass columns the
+    # transposed copy and the column scratch come on top
+    px = 12500 * 12800
     both = lib.workspace_size([(12500, 12800)])
-    assert both >= 17 * 12500 * 12800 * 4
+    assert 16 * px * 4 <= both < 17 * px * 4
+    try:
+        lib.set_column_mode(lib.COLUMN_TRANSPOSE)
+        assert lib.workspace_size([(12500, 12800)]) >= 17 * px * 4
+    finally:
+        lib.set_column_mode(lib.COLUMN_AUTO)
 
 
 def test_argument_errors_without_gpu(lib):
@@ -229,8 +238,8 @@ def test_autotune_candidates_are_valid(lib):
 def test_column_strategies(lib):
     """Which column pass each image gets, from its own shape.  Automatic mode: the register-stage
     slab kernels for heights from 256 on whose split is a power-of-two pair (one launch per split);
-    the transposed pass for other heights (the batched-LDS slab kernels are slower), the column
-    kernel on the row-major field for Bluestein heights, the four-step long path above 12288.
+    the transposed pass for other heights (the batched-LDS slab kernels are slower), tridiagonal
+    solves for Bluestein heights and above 12288.
     Slab mode: every 5-smooth height within the block limits (register kernels where the split
     allows them)."""
     d = lib.describe_columns
@@ -245,27 +254,26 @@ def test_column_strategies(lib):
         assert d([(6000, 6000)]) == ["transpose"]
         assert d([(96, 130)]) == ["transpose"]                     # below the slab threshold
         assert d([(127, 211)]) == ["tridiag"]                      # Bluestein columns
-        assert d([(12800, 8)]) == ["long"]                         # long sides: transposed two-pass path
+        assert d([(12800, 8)]) == ["tridiag"]                      # long columns: tridiagonal solves
         assert d([(1000, 12500)]) == ["transpose"]
-        assert d([(8192, 70), (127, 211), (12800, 8)]) == ["slab_register", "tridiag", "long"]
+        assert d([(8192, 70), (127, 211), (12800, 8)]) == ["slab_register", "tridiag", "tridiag"]
         lib.set_column_mode(lib.COLUMN_SLAB)
         assert d([(64, 48), (96, 130)]) == ["slab", "slab"]
         assert d([(8192, 70), (1000, 7)]) == ["slab_register", "slab"]  # 1000 = 40 x 25: batched LDS
         assert d([(12800, 8)]) == ["slab"]                         # 12800 = 160 x 80
         assert d([(40000, 64)]) == ["slab"]                        # 40000 = 250 x 160
-        assert d([(177147, 8)]) == ["long"]                        # 3^11: no split within the block limits
+        assert d([(177147, 8)]) == ["tridiag"]                     # 3^11: no split within the block limits
         assert d([(1000, 12500)]) == ["slab"]                      # long rows write T in slab order too
         assert d([(127, 211)]) == ["tridiag"]                      # no slab split: automatic choice
         lib.set_column_mode(lib.COLUMN_TRANSPOSE)
-        assert d([(8192, 8192)]) == ["transpose"]
+        assert d([(8192, 8192), (12800, 8), (13001, 9)]) == ["transpose", "long", "long"]
         lib.set_column_mode(lib.COLUMN_DIRECT)
         assert d([(127, 211), (8192, 70), (12800, 8)]) == ["direct", "direct", "long"]
         lib.set_column_mode(lib.COLUMN_TRIDIAG)
         assert d([(127, 211), (8192, 70), (96, 130), (12800, 8)]) == ["tridiag"] * 4
         lib.set_column_mode(lib.COLUMN_AUTO)
-        # columns with a prime factor above 5: tridiagonal solves at any height; 5-smooth long
-        # columns keep the two-pass transforms
-        assert d([(13001, 9), (7001, 9), (12800, 8), (9, 7001)]) == ["tridiag", "tridiag", "long", "transpose"]
+        # columns with a prime factor above 5 or above 12288: tridiagonal solves
+        assert d([(13001, 9), (7001, 9), (12800, 8), (9, 7001)]) == ["tridiag", "tridiag", "tridiag", "transpose"]
     finally:
         lib.set_column_mode(lib.COLUMN_AUTO)
 
