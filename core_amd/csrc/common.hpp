@@ -159,7 +159,10 @@ struct ImgDesc {
   int rowpart0;    // first block-partial slot of the image's row-pair blocks (short rows)
   int coldirect;   // 1: the column DCTs read column pairs straight from the row-major field
   int lbpart;      // Bluestein long rows: first slot of the pack / post block partials (partials2)
-  int coltri;      // 1: the column half of the preconditioner runs as tridiagonal solves (k_col_tri)
+  int coltri;      // 1: the column half of the preconditioner runs as tridiagonal solves (k_col_trc)
+  int tri_nb;      // tridiagonal solves: bands of TRC_CH chunks the columns are cut into (1: one pass)
+  long long tri_off;  // double offset of the image's band carries (6 x tri_nb x W) when tri_nb > 1
+  long long tri_dc;   // double offset of the image's column-0 input (H floats) for the singular solve
 };
 
 // Per-image solver state.  Written by k_state_init and k_finalize (one block per image), read by
@@ -201,7 +204,9 @@ struct Geo {
   const int *slab1, *slab2;                  // slab column passes: K1 / K3, and K2
   // Bluestein long lines: pack / post, fft1 / fft3 and fft2 blocks, rows then columns
   const int *browP, *brow1, *brow2, *bcolP, *bcol1, *bcol2;
-  const int *coltri;                         // tridiagonal column solves: 256 columns per block
+  // tridiagonal column solves, TRC_COLS columns x one band per block: the final solves per chunk
+  // length (4, 6, 8, 10, 12 rows), the band transfers, the band carry chains, column 0
+  const int *coltri[5], *tritr, *tricarry, *tridc;
 };
 
 struct Ctx {
@@ -215,6 +220,7 @@ struct Ctx {
   double *partials;
   double *partials2;  // block partials of the long-line passes (their own block numbering)
   float2 *S;          // long-transform scratch arena
+  double *tri;        // band carries of the tridiagonal column solves
   Geo geo;
   Fields f;
   unsigned char *ws_base;  // workspace base (host side: class tables are offsets from it)

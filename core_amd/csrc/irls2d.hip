@@ -469,7 +469,7 @@ static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Fixed block-start tables of a layout; the class tables (LaunchClass::geo / geo2) follow them.
 enum { G_TILE = 0, G_TRANS, G_ROWL1, G_ROWL2, G_COLL1, G_COLL2, G_BROWP, G_BROW1, G_BROW2, G_BCOLP, G_BCOL1,
-       G_BCOL2, G_TRI, G_FIXED };
+       G_BCOL2, G_TRI, G_TRI_END = G_TRI + TRC_NL - 1, G_TRIT, G_TRIC, G_TRIDC, G_FIXED };
 
 // Images whose transform pass shares one kernel instantiation: one launch per class and pass.
 // Every choice is a function of the image's own shape (and the global mode / plan overrides), so an
@@ -501,9 +501,10 @@ struct Layout {
   std::vector<double> dtab;
   int max_blocks = 0, max_blocks2 = 0;
   int64_t totalS = 0;  // float2 elements of the long-transform scratch arena
+  int64_t totalTri = 0;  // doubles of the band carries of the tridiagonal column solves
   size_t long_lds = 0;
   // byte offsets inside the workspace
-  size_t o_fields = 0, o_Tt = 0, o_S = 0, o_valid = 0, o_meta = 0, o_desc = 0, o_state = 0, o_plans = 0,
+  size_t o_fields = 0, o_Tt = 0, o_S = 0, o_tri = 0, o_valid = 0, o_meta = 0, o_desc = 0, o_state = 0, o_plans = 0,
          o_lplans = 0, o_splans = 0, o_tab = 0, o_dtab = 0, o_partials = 0, o_partials2 = 0, bytes = 0,
          meta_bytes = 0;
   std::vector<size_t> o_geo;
@@ -739,7 +740,26 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     adds[i].push_back({G_TILE, ((int64_t)(h + TILE_H - 1) / TILE_H) * ((w + TILE_W - 1) / TILE_W)});
     if (!slab && !d.coldirect && !d.coltri)
       adds[i].push_back({G_TRANS, ((int64_t)(h + TT - 1) / TT) * ((w + TT - 1) / TT)});
-    if (d.coltri) adds[i].push_back({G_TRI, (w - 1 + TRI_THREADS - 1) / TRI_THREADS + 1});
+    if (d.coltri) {
+      // one band of TRC_CH chunks of the shortest chunk length that covers the column; taller
+      // columns in bands of TRC_CH x TRC_L rows with carries (transfer, chain and final launches)
+      const int64_t ngrp = (w + TRC_COLS - 1) / TRC_COLS;
+      d.tri_nb = (int)((h + TRC_CH * TRC_L - 1) / (TRC_CH * TRC_L));
+      const int64_t lmin = (h + TRC_CH - 1) / TRC_CH;
+      const int cls = d.tri_nb > 1 ? TRC_NL - 1
+                                   : (int)std::max<int64_t>(0, (lmin - TRC_L_MIN + TRC_L_STEP - 1) / TRC_L_STEP);
+      // column 0's input (H floats), then the band carries
+      d.tri_dc = Lo.totalTri;
+      Lo.totalTri += (h + 1) / 2;
+      d.tri_off = Lo.totalTri;
+      adds[i].push_back({G_TRI + cls, ngrp * d.tri_nb});
+      adds[i].push_back({G_TRIDC, 1});
+      if (d.tri_nb > 1) {
+        Lo.totalTri += 6 * (int64_t)d.tri_nb * w;
+        adds[i].push_back({G_TRIT, ngrp * d.tri_nb});
+        adds[i].push_back({G_TRIC, (w - 1 + 255) / 256});
+      }
+    }
     if (!lw) {
       const DctPlan &R = Lo.plans[d.rowplan];
       LaunchClass &C = Lo.rowcls[row_class(R.ept)];
@@ -809,6 +829,8 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
   o = align_up(o + (size_t)(Lo.totalT + FIELD_PAD) * sizeof(float), 256);
   Lo.o_S = o;
   o = align_up(o + (size_t)Lo.totalS * sizeof(float2), 256);
+  Lo.o_tri = o;
+  o = align_up(o + (size_t)Lo.totalTri * sizeof(double), 256);
   Lo.o_valid = o;
   o = align_up(o + (size_t)Lo.total, 256);
   Lo.o_state = o;
@@ -929,7 +951,11 @@ static Ctx make_ctx(const Layout &Lo, unsigned char *ws, float *u, const Params 
   c.geo.bcolP = (const int *)(ws + Lo.o_geo[G_BCOLP]);
   c.geo.bcol1 = (const int *)(ws + Lo.o_geo[G_BCOL1]);
   c.geo.bcol2 = (const int *)(ws + Lo.o_geo[G_BCOL2]);
-  c.geo.coltri = (const int *)(ws + Lo.o_geo[G_TRI]);
+  for (int q = 0; q < TRC_NL; ++q) c.geo.coltri[q] = (const int *)(ws + Lo.o_geo[G_TRI + q]);
+  c.geo.tritr = (const int *)(ws + Lo.o_geo[G_TRIT]);
+  c.geo.tricarry = (const int *)(ws + Lo.o_geo[G_TRIC]);
+  c.geo.tridc = (const int *)(ws + Lo.o_geo[G_TRIDC]);
+  c.tri = (double *)(ws + Lo.o_tri);
   // class tables: set per launch (class_ctx)
   c.geo.rowpair = c.geo.colpair = c.geo.slab1 = c.geo.slab2 = nullptr;
   c.ws_base = ws;
@@ -1116,8 +1142,25 @@ static void col_pass(const Layout &Lo, const Ctx &c, int guard, hipStream_t s) {
   }
   lb_dct2(Lo, c, 1, false, c.f.Tt, c.f.Tt, nullptr, guard, s);
   lb_dct3(Lo, c, 1, c.f.Tt, c.f.Tt, 0, guard, s);
-  if (Lo.geo[G_TRI].back() > 0)
-    LAUNCH("k_col_tri", k_col_tri, dim3(Lo.geo[G_TRI].back()), dim3(TRI_THREADS), 0, s, c, c.f.T, guard);
+  if (Lo.geo[G_TRIT].back() > 0) {
+    LAUNCH("k_col_trc_band", (k_col_trc<TRC_L, true>), dim3(Lo.geo[G_TRIT].back()), dim3(TRC_THREADS),
+           trc_lds_bytes(true), s, c,
+           c.f.T, guard);
+    LAUNCH("k_col_trc_carry", k_col_trc_carry, dim3(Lo.geo[G_TRIC].back()), dim3(256), 0, s, c, guard);
+  }
+  auto tri = [&](int q, auto kern) {
+    if (Lo.geo[G_TRI + q].back() > 0)
+      LAUNCH("k_col_trc", kern, dim3(Lo.geo[G_TRI + q].back()), dim3(TRC_THREADS), trc_lds_bytes(false), s, c,
+             c.f.T, guard);
+  };
+  static_assert(TRC_NL == 5, "one launch per chunk length");
+  tri(0, k_col_trc<TRC_L_MIN, false>);
+  tri(1, k_col_trc<TRC_L_MIN + TRC_L_STEP, false>);
+  tri(2, k_col_trc<TRC_L_MIN + 2 * TRC_L_STEP, false>);
+  tri(3, k_col_trc<TRC_L_MIN + 3 * TRC_L_STEP, false>);
+  tri(4, k_col_trc<TRC_L, false>);
+  if (Lo.geo[G_TRIDC].back() > 0)
+    LAUNCH("k_col_trc_dc", k_col_trc_dc, dim3(Lo.geo[G_TRIDC].back()), dim3(64), 0, s, c, c.f.T, guard);
   for (const LaunchClass &C : Lo.colcls) {
     const int nb = Lo.geo[C.geo].back();
     if (nb == 0) continue;
@@ -1306,6 +1349,12 @@ static int set_attrs() {
   const int slds = (TW_LDS + SLAB_E * SLAB_MAX_THREADS + 2 * SLAB_MAX_THREADS) * (int)sizeof(float2);
   const void *sfns[] = {(const void *)k_slab1<SLAB_E>, (const void *)k_slab2<SLAB_E>, (const void *)k_slab3<SLAB_E>};
   for (const void *f : sfns) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, slds));
+  const void *tfns[] = {(const void *)k_col_trc<TRC_L_MIN, false>, (const void *)k_col_trc<TRC_L_MIN + TRC_L_STEP, false>,
+                        (const void *)k_col_trc<TRC_L_MIN + 2 * TRC_L_STEP, false>,
+                        (const void *)k_col_trc<TRC_L_MIN + 3 * TRC_L_STEP, false>, (const void *)k_col_trc<TRC_L, false>,
+                        (const void *)k_col_trc<TRC_L, true>};
+  for (const void *f : tfns)
+    HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)trc_lds_bytes(true)));
   if (dev >= 0 && dev < 64) g_attr_done[dev] = true;
   return IRLS2D_OK;
 }

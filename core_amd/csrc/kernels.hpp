@@ -960,24 +960,52 @@ __global__ __launch_bounds__(fft_threads_max<E>()) __attribute__((amdgpu_waves_p
 // nu_l = 4 sin^2(pi l / 2H): the DCT-II diagonalises the second-difference matrix with
 // reflecting ends).  After the row DCT-II, column k of the row spectrum holds c_k and the column
 // DCT pair with the division by -lambda equals the solve (L_H + mu_k I) x = c_k, L_H =
-// tridiag(-1, 2, -1) with corner entries 1: the same operator, applied without transforms.  k_col_tri
-// solves it by LU in float64, one thread per column, in place on the row-major spectrum field
-// (coalesced rows of 256 columns); column 0 (mu_0 = 0, singular) takes the pseudo-inverse with the
-// mean removed from input and output, which is the DC zeroing.
+// tridiag(-1, 2, -1) with corner entries 1: the same operator, applied without transforms.  It is
+// solved by LU in float64, in place on the row-major spectrum field; column 0 (mu_0 = 0, singular)
+// takes the pseudo-inverse with the mean removed from input and output, which is the DC zeroing.
 //
 // The LU pivots of L_H + mu I have a closed form: with cosh(theta) = 1 + mu / 2,
-//   1 / d_i = cosh((i + 1/2) theta) / cosh((i + 3/2) theta) = e^-theta (1 + t_i) / (1 + e^-2theta t_i)
+//   a_i = 1 / d_i = cosh((i + 1/2) theta) / cosh((i + 3/2) theta) = e^-theta (1 + t_i) / (1 + e^-2theta t_i)
 //   for i < H - 1, t_i = e^-(2i+1)theta, and the last (reflecting) row
 //   1 / d_{H-1} = (1 + t_{H-1}) / ((e^theta - 1)(1 - e^-2H theta))
 // (for H = 1 this is 1 / mu)
 // (from d_0 = 1 + mu and d_i = 2 + mu - 1 / d_{i-1} by the identity 2 cosh a cosh b = cosh(a + b) +
 // cosh(a - b)); no cancellation, unlike the recurrence, whose pivots lose digits for small mu.
-// Forward: w_i = (c_i + w_{i-1}) / d_i (the float32 w overwrites c); backward: x_i = w_i + x_{i+1} / d_i.
-// t_i is re-anchored with exp() at the start of every batch of TRI_U rows; a batch whose t starts
-// below 2^-60 in every lane takes 1 / d_i = e^-theta (the relative difference is below 2^-60).
+// Forward: w_i = a_i (c_i + w_{i-1}); backward: x_i = w_i + a_i x_{i+1}.  Once t_i < 2^-60 in every
+// lane of a wave, a_i = e^-theta to double precision and the closed form is skipped.
+//
+// Both recurrences are first-order linear, so a run of rows [s, e) maps its incoming carry
+// linearly: w_{e-1} = w0_{e-1} + (prod_{s<=j<e} a_j) w_{s-1} and x_s = x0_s + (prod a_j) x_e, with
+// w0 / x0 the results from a zero carry.  k_col_trc gives each thread four adjacent columns of a
+// chunk of L consecutive rows in registers (float4 loads and stores: a wave instruction moves eight
+// 128-byte row segments); a block covers TRC_COLS columns and a band of TRC_CH chunks.  A sweep
+// runs once for the chunk's transfer (end value and pivot product), the TRC_CH transfers of each
+// column are chained in LDS (4 chunks per thread, then an affine-map scan over 32 lanes), and the
+// sweep runs again from the true carry.  Forward and backward each take one such round; the field
+// is read and written once (8 B/px).  Columns taller than one band of TRC_CH x TRC_L rows are cut
+// into bands: k_col_trc<L, true> stores each band's transfers with zero outer carries (end w,
+// pivot product, x at the band's first row and that x's response to a unit incoming w),
+// k_col_trc_carry chains the bands of each column, and k_col_trc<L, false> solves every band from
+// its carries (12 B/px).  Column 0 runs through the same sweeps with meaningless values; the final
+// pass keeps its input, and k_col_trc_dc writes the singular solve over it.
 
-constexpr int TRI_THREADS = 256;
-constexpr int TRI_U = 16;
+constexpr int TRI_U = 16;  // rows per load batch of the singular column
+constexpr int TRC_COLS = 32;     // columns per block: 8 lanes of four columns
+constexpr int TRC_CH = 64;       // chunks per column and band
+constexpr int TRC_THREADS = TRC_COLS / 4 * TRC_CH;  // 512: two waves per SIMD, up to 256 VGPRs
+constexpr int TRC_SEG = TRC_CH / 4;                 // chaining: 4 chunks per thread, a TRC_SEG-lane scan
+constexpr int TRC_L = 24;        // rows per chunk of a band: bands of 1536 rows
+// one-band columns: chunks of TRC_L_MIN, TRC_L_MIN + TRC_L_STEP, ... TRC_L rows, the shortest that
+// covers the column in TRC_CH chunks (a kernel instantiation and a launch per length)
+constexpr int TRC_L_MIN = 8;
+constexpr int TRC_L_STEP = 4;
+constexpr int TRC_NL = (TRC_L - TRC_L_MIN) / TRC_L_STEP + 1;
+constexpr double TRI_TSMALL = 0x1p-60;
+// dynamic LDS bytes of k_col_trc: pivots (5 per column), then 2 (3 with the band transfers)
+// arrays of TRC_COLS x (TRC_CH + 1) doubles
+constexpr size_t trc_lds_bytes(bool transfer) {
+  return (5 * TRC_COLS + (transfer ? 3 : 2) * TRC_COLS * (TRC_CH + 1)) * sizeof(double);
+}
 
 struct TriPivots {
   double em, q, eq2, theta, inv_last;
@@ -993,22 +1021,8 @@ struct TriPivots {
   }
 };
 
-// Rows [i0, i0 + TRI_U) of column k (clamped to the first and last row) into registers.  Row bases
-// are uniform (scalar registers), the column is a per-lane 32-bit offset.
-template <typename F>
-__device__ __forceinline__ F *uniform_ptr(F *p) {
-  const unsigned long long a = (unsigned long long)p;
-  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a), hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
-  return (F *)(((unsigned long long)hi << 32) | lo);
-}
-
-__device__ __forceinline__ void tri_load(float (&v)[TRI_U], const float *__restrict__ base, unsigned k, unsigned P, int i0,
-                                         int H) {
-#pragma unroll
-  for (int u = 0; u < TRI_U; ++u) v[u] = uniform_ptr(base + (size_t)min(max(i0 + u, 0), H - 1) * P)[k];
-}
-
-// x = L_H^+ (c - mean c) with mean(x) = 0, for column 0, by one wave.  With m = mean c,
+// x = L_H^+ (c - mean c) with mean(x) = 0, for column 0 (input src, output column 0 of base), by one
+// wave.  With m = mean c,
 // G_i = sum_{j<=i} c_j and X_i = -sum_{j<i} G_j (X_0 = 0), the solution of x_{i+1} = x_i - g_i,
 // g_i = sum_{j<=i} (c_j - m), is x_i = X_i + m i (i + 1) / 2 - mean, where
 // sum_i X_i = -sum_j (H - 1 - j) G_j.  Lane l owns rows [l L, l L + L): one pass sums its rows
@@ -1024,7 +1038,7 @@ __device__ __forceinline__ double wave_scan_excl(double v, int lane) {
   return inc - v;
 }
 
-__device__ void tri_singular(float *__restrict__ base, unsigned P, int H, int lane) {
+__device__ void tri_singular(const float *__restrict__ src, float *__restrict__ base, unsigned P, int H, int lane) {
   const int L = (H + 63) / 64;
   const int r0 = min(lane * L, H), r1 = min(r0 + L, H);
   const int len = r1 - r0;
@@ -1032,7 +1046,7 @@ __device__ void tri_singular(float *__restrict__ base, unsigned P, int H, int la
   for (int i0 = r0; i0 < r1; i0 += TRI_U) {
     float v[TRI_U];
 #pragma unroll
-    for (int u = 0; u < TRI_U; ++u) v[u] = base[(size_t)min(i0 + u, H - 1) * P];
+    for (int u = 0; u < TRI_U; ++u) v[u] = src[min(i0 + u, H - 1)];
 #pragma unroll
     for (int u = 0; u < TRI_U; ++u) {
       const int i = i0 + u;
@@ -1055,7 +1069,7 @@ __device__ void tri_singular(float *__restrict__ base, unsigned P, int H, int la
   for (int i0 = r0; i0 < r1; i0 += TRI_U) {
     float v[TRI_U];
 #pragma unroll
-    for (int u = 0; u < TRI_U; ++u) v[u] = base[(size_t)min(i0 + u, H - 1) * P];
+    for (int u = 0; u < TRI_U; ++u) v[u] = src[min(i0 + u, H - 1)];
 #pragma unroll
     for (int u = 0; u < TRI_U; ++u) {
       const int i = i0 + u;
@@ -1068,25 +1082,16 @@ __device__ void tri_singular(float *__restrict__ base, unsigned P, int H, int la
   }
 }
 
-// An image's blocks: ceil((W - 1) / 256) blocks of one column k = 1 .. W - 1 per thread, then one
-// block whose first wave solves column 0.  Batches of TRI_U rows end before row H - 1 except the
-// last; in a batch where every lane's t has fallen below 2^-60, 1 / d_i = e^-theta exactly in
-// double and the division is skipped.
-__global__ __launch_bounds__(TRI_THREADS, 4) void k_col_tri(Ctx c, float *__restrict__ T, int guard) {
-  const int img = find_img(c.geo.coltri, c.n, blockIdx.x);
-  const ImgDesc d = c.img[img];
-  if (guard_off(&c.st[img], guard)) return;
-  const int bi = (int)(blockIdx.x - c.geo.coltri[img]);
-  const int H = d.H, W = d.W;
-  float *__restrict__ base = T + d.off;
-  const unsigned P = d.P;
-  if (bi == (W - 1 + TRI_THREADS - 1) / TRI_THREADS) {
-    if (threadIdx.x < 64) tri_singular(base, P, H, (int)threadIdx.x);
-    return;
-  }
-  const int j = bi * TRI_THREADS + (int)threadIdx.x;
-  if (j >= W - 1) return;
-  const unsigned k = (unsigned)(j + 1);
+// A pointer the compiler can keep in scalar registers (every lane of the wave holds the same one).
+template <typename F>
+__device__ __forceinline__ F *uniform_ptr(F *p) {
+  const unsigned long long a = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)a), hi = __builtin_amdgcn_readfirstlane((unsigned)(a >> 32));
+  return (F *)(((unsigned long long)hi << 32) | lo);
+}
+
+// Pivots of column k (mu_k = 4 sin^2(pi k / 2W)) for a column of H rows.
+__device__ __forceinline__ TriPivots tri_pivots(int k, int W, int H) {
   const double sn = sin(M_PI * (double)k / (2.0 * W));
   const double mu = 4.0 * sn * sn;
   const double ch = 1.0 + 0.5 * mu, sh = sqrt(mu * (1.0 + 0.25 * mu));
@@ -1098,98 +1103,356 @@ __global__ __launch_bounds__(TRI_THREADS, 4) void k_col_tri(Ctx c, float *__rest
   tp.theta = log1p(ep1);
   // 1 - e^-theta t_{H-1} = 1 - e^-2H theta
   tp.inv_last = (1.0 + tp.t_at(H - 1)) / (ep1 * -expm1(-2.0 * H * tp.theta));
-  const double em = tp.em;
-  constexpr double TSMALL = 0x1p-60;
-  const int nfull = (H - 1) / TRI_U;  // full batches, all rows before H - 1
-  const int il = nfull * TRI_U;        // the last batch: rows il .. H - 1
-  // three batch buffers: the loads of the batch two ahead are in flight during each batch
-  float A[TRI_U], B[TRI_U], C[TRI_U];
-  // forward: w_i = (c_i + w_{i-1}) / d_i
-  double w = 0.0;
-  auto fwd = [&](int b, float (&cur)[TRI_U], float (&pre)[TRI_U]) {
-    const int i0 = b * TRI_U;
-    tri_load(pre, base, k, P, i0 + 2 * TRI_U, H);
-    double t = tp.t_at(i0);
-    float *__restrict__ row = base + (size_t)i0 * P;
-    if (__ballot(t > TSMALL)) {
+  return tp;
+}
+
+// Inclusive scan of the affine maps x -> a x + b over groups of TRC_SEG lanes: lane l of a group
+// ends with the composition f_l o ... o f_0 of its own map and those of the lanes below it.
+__device__ __forceinline__ void affine_scan_seg(double &a, double &b, int l) {
 #pragma unroll
-      for (int u = 0; u < TRI_U; ++u) {
-        const double iv = tp.inv(t);
-        w = fma(iv, w, iv * (double)cur[u]);
-        uniform_ptr(row + (size_t)u * P)[k] = (float)w;
-        t *= tp.q;
-      }
-    } else {
-#pragma unroll
-      for (int u = 0; u < TRI_U; ++u) {
-        w = fma(em, w, em * (double)cur[u]);
-        uniform_ptr(row + (size_t)u * P)[k] = (float)w;
-      }
+  for (int dd = 1; dd < TRC_SEG; dd <<= 1) {
+    const double ap = __shfl_up(a, dd, TRC_SEG), bp = __shfl_up(b, dd, TRC_SEG);
+    if (l >= dd) {
+      b = fma(a, bp, b);
+      a *= ap;
     }
-  };
-  tri_load(A, base, k, P, 0, H);
-  tri_load(B, base, k, P, TRI_U, H);
-  for (int b = 0; b < nfull; b += 3) {
-    fwd(b, A, C);
-    if (b + 1 < nfull) fwd(b + 1, B, A);
-    if (b + 2 < nfull) fwd(b + 2, C, B);
   }
-  tri_load(A, base, k, P, il, H);
+}
+
+// Block of TRC_COLS columns 32 g .. 32 g + 31 and one band of TRC_CH chunks of L rows (the image's
+// blocks: g fastest, then the band).  Thread t holds columns 32 g + 4 (t mod 8) + i (i < 4) of
+// chunk t / 8.  Band carries, [6][nb][W] doubles from tri_off: the band's end w, pivot product,
+// first-row x and unit response (written by TRANSFER), its incoming w and x (written by
+// k_col_trc_carry).  Columns from W to the next multiple of 32 lie in the row padding (the pitch
+// is a multiple of 64): they are solved with meaningless values that no reader uses.
+template <int L, bool TRANSFER>
+__global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__restrict__ T, int guard) {
+  const int *start = TRANSFER ? c.geo.tritr : c.geo.coltri[(L - TRC_L_MIN) / TRC_L_STEP];
+  const int img = find_img(start, c.n, blockIdx.x);
+  const ImgDesc d = c.img[img];
+  if (guard_off(&c.st[img], guard)) return;
+  const int H = d.H, W = d.W;
+  const unsigned P = d.P;
+  float *__restrict__ base = T + d.off;
+  const int bi = (int)(blockIdx.x - start[img]);
+  const int ngrp = (W + TRC_COLS - 1) / TRC_COLS;
+  const int grp = bi % ngrp, band = bi / ngrp;
+  const int nb = d.tri_nb;
+  const int tid = (int)threadIdx.x, qd = tid & 7, ch = tid >> 3;
+  const int k0 = grp * TRC_COLS + 4 * qd;  // the thread's first column
+  const int r0 = band * TRC_CH * L, rend = min(H, r0 + TRC_CH * L);
+  const int s = r0 + ch * L;
+  const int len = max(0, min(L, rend - s));
+  const int lastu = max(len - 1, 0);
+  const int sc = min(s, H - 1);
+  double *__restrict__ tri = c.tri + d.tri_off;
+  const size_t bandW = (size_t)nb * W;
+
+  // pivots of the block's columns, computed once per column by the first wave while the other
+  // waves' loads are in flight
+  // LDS (dynamic: the block needs more than 64 KiB, which static __shared__ arrays did not give
+  // correctly on this platform): pivots of the block's columns, then the chained transfers per
+  // column and chunk (end w or x, pivot product, and for TRANSFER the unit response)
+  extern __shared__ double trc_lds[];
+  double(*sPiv)[TRC_COLS] = reinterpret_cast<double(*)[TRC_COLS]>(trc_lds);
+  double(*sW)[TRC_CH + 1] = reinterpret_cast<double(*)[TRC_CH + 1]>(trc_lds + 5 * TRC_COLS);
+  double(*sP)[TRC_CH + 1] = sW + TRC_COLS;
+  double(*sR)[TRC_CH + 1] = sP + TRC_COLS;  // TRANSFER only
+  if (tid < TRC_COLS) {
+    const TriPivots t = tri_pivots(min(grp * TRC_COLS + tid, W - 1), W, H);
+    sPiv[0][tid] = t.em;
+    sPiv[1][tid] = t.q;
+    sPiv[2][tid] = t.eq2;
+    sPiv[3][tid] = t.inv_last;
+    sPiv[4][tid] = t.theta;
+  }
+
+  // the chunk's rows (float4 of the thread's four columns; the row pitch is a multiple of 64 and
+  // k0 of 4, so every access is 16-byte aligned and inside the row).  A short chunk re-reads its
+  // last row, an empty one row H - 1.
+  float4 *__restrict__ rowp = reinterpret_cast<float4 *>(base + (size_t)sc * P + k0);
+  const size_t P4 = P / 4;
+  const bool full = !__ballot(!(len == L && s + L < H));
+  float v[L][4];
+#pragma unroll
+  for (int u = 0; u < L; ++u) {
+    const float4 q = rowp[(size_t)min(u, lastu) * P4];
+    v[u][0] = q.x;
+    v[u][1] = q.y;
+    v[u][2] = q.z;
+    v[u][3] = q.w;
+  }
+  // column 0's input for k_col_trc_dc (the sweeps below overwrite it)
+  if (!TRANSFER && k0 == 0) {
+    float *__restrict__ dc = (float *)(c.tri + d.tri_dc);
+#pragma unroll
+    for (int u = 0; u < L; ++u)
+      if (u < len) dc[s + u] = v[u][0];
+  }
+  __syncthreads();
+  // the pivots of the thread's four columns, copied to registers once (the sweeps read no LDS)
+  double em[4], pq[4], peq2[4], plast[4], pth[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    em[i] = sPiv[0][4 * qd + i];
+    pq[i] = sPiv[1][4 * qd + i];
+    peq2[i] = sPiv[2][4 * qd + i];
+    plast[i] = sPiv[3][4 * qd + i];
+    pth[i] = sPiv[4][4 * qd + i];
+  }
+  // every chunk of the wave complete, without row H - 1, and t_s <= 2^-60 in every column of
+  // every lane: a_i = e^-theta on the whole chunk, branch-free sweeps; otherwise every row is tested
+  bool cl = true;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) cl = cl && pth[i] * (2.0 * sc + 1.0) > 60.0 * M_LN2;
+  const bool conv = full && !__ballot(!cl);
+
+  // chaining layout: thread t takes column t / TRC_SEG and chunks 4 (t mod TRC_SEG) .. + 3
+  const int cc = tid / TRC_SEG, sg = tid % TRC_SEG;
+  const int kc = min(grp * TRC_COLS + cc, W - 1);
+  const bool actc = grp * TRC_COLS + cc < W && grp * TRC_COLS + cc > 0;
+
+  // The sweeps branch per wave (converged or general); every barrier and chaining step below is
+  // reached by all threads of the block.
   {
-    double t = tp.t_at(il);
+    double Pc[4] = {1.0, 1.0, 1.0, 1.0}, Rc[4] = {0.0, 0.0, 0.0, 0.0};
+    // The four sweeps each read what the previous one stored in v (never the same values twice, so
+    // the compiler keeps no conversions alive across sweeps):
+    //   F1  w0 from a zero carry (stored), with the chunk's pivot product and, for TRANSFER, R;
+    //   F3  w = w0 + (a_s ... a_u) w_in (stored);
+    //   B1  x0 = w + a x0 from a zero carry (stored), x0 at the chunk's first row returned;
+    //   B3  x = x0 + (a_u ... a_{e-1}) x_in, to the field.
+    // Rows outer, the four columns interleaved, a scheduling barrier per row (converting every row
+    // ahead of the recurrence costs two registers per value).  The pivot of a row is e^-theta in a
+    // converged wave (wave-uniform branch), else the closed form with t advanced per row; rows past
+    // a short chunk's end are skipped.
+    double tf[4], tb[4];
+    auto t_reset = [&]() {
 #pragma unroll
-    for (int u = 0; u < TRI_U; ++u) {
-      const int i = il + u;
-      if (i < H) {
-        const double iv = i == H - 1 ? tp.inv_last : tp.inv(t);
-        w = fma(iv, w, iv * (double)A[u]);
-        uniform_ptr(base + (size_t)i * P)[k] = (float)w;
+      for (int i = 0; i < 4; ++i) {
+        tf[i] = 0.0;
+        tb[i] = 0.0;
+        if (!conv) {
+          tf[i] = exp(-(2.0 * sc + 1.0) * pth[i]);
+          tb[i] = exp(-(2.0 * (sc + lastu) + 1.0) * pth[i]);
+        }
       }
-      t *= tp.q;
+    };
+    // pivot of row u of column i; fw: forward order (t_i -> t_{i+1}), else backward
+    auto pivot = [&](int u, int i, bool fw) -> double {
+      if (conv) return em[i];
+      double &t = fw ? tf[i] : tb[i];
+      TriPivots p;
+      p.em = em[i];
+      p.q = pq[i];
+      const double a = s + u == H - 1 ? plast[i] : p.inv(t);
+      t *= fw ? pq[i] : peq2[i];
+      return a;
+    };
+    auto f1 = [&](double (&w)[4]) {
+      t_reset();
+#pragma unroll
+      for (int u = 0; u < L; ++u) {
+        if (conv || u < len) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const double a = pivot(u, i, true);
+            w[i] = a * ((double)v[u][i] + w[i]);
+            v[u][i] = (float)w[i];
+            if (TRANSFER) Rc[i] = fma(a * Pc[i], Pc[i], Rc[i]);
+            Pc[i] *= a;
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    };
+    auto f3 = [&](double (&pr)[4]) {
+      t_reset();
+#pragma unroll
+      for (int u = 0; u < L; ++u) {
+        if (conv || u < len) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            pr[i] *= pivot(u, i, true);
+            v[u][i] = (float)((double)v[u][i] + pr[i]);
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    };
+    auto b1 = [&](double (&x)[4]) {
+      t_reset();
+#pragma unroll
+      for (int u = L - 1; u >= 0; --u) {
+        if (conv || u < len) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            x[i] = fma(pivot(u, i, false), x[i], (double)v[u][i]);
+            v[u][i] = (float)x[i];
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    };
+    auto b3 = [&](double (&qr)[4]) {
+      t_reset();
+#pragma unroll
+      for (int u = L - 1; u >= 0; --u) {
+        if (conv || u < len) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            qr[i] *= pivot(u, i, false);
+            v[u][i] = (float)((double)v[u][i] + qr[i]);
+          }
+          rowp[(size_t)u * P4] = make_float4(v[u][0], v[u][1], v[u][2], v[u][3]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    };
+
+    // forward transfers, chained per column from the band's incoming w
+    double w[4] = {0.0, 0.0, 0.0, 0.0};
+    f1(w);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      sW[4 * qd + i][ch] = w[i];
+      sP[4 * qd + i][ch] = Pc[i];
+      if (TRANSFER) sR[4 * qd + i][ch] = Rc[i];
     }
+    __syncthreads();
+    {
+      // the thread's four chunks composed in order, the 32 segments scanned, then the incoming w
+      // of each of the four chunks
+      double a4[4], b4[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a4[j] = sP[cc][4 * sg + j];
+        b4[j] = sW[cc][4 * sg + j];
+      }
+      double A = a4[0], B = b4[0];
+#pragma unroll
+      for (int j = 1; j < 4; ++j) {
+        B = fma(a4[j], B, b4[j]);
+        A *= a4[j];
+      }
+      affine_scan_seg(A, B, sg);
+      const double wext = (!TRANSFER && nb > 1) ? tri[4 * bandW + (size_t)band * W + kc] : 0.0;
+      const double Ap = __shfl_up(A, 1, TRC_SEG), Bp = __shfl_up(B, 1, TRC_SEG);
+      double wv = sg == 0 ? wext : fma(Ap, wext, Bp);
+      double wu = sg == 0 ? 1.0 : Ap;  // response to a unit incoming w of the band (TRANSFER)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        sW[cc][4 * sg + j] = wv;
+        if (TRANSFER) sR[cc][4 * sg + j] *= wu;
+        wv = fma(a4[j], wv, b4[j]);
+        wu *= a4[j];
+      }
+      if (TRANSFER && sg == TRC_SEG - 1 && actc) {
+        tri[(size_t)band * W + kc] = wv;          // w at the band's last row (zero incoming w)
+        tri[bandW + (size_t)band * W + kc] = wu;  // pivot product of the band
+      }
+    }
+    __syncthreads();
+    // the true w of the chunk, then its backward transfer
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = sW[4 * qd + i][ch];
+    f3(w);
+    double x[4] = {0.0, 0.0, 0.0, 0.0};
+    b1(x);
+    __syncthreads();  // every thread has read its incoming w: sW now takes the x transfers
+#pragma unroll
+    for (int i = 0; i < 4; ++i) sW[4 * qd + i][ch] = x[i];
+    __syncthreads();
+    {
+      // backward chain: thread (column, segment 31 - sg) composes its four chunks from the last,
+      // the scan runs from the band's last segment upwards; the unit response shares the products
+      const int g4 = 4 * (TRC_SEG - 1 - sg);
+      double a4[4], b4[4], u4[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        a4[j] = sP[cc][g4 + j];
+        b4[j] = sW[cc][g4 + j];
+        u4[j] = TRANSFER ? sR[cc][g4 + j] : 0.0;
+      }
+      double A = a4[3], B = b4[3], U = u4[3];
+#pragma unroll
+      for (int j = 2; j >= 0; --j) {
+        B = fma(a4[j], B, b4[j]);
+        if (TRANSFER) U = fma(a4[j], U, u4[j]);
+        A *= a4[j];
+      }
+#pragma unroll
+      for (int dd = 1; dd < TRC_SEG; dd <<= 1) {
+        const double ap = __shfl_up(A, dd, TRC_SEG), bp = __shfl_up(B, dd, TRC_SEG);
+        double up = 0.0;
+        if (TRANSFER) up = __shfl_up(U, dd, TRC_SEG);
+        if (sg >= dd) {
+          B = fma(A, bp, B);
+          if (TRANSFER) U = fma(A, up, U);
+          A *= ap;
+        }
+      }
+      const double xext = (!TRANSFER && nb > 1) ? tri[5 * bandW + (size_t)band * W + kc] : 0.0;
+      const double Ap = __shfl_up(A, 1, TRC_SEG), Bp = __shfl_up(B, 1, TRC_SEG);
+      double xv = sg == 0 ? xext : fma(Ap, xext, Bp);
+      double xu = 0.0;
+      if (TRANSFER) {
+        const double Up = __shfl_up(U, 1, TRC_SEG);
+        xu = sg == 0 ? 0.0 : Up;
+      }
+#pragma unroll
+      for (int j = 3; j >= 0; --j) {
+        sW[cc][g4 + j] = xv;
+        xv = fma(a4[j], xv, b4[j]);
+        if (TRANSFER) xu = fma(a4[j], xu, u4[j]);
+      }
+      if (TRANSFER && sg == TRC_SEG - 1 && actc) {
+        tri[2 * bandW + (size_t)band * W + kc] = xv;  // x at the band's first row (zero carries)
+        tri[3 * bandW + (size_t)band * W + kc] = xu;  // its response to a unit incoming w
+      }
+    }
+    if (TRANSFER) return;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) x[i] = sW[4 * qd + i][ch];
+    b3(x);
   }
-  // backward: x_i = w_i + x_{i+1} / d_i, the last batch first
-  double x = 0.0;
-  tri_load(A, base, k, P, il, H);
-  tri_load(B, base, k, P, il - TRI_U, H);
-  tri_load(C, base, k, P, il - 2 * TRI_U, H);
-  {
-    double t = tp.t_at(il + TRI_U - 1);
-#pragma unroll
-    for (int u = TRI_U - 1; u >= 0; --u) {
-      const int i = il + u;
-      if (i < H) {
-        const double iv = i == H - 1 ? tp.inv_last : tp.inv(t);
-        x = fma(iv, x, (double)A[u]);
-        uniform_ptr(base + (size_t)i * P)[k] = (float)x;
-      }
-      t *= tp.eq2;
-    }
+}
+
+// Column 0 (the singular solve), one wave per image, after the final k_col_trc launches.
+__global__ __launch_bounds__(64) void k_col_trc_dc(Ctx c, float *__restrict__ T, int guard) {
+  const int img = find_img(c.geo.tridc, c.n, blockIdx.x);
+  const ImgDesc d = c.img[img];
+  if (guard_off(&c.st[img], guard)) return;
+  tri_singular((const float *)(c.tri + d.tri_dc), T + d.off, (unsigned)d.P, d.H, (int)threadIdx.x);
+}
+
+// Chains the bands of each column k >= 1 (256 columns per block): the incoming w of band b + 1 is
+// w_end(b) + prod(b) w_in(b), the incoming x of band b - 1 is x0(b) + prod(b) x_in(b) + unit(b) w_in(b).
+__global__ __launch_bounds__(256) void k_col_trc_carry(Ctx c, int guard) {
+  const int img = find_img(c.geo.tricarry, c.n, blockIdx.x);
+  const ImgDesc d = c.img[img];
+  if (guard_off(&c.st[img], guard)) return;
+  const int W = d.W, nb = d.tri_nb;
+  const int k = 1 + (int)(blockIdx.x - c.geo.tricarry[img]) * 256 + (int)threadIdx.x;
+  if (k >= W) return;
+  double *__restrict__ tri = c.tri + d.tri_off;
+  const size_t bandW = (size_t)nb * W;
+  const double *__restrict__ wend = tri + k;
+  const double *__restrict__ prod = tri + bandW + k;
+  const double *__restrict__ x0 = tri + 2 * bandW + k;
+  const double *__restrict__ unit = tri + 3 * bandW + k;
+  double *__restrict__ win = tri + 4 * bandW + k;
+  double *__restrict__ xin = tri + 5 * bandW + k;
+  double wv = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    win[(size_t)b * W] = wv;
+    wv = fma(prod[(size_t)b * W], wv, wend[(size_t)b * W]);
   }
-  auto bwd = [&](int b, float (&cur)[TRI_U], float (&pre)[TRI_U]) {
-    const int i0 = b * TRI_U;
-    tri_load(pre, base, k, P, i0 - 2 * TRI_U, H);
-    double t = tp.t_at(i0 + TRI_U - 1);
-    float *__restrict__ row = base + (size_t)i0 * P;
-    if (__ballot(tp.t_at(i0) > TSMALL)) {  // t grows towards row i0
-#pragma unroll
-      for (int u = TRI_U - 1; u >= 0; --u) {
-        x = fma(tp.inv(t), x, (double)cur[u]);
-        uniform_ptr(row + (size_t)u * P)[k] = (float)x;
-        t *= tp.eq2;
-      }
-    } else {
-#pragma unroll
-      for (int u = TRI_U - 1; u >= 0; --u) {
-        x = fma(em, x, (double)cur[u]);
-        uniform_ptr(row + (size_t)u * P)[k] = (float)x;
-      }
-    }
-  };
-  for (int b = nfull - 1; b >= 0; b -= 3) {
-    bwd(b, B, A);
-    if (b - 1 >= 0) bwd(b - 1, C, B);
-    if (b - 2 >= 0) bwd(b - 2, A, C);
+  double xv = 0.0;
+  for (int b = nb - 1; b >= 0; --b) {
+    xin[(size_t)b * W] = xv;
+    xv = fma(prod[(size_t)b * W], xv, fma(unit[(size_t)b * W], win[(size_t)b * W], x0[(size_t)b * W]));
   }
 }
 

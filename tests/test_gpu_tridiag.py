@@ -1,9 +1,10 @@
-"""GPU tests of the tridiagonal column pass (k_col_tri): after the row DCT-II, the column DCT pair
+"""GPU tests of the tridiagonal column pass (k_col_trc): after the row DCT-II, the column DCT pair
 with the division by -lambda is replaced by the float64 solves (L_H + mu_k I) x = c_k (the same
 operator, utils_unwrap2d.py:1390-1403).  Checked against scipy's FP64 DCT preconditioner on
-edge shapes (one row, one or two columns, prime heights up to 12007, Bluestein rows) in batches
-with slab, transposed and long images; against the DCT column pass; solver parity with the FP64
-and float32 oracles; batched = plain bitwise."""
+edge shapes (one row, one or two columns, prime heights up to 12007, Bluestein rows, heights at
+the chunk and band edges 512 / 513 / 1536 / 1537 / 3072) in batches with slab, transposed and
+long images; against the DCT column pass; solver parity with the FP64 and float32 oracles;
+batched = plain bitwise."""
 import numpy as np
 import pytest
 
@@ -14,7 +15,8 @@ pytestmark = pytest.mark.gpu
 FIXED = dict(max_iter=10, tol=0.0, cg_max_iter=50, cg_tol=0.0)
 
 SHAPES = [(1, 5), (2, 7), (3, 3), (17, 1), (1, 1), (4, 3), (64, 48), (127, 211), (1030, 5000), (1201, 4999),
-          (12007, 9), (5, 12007), (300, 2), (8192, 70), (12800, 8), (96, 130)]
+          (12007, 9), (5, 12007), (300, 2), (8192, 70), (12800, 8), (96, 130), (512, 100), (513, 65),
+          (1536, 31), (1537, 40), (3072, 33), (33, 77)]
 
 
 def _precond64(r):
@@ -56,7 +58,8 @@ def test_tridiag_preconditioner_matches_scipy_fp64(hip, mode):
         strat = _lib.describe_columns(SHAPES)
         print(f"tridiag preconditioner, mode {mode}: {strat}", flush=True)
         if mode == "tridiag":
-            assert [s for s, (h, _) in zip(strat, SHAPES) if h <= 12288] == ["tridiag"] * 15
+            short = [s for s, (h, _) in zip(strat, SHAPES) if h <= 12288]
+            assert short == ["tridiag"] * len(short)
         else:
             assert strat[SHAPES.index((1030, 5000))] == "tridiag" and strat[SHAPES.index((8192, 70))] == "slab_register"
         zs = _precond_gpu(hip, rs)
