@@ -284,19 +284,23 @@ __device__ __forceinline__ float4 mask4(float4 v, int c0, int W) {
   return make_float4(c0 < W ? v.x : 0.f, c0 + 1 < W ? v.y : 0.f, c0 + 2 < W ? v.z : 0.f, c0 + 3 < W ? v.w : 0.f);
 }
 
-// Row-marching driver of the float4 stencils: wave w walks down the strip j0 + 256 w of its
-// tile; lane l owns columns c0 = strip + 4l .. c0 + 3.  X(i) loads the operand at (i, c0..c0+3)
-// (zero outside the image), XE(i, j) one operand value (zero outside), E(i, ...) consumes row i.
-// Lanes with c0 >= W own no pixel: they touch no memory (c0 < W keeps c0 + 3 inside the row
-// pitch, a multiple of 64) and only feed zeros to the shuffles.
-// Rows i - 1, i, i + 1 of the operand and the y-weights of row i - 1 stay in registers; the
-// left/right neighbours come from the adjacent lanes, at the strip edges from one extra load.
-template <class LX, class LXE, class EMIT>
-__device__ __forceinline__ void march_rows(const Ctx &c, const ImgDesc &d, const TilePos &t, LX &&X, LXE &&XE,
-                                           EMIT &&E) {
+// Row-marching driver of the float4 stencils: wave w walks the strip j0 + 256 w of its tile, one
+// row at a time; lane l owns columns c0 = strip + 4l .. c0 + 3.  X(i) loads the operand at
+// (i, c0..c0+3) (zero outside the image), XE(i, j) one operand value (zero outside), E(i, ...)
+// consumes row i.  Lanes with c0 >= W own no pixel: they touch no memory (c0 < W keeps c0 + 3
+// inside the row pitch, a multiple of 64) and only feed zeros to the shuffles.
+// Three rows of the operand and the y-weights between the current row and the one behind stay in
+// registers; the left/right neighbours come from the adjacent lanes, at the strip edges from loads
+// by lanes 0 and 63 one row ahead.  Those edge columns are the outer columns of the neighbouring
+// strips, which the neighbouring waves (of this block and of the tiles left and right, run at the
+// same time on the same XCD) read in step with this one, so they are L2 hits.  For the same
+// reason even tile rows march down and odd ones up: the halo rows two vertically adjacent tiles
+// share are then read by both near their starts, or both near their ends.
+template <bool UP, class LX, class LXE, class EMIT>
+__device__ __forceinline__ void march_dir(const Ctx &c, const ImgDesc &d, const TilePos &t, LX &&X, LXE &&XE,
+                                          EMIT &&E) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int jw = t.j0 + STRIP * wv;
-  if (jw >= d.W) return;
   const int c0 = jw + 4 * lane;
   const int H = d.H, W = d.W;
   const int iend = min(t.i0 + TILE_H, H);
@@ -304,41 +308,62 @@ __device__ __forceinline__ void march_rows(const Ctx &c, const ImgDesc &d, const
   const float *__restrict__ wyi = c.f.wyi;
   const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
   const bool own = c0 < W;
-  float4 xu = X(t.i0 - 1), xc = X(t.i0);
-  float4 wyu = (own && t.i0 > 0) ? ld4(wyi, d.off + (long long)(t.i0 - 1) * d.P + c0) : z4;
-  // The strip's edge columns for all its rows, loaded once: lane e < TILE_H holds the operand left
-  // of the strip in row i0 + e, lane TILE_H + e the one right of it, lane 2 TILE_H + e the x-weight
-  // left of it; row i reads them with readlane (a uniform lane index).
-  static_assert(3 * TILE_H <= 64, "edge columns of a tile fit one wave");
-  float edge = 0.f;
-  {
-    const int e = lane % TILE_H, grp = lane / TILE_H, ie = t.i0 + e;
-    if (ie < iend) {
-      if (grp == 0) edge = XE(ie, jw - 1);
-      else if (grp == 1) edge = XE(ie, jw + STRIP);
-      else if (grp == 2 && jw > 0) edge = wxi[d.off + (long long)ie * d.P + jw - 1];
+  constexpr int step = UP ? -1 : 1;
+  const int ifirst = UP ? iend - 1 : t.i0;
+  const int ilast = UP ? t.i0 : iend - 1;
+  const bool edge_lane = lane == 0 || lane == 63;
+  const int je = lane == 0 ? jw - 1 : jw + STRIP;
+  // operand / x-weight left of the strip in row i, for lanes 0 (left) and 63 (right)
+  auto edges = [&](int i, float &ex, float &ew) {
+    ex = 0.f;
+    ew = 0.f;
+    if (edge_lane) {
+      ex = XE(i, je);
+      if (lane == 0 && jw > 0) ew = wxi[d.off + (long long)i * d.P + jw - 1];
     }
-  }
-  const int edge_bits = __float_as_int(edge);
-  for (int i = t.i0; i < iend; ++i) {
+  };
+  // xb: the row behind the march (above when marching down), xc: the current row; wb: the
+  // y-weights between them (row i - 1 down, row i up)
+  float4 xb = X(ifirst - step), xc = X(ifirst);
+  const int iwb = UP ? ifirst : ifirst - 1;
+  float4 wb = (own && iwb >= 0) ? ld4(wyi, d.off + (long long)iwb * d.P + c0) : z4;
+  float ex, ew;
+  edges(ifirst, ex, ew);
+  for (int i = ifirst;; i += step) {
     const long long g = d.off + (long long)i * d.P + c0;
-    const float4 xd = X(i + 1);
-    const float4 wx = own ? ld4(wxi, g) : z4, wy = own ? ld4(wyi, g) : z4;
+    const int ia = i + step;
+    const float4 xa = X(ia);
+    const int iwf = UP ? i - 1 : i;
+    const float4 wx = own ? ld4(wxi, g) : z4;
+    const float4 wf = (own && iwf >= 0) ? ld4(wyi, d.off + (long long)iwf * d.P + c0) : z4;
+    float exa = 0.f, ewa = 0.f;
+    if (i != ilast) edges(ia, exa, ewa);
     float xl = __shfl_up(xc.w, 1, 64);
     float wxl = __shfl_up(wx.w, 1, 64);
     float xr = __shfl_down(xc.x, 1, 64);
-    const int e = i - t.i0;
     if (lane == 0) {
-      xl = __int_as_float(__builtin_amdgcn_readlane(edge_bits, e));
-      wxl = __int_as_float(__builtin_amdgcn_readlane(edge_bits, 2 * TILE_H + e));
+      xl = ex;
+      wxl = ew;
     }
-    if (lane == 63) xr = __int_as_float(__builtin_amdgcn_readlane(edge_bits, TILE_H + e));
-    const float4 ax = apply_A4(i, c0, H, W, xc, xl, xr, xu, xd, wx, wxl, wyu, wy);
+    if (lane == 63) xr = ex;
+    const float4 ax = UP ? apply_A4(i, c0, H, W, xc, xl, xr, xa, xb, wx, wxl, wf, wb)
+                         : apply_A4(i, c0, H, W, xc, xl, xr, xb, xa, wx, wxl, wb, wf);
     if (own) E(i, g, xc, ax);
-    xu = xc;
-    xc = xd;
-    wyu = wy;
+    if (i == ilast) break;
+    xb = xc;
+    xc = xa;
+    wb = wf;
+    ex = exa;
+    ew = ewa;
   }
+}
+
+template <class LX, class LXE, class EMIT>
+__device__ __forceinline__ void march_rows(const Ctx &c, const ImgDesc &d, const TilePos &t, LX &&X, LXE &&XE,
+                                           EMIT &&E) {
+  if (t.j0 + STRIP * (int)(threadIdx.x >> 6) >= d.W) return;
+  if ((t.i0 / TILE_H) & 1) march_dir<true>(c, d, t, X, XE, E);
+  else march_dir<false>(c, d, t, X, XE, E);
 }
 
 // PCG start: r = b - A x (x = u); a non-finite r returns x unchanged (utils_unwrap2d.py:1414-1419).

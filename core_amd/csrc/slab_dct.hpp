@@ -402,16 +402,19 @@ __global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = make_float2(0.f, 0.f);
   }
-  // per-block tables at k = k1 + N1 kk of both sets: exp(-i pi k / 2N) and 2 cos(pi k / H)
+  // per-block tables at k = k1 + N1 kk of both sets: exp(-i pi k / 2N) and the column half of
+  // the eigenvalue, s_k = 4 sin^2(pi k / 2H) = 2 - 2 cos(pi k / H) (from the float64 cosine,
+  // rounded once).  -lambda_kj = s_k + s_j: two positive terms, so the float32 sum keeps the
+  // float32-rounded float64 eigenvalue of the reference to about one unit in the last place.
   float2 *mkT = buf + N2 * SB;
-  double *chT = (double *)(mkT + 2 * N2);
+  float *sT = (float *)(mkT + 2 * N2);
   {
     const float2 *mk = c.tab + L.mk_off;
     const double *cosH = c.dtab + b.d->cosH_off;
     for (int e = threadIdx.x; e < 2 * N2; e += blockDim.x) {
       const int k = (e < N2 ? k1a : k1b) + N1 * (e & (N2 - 1));
       mkT[e] = mk[k];
-      chT[e] = 2.0 * cosH[k];
+      sT[e] = (float)(2.0 - 2.0 * cosH[k]);
     }
   }
   __syncthreads();
@@ -431,8 +434,8 @@ __global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
   // of k1 = 0 at -kk mod N2, of k1 = N1 / 2 at N2 - 1 - kk), / -lambda, DCT-III pre-step at k
   const int ja = 2 * (b.slab * SLAB_CP + cp);
   const bool hasa = live && ja < W, hasb = hasa && ja + 1 < W;
-  const double cwa = hasa ? 2.0 * c.dtab[b.d->cosW_off + ja] : 0.0;
-  const double cwb = hasb ? 2.0 * c.dtab[b.d->cosW_off + ja + 1] : 0.0;
+  const float swa = hasa ? (float)(2.0 - 2.0 * c.dtab[b.d->cosW_off + ja]) : 0.f;
+  const float swb = hasb ? (float)(2.0 - 2.0 * c.dtab[b.d->cosW_off + ja + 1]) : 0.f;
   const int sp = two ? (s ^ SLAB_CP) : s;
   const int ts = sig ? N2 : 0, tp = (sp >> 5) ? N2 : 0;  // table rows of the own and partner set
 #pragma unroll
@@ -448,11 +451,11 @@ __global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
         float Xak, Xbk, Xan, Xbn;
         const float2 wk = mkT[ts + kk];
         dct2_coef_w(o[i * R + k2], buf[kp * SB + sp], wk, mkT[tp + kp], Xak, Xbk, Xan, Xbn);
-        const double ck = chT[ts + kk], cn = chT[tp + kp];
-        Xak = (k == 0 && ja == 0) ? 0.f : -(Xak * __builtin_amdgcn_rcpf((float)(ck + cwa - 4.0)));
-        Xbk = hasb ? -(Xbk * __builtin_amdgcn_rcpf((float)(ck + cwb - 4.0))) : 0.f;
-        Xan = (nk == 0 && ja == 0) ? 0.f : -(Xan * __builtin_amdgcn_rcpf((float)(cn + cwa - 4.0)));
-        Xbn = hasb ? -(Xbn * __builtin_amdgcn_rcpf((float)(cn + cwb - 4.0))) : 0.f;
+        const float sk = sT[ts + kk], sn = sT[tp + kp];
+        Xak = (k == 0 && ja == 0) ? 0.f : Xak * __builtin_amdgcn_rcpf(sk + swa);
+        Xbk = hasb ? Xbk * __builtin_amdgcn_rcpf(sk + swb) : 0.f;
+        Xan = (nk == 0 && ja == 0) ? 0.f : Xan * __builtin_amdgcn_rcpf(sn + swa);
+        Xbn = hasb ? Xbn * __builtin_amdgcn_rcpf(sn + swb) : 0.f;
         cz = dct3_pre_w(wk, Xak, Xbk, k ? Xan : 0.f, k ? Xbn : 0.f);
       }
       o[i * R + k2] = cz;
