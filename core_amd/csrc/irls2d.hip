@@ -816,8 +816,8 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
   for (auto &C : Lo.slabcls)
     if (C.f1) {
       // register-stage kernels: K1 / K3 on 256 threads, K2 on 4 N2; exchange buffer, then the K2
-      // tables of 2 N2 float2 twiddles and 2 N2 floats (room for 2 N2 doubles)
-      C.lds = (size_t)(TW_LDS + std::max(C.f1 * SLAB_CP * (128 / C.f1), C.f2 * 2 * SLAB_CP + 4 * C.f2)) *
+      // tables (5 N2 float2 twiddles and 2 N2 floats)
+      C.lds = (size_t)(TW_LDS + std::max(C.f1 * SLAB_CP * (128 / C.f1), C.f2 * 2 * SLAB_CP + 6 * C.f2)) *
               sizeof(float2);
     }
   // workspace map

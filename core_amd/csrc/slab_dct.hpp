@@ -378,6 +378,12 @@ __global__ __launch_bounds__(256) void k_slab3_f(Ctx c, float *__restrict__ dst,
 }
 
 // K2 for N2 = 16 R: 64 R threads, slots = set (k1a, k1b) x SLAB_CP pairs.
+// Per-block LDS tables (built once, before the first barrier): rt[j] = exp(-2 pi i j / N2) for the
+// twiddles inside both length-N2 FFTs, ft[set][m] = exp(-2 pi i k1 m / N) for the last one, and at
+// k = k1 + N1 kk of both sets the Makhoul post-twiddle exp(-i pi k / 2N) and 2 s_k, the doubled
+// column half of -lambda.  Each thread forms the scaled DCT-II coefficients of both lines at its
+// own k only; the partner's (at N - k) come through LDS, so every coefficient and reciprocal is
+// computed once.
 template <int N2>
 __global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
   constexpr int R = N2 / 16, Q = 16 / R, SB = 2 * SLAB_CP;
@@ -387,39 +393,53 @@ __global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
   const SlabPlan &L = *b.L;
   const int N = L.N, N1 = L.N1, W = b.d->W;
   float2 *tws = lds, *buf = lds + TW_LDS;
+  float2 *mkT = buf + N2 * SB, *rt = mkT + 2 * N2, *ft = rt + N2;
+  float *sT = (float *)(ft + 2 * N2);
   load_tws_len(tws, c.tab + L.tw_off, N);
   const int s = threadIdx.x & (SB - 1), t = threadIdx.x / SB;
   const int cp = s & (SLAB_CP - 1), sig = s >> 5;
   const int k1a = b.grp, k1b = b.grp ? N1 - b.grp : 0;
   const bool two = k1a != k1b, live = sig == 0 || two;
   const int k1 = sig ? k1b : k1a;
-  float2 *Srow = b.S + (long long)k1 * N2 * b.ss + cp;
+  // element n2 = t + R j of this slot's sequence: S row k1 N2 + n2 (rows ss float2 apart)
+  const float2 *Sl = b.S + ((long long)k1 * N2 + t) * b.ss + cp;
+  const unsigned rs = (unsigned)(R * b.ss);
   float2 v[16];
   if (live) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) v[j] = Srow[(t + R * j) * b.ss];
+    for (int j = 0; j < 16; ++j) v[j] = Sl[j * rs];
   } else {
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = make_float2(0.f, 0.f);
   }
-  // per-block tables at k = k1 + N1 kk of both sets: exp(-i pi k / 2N) and the column half of
-  // the eigenvalue, s_k = 4 sin^2(pi k / 2H) = 2 - 2 cos(pi k / H) (from the float64 cosine,
-  // rounded once).  -lambda_kj = s_k + s_j: two positive terms, so the float32 sum keeps the
-  // float32-rounded float64 eigenvalue of the reference to about one unit in the last place.
-  float2 *mkT = buf + N2 * SB;
-  float *sT = (float *)(mkT + 2 * N2);
+  // s_k = 4 sin^2(pi k / 2H) = 2 - 2 cos(pi k / H) from the float64 cosine, rounded once; the
+  // float32 sum -lambda_kj = s_k + s_j of two positive terms keeps the float32-rounded float64
+  // eigenvalue to about one unit in the last place.  Both are stored doubled (exactly), which
+  // folds the DCT-III pre-step's factor 1/2 into the reciprocal.
+  const int tstep = N / N2;
   {
     const float2 *mk = c.tab + L.mk_off;
     const double *cosH = c.dtab + b.d->cosH_off;
     for (int e = threadIdx.x; e < 2 * N2; e += blockDim.x) {
-      const int k = (e < N2 ? k1a : k1b) + N1 * (e & (N2 - 1));
+      const int kk = e & (N2 - 1), k1e = e < N2 ? k1a : k1b;
+      const int k = k1e + N1 * kk;
       mkT[e] = mk[k];
-      sT[e] = (float)(2.0 - 2.0 * cosH[k]);
+      sT[e] = (float)(4.0 - 4.0 * cosH[k]);
+      ft[e] = twiddle(tws, k1e * kk);
+      if (e < N2) rt[e] = twiddle(tws, e * tstep);
     }
   }
   __syncthreads();
-  const int tstep = N / N2;
-  reg_stage1<N2>(v, t, tws, tstep, buf, SB, s);
+  // first FFT, stage 1: DFT-16 over j, times exp(-2 pi i t k' / N2), exchange
+  Dft<16>::run(v);
+  if (R > 1) {
+    if (t) {
+#pragma unroll
+      for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], rt[t * k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) buf[(k * R + t) * SB + s] = v[k];
+  }
   __syncthreads();
   float2 o[16];
   reg_stage2<N2>(o, v, t, buf, SB, s);
@@ -430,35 +450,52 @@ __global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
 #pragma unroll
     for (int k2 = 0; k2 < R; ++k2) buf[(t * Q + i + 16 * k2) * SB + s] = o[i * R + k2];
   __syncthreads();
-  // DCT-II coefficients at k and N - k (partner: the other set at N2 - 1 - kk; in the single set
-  // of k1 = 0 at -kk mod N2, of k1 = N1 / 2 at N2 - 1 - kk), / -lambda, DCT-III pre-step at k
+  // partner of kk: the other set at N2 - 1 - kk; in the single set of k1 = 0 at -kk mod N2, of
+  // k1 = N1 / 2 at N2 - 1 - kk
   const int ja = 2 * (b.slab * SLAB_CP + cp);
   const bool hasa = live && ja < W, hasb = hasa && ja + 1 < W;
-  const float swa = hasa ? (float)(2.0 - 2.0 * c.dtab[b.d->cosW_off + ja]) : 0.f;
-  const float swb = hasb ? (float)(2.0 - 2.0 * c.dtab[b.d->cosW_off + ja + 1]) : 0.f;
+  const float swa = hasa ? (float)(4.0 - 4.0 * c.dtab[b.d->cosW_off + ja]) : 0.f;
+  const float swb = hasb ? (float)(4.0 - 4.0 * c.dtab[b.d->cosW_off + ja + 1]) : 0.f;
   const int sp = two ? (s ^ SLAB_CP) : s;
-  const int ts = sig ? N2 : 0, tp = (sp >> 5) ? N2 : 0;  // table rows of the own and partner set
+  const int ts = sig ? N2 : 0;
+  const bool refl0 = !two && k1a == 0;
+  auto partner = [&](int kk) { return refl0 ? (kk ? N2 - kk : 0) : N2 - 1 - kk; };
+  // own DCT-II coefficients of both lines at k, divided by -2 lambda
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
 #pragma unroll
     for (int k2 = 0; k2 < R; ++k2) {
       const int kk = t * Q + i + 16 * k2;
-      const int kp = (two || k1a) ? N2 - 1 - kk : (kk ? N2 - kk : 0);
-      const int k = k1 + N1 * kk;
-      const int nk = k ? N - k : 0;  // = partner set's k1 + N1 kp
-      float2 cz = make_float2(0.f, 0.f);
-      if (hasa) {
-        float Xak, Xbk, Xan, Xbn;
-        const float2 wk = mkT[ts + kk];
-        dct2_coef_w(o[i * R + k2], buf[kp * SB + sp], wk, mkT[tp + kp], Xak, Xbk, Xan, Xbn);
-        const float sk = sT[ts + kk], sn = sT[tp + kp];
-        Xak = (k == 0 && ja == 0) ? 0.f : Xak * __builtin_amdgcn_rcpf(sk + swa);
-        Xbk = hasb ? Xbk * __builtin_amdgcn_rcpf(sk + swb) : 0.f;
-        Xan = (nk == 0 && ja == 0) ? 0.f : Xan * __builtin_amdgcn_rcpf(sn + swa);
-        Xbn = hasb ? Xbn * __builtin_amdgcn_rcpf(sn + swb) : 0.f;
-        cz = dct3_pre_w(wk, Xak, Xbk, k ? Xan : 0.f, k ? Xbn : 0.f);
-      }
-      o[i * R + k2] = cz;
+      const float2 Zk = o[i * R + k2], Zn = buf[partner(kk) * SB + sp];
+      const float2 wk = mkT[ts + kk];
+      const float sk = sT[ts + kk];
+      const float xa = wk.x * (Zk.x + Zn.x) - wk.y * (Zk.y - Zn.y);
+      const float xb = wk.x * (Zk.y + Zn.y) + wk.y * (Zk.x - Zn.x);
+      const bool dc = ja == 0 && k1 == 0 && kk == 0;
+      o[i * R + k2] = make_float2((hasa && !dc) ? xa * __builtin_amdgcn_rcpf(sk + swa) : 0.f,
+                                  hasb ? xb * __builtin_amdgcn_rcpf(sk + swb) : 0.f);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < Q; ++i)
+#pragma unroll
+    for (int k2 = 0; k2 < R; ++k2) buf[(t * Q + i + 16 * k2) * SB + s] = o[i * R + k2];
+  __syncthreads();
+  // DCT-III pre-step at k from the scaled coefficients at k and N - k (none at k = 0)
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+#pragma unroll
+    for (int k2 = 0; k2 < R; ++k2) {
+      const int kk = t * Q + i + 16 * k2;
+      const float2 y = o[i * R + k2];
+      float2 yn = buf[partner(kk) * SB + sp];
+      if (k1 == 0 && kk == 0) yn = make_float2(0.f, 0.f);
+      const float2 wk = mkT[ts + kk];
+      const float cx = wk.x, cy = -wk.y;
+      const float vax = cx * y.x + cy * yn.x, vay = cy * y.x - cx * yn.x;
+      const float vbx = cx * y.y + cy * yn.y, vby = cy * y.y - cx * yn.y;
+      o[i * R + k2] = make_float2(vax - vby, -(vay + vbx));
     }
   }
   // second FFT over kk = a + 16 b (a = t Q + i): DFT-R over b, times exp(-2 pi i a m' / N2),
@@ -472,22 +509,18 @@ __global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
     Dft<R>::run(w);
     const int a = t * Q + i;
 #pragma unroll
-    for (int m = 0; m < R; ++m) {
-      const float2 y = (a && m) ? cmul(w[m], twiddle(tws, a * m * tstep)) : w[m];
-      buf[(a * R + m) * SB + s] = y;
-    }
+    for (int m = 0; m < R; ++m) buf[(a * R + m) * SB + s] = m ? cmul(w[m], rt[a * m]) : w[m];
   }
   __syncthreads();
 #pragma unroll
   for (int a = 0; a < 16; ++a) v[a] = buf[(a * R + t) * SB + s];
   Dft<16>::run(v);
   if (!live) return;
+  float2 *So = const_cast<float2 *>(Sl);
 #pragma unroll
   for (int mm = 0; mm < 16; ++mm) {
     const int m = t + R * mm;
-    float2 y = v[mm];
-    if (k1 && m) y = cmul(y, twiddle(tws, k1 * m));
-    Srow[m * b.ss] = y;
+    So[mm * rs] = (k1 && m) ? cmul(v[mm], ft[ts + m]) : v[mm];
   }
 }
 
