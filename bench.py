@@ -74,6 +74,9 @@ ALG_BPX_LONG = {
     "k_long_p2_dct3_rows": 12,    # scratch, r -> z
     "k_cg_pap": 24,
     "k_transpose": 8,
+    # column pass of tall images as tridiagonal solves: bands of 1536 rows
+    "k_col_trc_band": 4,          # T -> per-band transfers (a few values per column and band)
+    "k_col_trc": 8,               # T -> T
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_8192.json")
 
@@ -311,7 +314,8 @@ def max_scale(args, dev):
     valid = upload(ws, [(~np.isnan(psi)).astype(np.float32)], dev) > 0.5
     del psi, coh, true
     out = torch.empty(ws.total, dtype=torch.float32, device=dev)
-    kw = dict(max_iter=N_IRLS, tol=0.0, cg_max_iter=N_CG, cg_tol=0.0, out=out, workspace=ws)
+    n_irls = getattr(args, "max_scale_irls", None) or N_IRLS
+    kw = dict(max_iter=n_irls, tol=0.0, cg_max_iter=N_CG, cg_tol=0.0, out=out, workspace=ws)
     print(f"max_scale: inputs in HBM, workspace {ws.nbytes / 2**30:.1f} GiB; graph solve", file=sys.stderr, flush=True)
     ms, _ = time_steps(lambda: solve_device(ph, co, shapes, graph=True, **kw), 1, 1)
     info = ws.info.cpu().tolist()[:4]
@@ -340,9 +344,10 @@ def max_scale(args, dev):
     torch.cuda.empty_cache()
     return {
         "workload": "40000x40000, ramp 0-1500 rad, 20 bumps, 3 pi fault, 5% masked blocks, seed 4; "
-                    "10 IRLS x 50 PCG; four-step two-pass DCT (200 x 200) both ways",
+                    f"{n_irls} IRLS x {N_CG} PCG; rows: four-step two-pass DCT (200 x 200); "
+                    "columns: tridiagonal solves",
         "pixels": npx, "ms_per_solve": round(ms, 1), "mpx_s": round(npx / ms / 1e3, 2),
-        "solve_model_frac_hbm": round(npx * model_bytes_per_px() / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "solve_model_frac_hbm": round(npx * model_bytes_per_px(n_irls) / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         "info": info, "finite": finite,
         "vs_truth": {"cycle_correct_frac": round(cyc, 6), "rms_rad": round(rms, 4),
                      "note": "the 3 pi fault wraps to a pi step, so any unwrapper leaves one side of it "
@@ -559,6 +564,8 @@ def main():
                     help="run only the 48-burst batched stack, with per-kernel durations")
     ap.add_argument("--stack-irls", type=int, default=None,
                     help="IRLS steps of the batched stack (default 10; fewer for counter runs under a profiler)")
+    ap.add_argument("--max-scale-irls", type=int, default=None,
+                    help="IRLS steps of the 40000 x 40000 solve (default 10; fewer for counter runs under a profiler)")
     ap.add_argument("--no-sequential", action="store_true",
                     help="skip the 48 sequential plain calls the batched stack is compared with")
     ap.add_argument("--column-mode", choices=["auto", "transpose", "direct", "slab", "tridiag"], default="auto",

@@ -145,6 +145,7 @@ static_assert(LONG_THREADS % LONG_MAX_BATCH == 0, "a thread must keep its slot a
 struct LongLine {
   const LongPlan *L;
   int img, lp, blk;  // image, line pair, block within the line pair
+  int gb;            // logical block index of the launch (partials slot)
   long long la;      // element offset of line a (line b = la + pitch)
   long long ta, tb;  // offsets of lines a and b in the row-pass output T (its slab row order for
                      // an image whose column pass runs on slabs, else la and la + pitch)
@@ -155,15 +156,18 @@ struct LongLine {
 
 // The line pair and block of this workgroup in a launch over `geo`, whose images have `which`
 // blocks per line pair: 1 = nb1, 2 = nb2, 0 = nbp (Bluestein pack / post).  `local` (optional)
-// receives the block's index among its image's blocks.
+// receives the block's index among its image's blocks.  Logical blocks are numbered XCD-major
+// (xcd_block): the blocks of one line pair, whose row segments share cache lines at their ends,
+// then run on one XCD at about the same time and fetch those lines once into its L2.
 __device__ __forceinline__ bool long_line_geo(const Ctx &c, int dir, const int *geo, int which, int guard,
                                               LongLine &ln, int *local_out = nullptr) {
-  ln.img = find_img(geo, c.n, blockIdx.x);
+  ln.gb = xcd_block(blockIdx.x, gridDim.x);
+  ln.img = find_img(geo, c.n, ln.gb);
   const ImgDesc &d = c.img[ln.img];
   if (guard_off(&c.st[ln.img], guard)) return false;
   ln.L = c.lplans + (dir ? d.collong : d.rowlong);
   const int nbp = which == 1 ? ln.L->nb1 : (which == 2 ? ln.L->nb2 : ln.L->nbp);
-  const int local = blockIdx.x - geo[ln.img];
+  const int local = ln.gb - geo[ln.img];
   if (local_out) *local_out = local;
   ln.lp = local / nbp;
   ln.blk = local - ln.lp * nbp;
@@ -361,7 +365,7 @@ __global__ __launch_bounds__(LONG_THREADS) void k_long_p1(Ctx c, int dir, float 
       if (k1 < N1) ln.S[k1 * N2 + col] = (k1 && col) ? cmul(v[u], w[u]) : v[u];
     }
   }
-  if (MODE == LONG_DCT2_UPDATE) block_partial<2>(acc, c.partials2);
+  if (MODE == LONG_DCT2_UPDATE) block_partial<2>(acc, c.partials2, ln.gb);
 }
 
 // Pass 2.  LONG_DCT2: DCT-II coefficients from the spectrum at k and N - k -> dst lines.
@@ -479,7 +483,7 @@ __global__ __launch_bounds__(LONG_THREADS) void k_long_p2(Ctx c, int dir, float 
       }
     }
   }
-  if (!dct2 && dot) block_partial<1>(acc, c.partials2);
+  if (!dct2 && dot) block_partial<1>(acc, c.partials2, ln.gb);
 }
 
 // ---------------------------------------------------------------------------------------------
