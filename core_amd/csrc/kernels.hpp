@@ -1026,14 +1026,15 @@ constexpr int TRC_L_MIN = 8;
 constexpr int TRC_L_STEP = 4;
 constexpr int TRC_NL = (TRC_L - TRC_L_MIN) / TRC_L_STEP + 1;
 constexpr double TRI_TSMALL = 0x1p-60;
-// dynamic LDS bytes of k_col_trc: pivots (5 per column), then 2 (3 with the band transfers)
-// arrays of TRC_COLS x (TRC_CH + 1) doubles
+// dynamic LDS bytes of k_col_trc: pivot constants (TRC_NPIV per column), then 2 (3 with the band
+// transfers) arrays of TRC_COLS x (TRC_CH + 1) doubles
+constexpr int TRC_NPIV = 6;
 constexpr size_t trc_lds_bytes(bool transfer) {
-  return (5 * TRC_COLS + (transfer ? 3 : 2) * TRC_COLS * (TRC_CH + 1)) * sizeof(double);
+  return (TRC_NPIV * TRC_COLS + (transfer ? 3 : 2) * TRC_COLS * (TRC_CH + 1)) * sizeof(double);
 }
 
 struct TriPivots {
-  double em, q, eq2, theta, inv_last;
+  double em, ep, q, eq2, theta, inv_last;  // e^-theta, e^theta, e^-2theta, e^2theta
   __device__ __forceinline__ double t_at(int i) const { return exp(-(2.0 * i + 1.0) * theta); }
   // 1 / d_i for i < H - 1: e^-theta (1 + t) / (1 + e^-2theta t), 1 + e^-2theta t in [1, 2), by the
   // hardware reciprocal and two Newton steps
@@ -1121,6 +1122,7 @@ __device__ __forceinline__ TriPivots tri_pivots(int k, int W, int H) {
   const double mu = 4.0 * sn * sn;
   const double ch = 1.0 + 0.5 * mu, sh = sqrt(mu * (1.0 + 0.25 * mu));
   TriPivots tp;
+  tp.ep = ch + sh;
   tp.em = 1.0 / (ch + sh);
   tp.q = tp.em * tp.em;
   tp.eq2 = (ch + sh) * (ch + sh);
@@ -1180,7 +1182,7 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
   // column and chunk (end w or x, pivot product, and for TRANSFER the unit response)
   extern __shared__ double trc_lds[];
   double(*sPiv)[TRC_COLS] = reinterpret_cast<double(*)[TRC_COLS]>(trc_lds);
-  double(*sW)[TRC_CH + 1] = reinterpret_cast<double(*)[TRC_CH + 1]>(trc_lds + 5 * TRC_COLS);
+  double(*sW)[TRC_CH + 1] = reinterpret_cast<double(*)[TRC_CH + 1]>(trc_lds + TRC_NPIV * TRC_COLS);
   double(*sP)[TRC_CH + 1] = sW + TRC_COLS;
   double(*sR)[TRC_CH + 1] = sP + TRC_COLS;  // TRANSFER only
   if (tid < TRC_COLS) {
@@ -1190,6 +1192,7 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
     sPiv[2][tid] = t.eq2;
     sPiv[3][tid] = t.inv_last;
     sPiv[4][tid] = t.theta;
+    sPiv[5][tid] = t.ep;
   }
 
   // the chunk's rows (float4 of the thread's four columns; the row pitch is a multiple of 64 and
@@ -1216,7 +1219,7 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
   }
   __syncthreads();
   // the pivots of the thread's four columns, copied to registers once (the sweeps read no LDS)
-  double em[4], pq[4], peq2[4], plast[4], pth[4];
+  double em[4], pq[4], peq2[4], plast[4], pth[4], ep[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     em[i] = sPiv[0][4 * qd + i];
@@ -1224,6 +1227,7 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
     peq2[i] = sPiv[2][4 * qd + i];
     plast[i] = sPiv[3][4 * qd + i];
     pth[i] = sPiv[4][4 * qd + i];
+    ep[i] = sPiv[5][4 * qd + i];
   }
   // every chunk of the wave complete, without row H - 1, and t_s <= 2^-60 in every column of
   // every lane: a_i = e^-theta on the whole chunk, branch-free sweeps; otherwise every row is tested
@@ -1237,20 +1241,20 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
   const int kc = min(grp * TRC_COLS + cc, W - 1);
   const bool actc = grp * TRC_COLS + cc < W && grp * TRC_COLS + cc > 0;
 
-  // The sweeps branch per wave (converged or general); every barrier and chaining step below is
-  // reached by all threads of the block.
+  // The sweeps branch per wave and row (converged or general); every barrier and chaining step
+  // below is reached by all threads of the block.
   {
     double Pc[4] = {1.0, 1.0, 1.0, 1.0}, Rc[4] = {0.0, 0.0, 0.0, 0.0};
-    // The four sweeps each read what the previous one stored in v (never the same values twice, so
-    // the compiler keeps no conversions alive across sweeps):
-    //   F1  w0 from a zero carry (stored), with the chunk's pivot product and, for TRANSFER, R;
-    //   F3  w = w0 + (a_s ... a_u) w_in (stored);
-    //   B1  x0 = w + a x0 from a zero carry (stored), x0 at the chunk's first row returned;
-    //   B3  x = x0 + (a_u ... a_{e-1}) x_in, to the field.
-    // Rows outer, the four columns interleaved, a scheduling barrier per row (converting every row
-    // ahead of the recurrence costs two registers per value).  The pivot of a row is e^-theta in a
-    // converged wave (wave-uniform branch), else the closed form with t advanced per row; rows past
-    // a short chunk's end are skipped.
+    // Three sweeps, each reading what the previous one stored in v:
+    //   F   w0 from a zero carry (stored), with the chunk's pivot product and, for TRANSFER, R;
+    //   B   x0 from a zero carry (stored) on the true w = w0 + P_u w_in, x0 at the chunk's first
+    //       row returned.  P_u = a_s ... a_u = e^-(u+1)theta (1 + t_s) / (1 + t_{s+u+1}) below row
+    //       H - 1 (the closed-form pivots telescope); its last factor is the reciprocal the pivot
+    //       of row u needs anyway, and converged rows have P_u = e^-(u+1)theta;
+    //   C   x = x0 + (a_u ... a_{e-1}) x_in, to the field.
+    // Rows outer, the four columns interleaved, a scheduling barrier per row.  The pivot of a row
+    // is e^-theta in a converged wave, else the closed form with t advanced per row; rows past a
+    // short chunk's end are skipped.
     double tf[4], tb[4];
     auto t_reset = [&]() {
 #pragma unroll
@@ -1263,25 +1267,40 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
         }
       }
     };
-    // pivot of row u of column i; fw: forward order (t_i -> t_{i+1}), else backward
-    auto pivot = [&](int u, int i, bool fw) -> double {
-      if (conv) return em[i];
+    // closed-form pivot of row u of column i and r = 1 / (1 + t_{s+u+1}) (row H - 1: its own
+    // pivot, r unused); fw: forward order (t_i -> t_{i+1}), else backward
+    auto gpiv = [&](int u, int i, bool fw, double &r) -> double {
       double &t = fw ? tf[i] : tb[i];
-      TriPivots p;
-      p.em = em[i];
-      p.q = pq[i];
-      const double a = s + u == H - 1 ? plast[i] : p.inv(t);
+      double a = plast[i];
+      r = 1.0;
+      if (s + u != H - 1) {
+        const double x = fma(pq[i], t, 1.0);
+        double y = __builtin_amdgcn_rcp(x);
+        y = fma(y, fma(-x, y, 1.0), y);
+        y = fma(y, fma(-x, y, 1.0), y);
+        r = y;
+        a = em[i] * (1.0 + t) * y;
+      }
       t *= fw ? pq[i] : peq2[i];
       return a;
     };
-    auto f1 = [&](double (&w)[4]) {
+    auto sweep_f = [&](double (&w)[4]) {
       t_reset();
 #pragma unroll
       for (int u = 0; u < L; ++u) {
-        if (conv || u < len) {
+        if (conv) {
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            const double a = pivot(u, i, true);
+            w[i] = em[i] * ((double)v[u][i] + w[i]);
+            v[u][i] = (float)w[i];
+            if (TRANSFER) Rc[i] = fma(em[i] * Pc[i], Pc[i], Rc[i]);
+            Pc[i] *= em[i];
+          }
+        } else if (u < len) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            double r;
+            const double a = gpiv(u, i, true, r);
             w[i] = a * ((double)v[u][i] + w[i]);
             v[u][i] = (float)w[i];
             if (TRANSFER) Rc[i] = fma(a * Pc[i], Pc[i], Rc[i]);
@@ -1291,42 +1310,52 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
         __builtin_amdgcn_sched_barrier(0);
       }
     };
-    auto f3 = [&](double (&pr)[4]) {
+    auto sweep_b = [&](const double (&win)[4], double (&x)[4]) {
       t_reset();
+      // ge = w_in (1 + t_s) e^-(u+1)theta at the current row u (converged: (1 + t_s) = 1)
+      double ge[4];
 #pragma unroll
-      for (int u = 0; u < L; ++u) {
-        if (conv || u < len) {
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            pr[i] *= pivot(u, i, true);
-            v[u][i] = (float)((double)v[u][i] + pr[i]);
-          }
-        }
-        __builtin_amdgcn_sched_barrier(0);
-      }
-    };
-    auto b1 = [&](double (&x)[4]) {
-      t_reset();
+      for (int i = 0; i < 4; ++i)
+        ge[i] = conv ? Pc[i] * win[i] : win[i] * (1.0 + tf[i]) * exp(-(double)len * pth[i]);
 #pragma unroll
       for (int u = L - 1; u >= 0; --u) {
-        if (conv || u < len) {
+        if (conv) {
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            x[i] = fma(pivot(u, i, false), x[i], (double)v[u][i]);
+            x[i] = fma(em[i], x[i], (double)v[u][i] + ge[i]);
             v[u][i] = (float)x[i];
+            ge[i] *= ep[i];
+          }
+        } else if (u < len) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            double r;
+            const double a = gpiv(u, i, false, r);
+            const double g = s + u == H - 1 ? Pc[i] * win[i] : ge[i] * r;
+            x[i] = fma(a, x[i], (double)v[u][i] + g);
+            v[u][i] = (float)x[i];
+            ge[i] *= ep[i];
           }
         }
         __builtin_amdgcn_sched_barrier(0);
       }
     };
-    auto b3 = [&](double (&qr)[4]) {
+    auto sweep_c = [&](double (&qr)[4]) {
       t_reset();
 #pragma unroll
       for (int u = L - 1; u >= 0; --u) {
-        if (conv || u < len) {
+        if (conv) {
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            qr[i] *= pivot(u, i, false);
+            qr[i] *= em[i];
+            v[u][i] += (float)qr[i];
+          }
+          rowp[(size_t)u * P4] = make_float4(v[u][0], v[u][1], v[u][2], v[u][3]);
+        } else if (u < len) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            double r;
+            qr[i] *= gpiv(u, i, false, r);
             v[u][i] = (float)((double)v[u][i] + qr[i]);
           }
           rowp[(size_t)u * P4] = make_float4(v[u][0], v[u][1], v[u][2], v[u][3]);
@@ -1337,7 +1366,7 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
 
     // forward transfers, chained per column from the band's incoming w
     double w[4] = {0.0, 0.0, 0.0, 0.0};
-    f1(w);
+    sweep_f(w);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       sW[4 * qd + i][ch] = w[i];
@@ -1378,12 +1407,11 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
       }
     }
     __syncthreads();
-    // the true w of the chunk, then its backward transfer
+    // the chunk's incoming w, then its backward transfer on the true w
 #pragma unroll
     for (int i = 0; i < 4; ++i) w[i] = sW[4 * qd + i][ch];
-    f3(w);
     double x[4] = {0.0, 0.0, 0.0, 0.0};
-    b1(x);
+    sweep_b(w, x);
     __syncthreads();  // every thread has read its incoming w: sW now takes the x transfers
 #pragma unroll
     for (int i = 0; i < 4; ++i) sW[4 * qd + i][ch] = x[i];
@@ -1440,7 +1468,7 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < 4; ++i) x[i] = sW[4 * qd + i][ch];
-    b3(x);
+    sweep_c(x);
   }
 }
 
