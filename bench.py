@@ -11,7 +11,9 @@ Also reported:
                 algorithmic bytes per launch / its mean launch duration vs 8 TB/s HBM3E;
                 traffic = measured HBM bytes per launch of that kernel (rocprofv3 FETCH_SIZE and
                 WRITE_SIZE, copy-calibrated) from the committed profile, when present.
-  solve_model   whole-solve algorithmic bytes N [100 + 10 (109 + 50 * 76)] / step time.
+  solve_model   whole-solve algorithmic bytes N [100 + 10 (109 + 50 * 76)] / step time: the
+                reference algorithm's field traffic (76 B/px per PCG step with the x update
+                and the column pass as separate 24 and 8 B/px passes), an effective rate.
   batched       48 uneven SBAS-burst interferograms (H ~ U{1000..1500}, W ~ U{4000..6000},
                 seeds 1000 + i) solved in one batched graph launch vs 48 sequential plain calls
                 (--only-batched: this workload alone, with per-kernel durations).
@@ -45,9 +47,12 @@ METRIC = "Mpx/s unwrapped (10 IRLS iters) plain vs batched at ≤1e-3 rad; % HBM
 HBM_PEAK_GBS = 8000.0
 N_IRLS, N_CG = 10, 50
 # Algorithmic bytes per pixel of one launch of each PCG kernel (reads + writes of float32 fields).
+# Images whose row transform is a Cooley-Tukey length (8192^2, 40000^2) take the PCG step's
+# x += alpha p in the next step's k_cg_pap, which reads p anyway; Bluestein rows keep it in the
+# row DCT-II pass (ALG_BPX_STACK).
 ALG_BPX = {
-    "k_cg_pap": 24,           # z, p_old, wxi, wyi -> p_new, Ap
-    "k_row_dct2_update": 28,  # x, p, r, Ap -> x, r, T
+    "k_cg_pap": 32,           # z, p_old, wxi, wyi, x -> p_new, Ap, x
+    "k_row_dct2_update": 16,  # r, Ap -> r, T
     "k_col": 8,               # Tt -> Tt
     "k_col_direct": 8,        # T -> T, column pairs read and written in place
     "k_row_dct3": 12,         # T, r -> z
@@ -64,7 +69,7 @@ ALG_BPX = {
 # The same for the two-pass transforms of sides above 12288 (the 40000 x 40000 workload); the
 # scratch line holds one complex value per pixel pair, i.e. 4 B per pixel each way.
 ALG_BPX_LONG = {
-    "k_long_p1_dct2_update": 28,  # x, p, r, Ap -> x, r, scratch
+    "k_long_p1_dct2_update": 16,  # r, Ap -> r, scratch (x updated in k_cg_pap)
     "k_long_p2_dct2_rows": 8,     # scratch -> T
     "k_long_p1_dct2_cols": 8,     # Tt -> scratch
     "k_long_p2_dct2_cols": 8,     # scratch -> Tt
@@ -72,12 +77,14 @@ ALG_BPX_LONG = {
     "k_long_p2_dct3_cols": 8,     # scratch -> Tt
     "k_long_p1_dct3_rows": 8,     # T -> scratch
     "k_long_p2_dct3_rows": 12,    # scratch, r -> z
-    "k_cg_pap": 24,
+    "k_cg_pap": 32,
     "k_transpose": 8,
     # column pass of tall images as tridiagonal solves: bands of 1536 rows
     "k_col_trc_band": 4,          # T -> per-band transfers (a few values per column and band)
     "k_col_trc": 8,               # T -> T
 }
+# the 48-burst stack: most widths are Bluestein lengths, whose images keep the update in the row pass
+ALG_BPX_STACK = dict(ALG_BPX, k_cg_pap=24, k_row_dct2_update=28)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_8192.json")
 
 
@@ -190,8 +197,8 @@ def batched_stack(args, dev, kernels=False):
         kstats = {}
         for k, (tot, n) in sorted(_lib.kernel_stats().items(), key=lambda kv: -kv[1][0]):
             e = {"total_ms": round(tot, 1), "launches": n, "mean_us": round(tot / n * 1e3, 1)}
-            if k in ALG_BPX:
-                e["gbs"] = round(ALG_BPX[k] * npx / (tot / n / 1e3) / 1e9, 1)
+            if k in ALG_BPX_STACK:
+                e["gbs"] = round(ALG_BPX_STACK[k] * npx / (tot / n / 1e3) / 1e9, 1)
             kstats[k] = e
     ws.release()
     del ws, ph, co, out

@@ -163,6 +163,9 @@ struct ImgDesc {
   int tri_nb;      // tridiagonal solves: bands of TRC_CH chunks the columns are cut into (1: one pass)
   long long tri_off;  // double offset of the image's band carries (6 x tri_nb x W) when tri_nb > 1
   long long tri_dc;   // double offset of the image's column-0 input (H floats) for the singular solve
+  int xdefer;      // 1: the PCG step's x += alpha p runs in the next step's k_cg_pap (or k_cg_xcommit)
+                   // instead of the row DCT-II pass (images whose row transform is bandwidth-bound)
+  int pad_;
 };
 
 // Per-image solver state.  Written by k_state_init and k_finalize (one block per image), read by
@@ -182,7 +185,8 @@ struct ImgState {
   int improved_it;  // IRLS iteration whose u became u_best
   int restore_it;   // IRLS iteration that restored u_best after a non-finite u
   int status;
-  int pad0, pad1;
+  int xpend;  // xdefer images: x += alpha p of the last PCG step is still to be applied
+  int pad1;
 };
 
 struct Fields {

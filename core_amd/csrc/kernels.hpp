@@ -104,7 +104,7 @@ __global__ void k_state_init(Ctx c) {
     s.irls_active = 1; s.cg_active = 0;
     s.n_cg = 0; s.n_cg_total = 0; s.n_irls = 0;
     s.improved_it = -2; s.restore_it = -2; s.status = 0;
-    s.pad0 = 0; s.pad1 = 0;
+    s.xpend = 0; s.pad1 = 0;
     c.st[i] = s;
   }
 }
@@ -400,6 +400,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_cg_init(Ctx c) {
 
 // p = z (first iteration) or z + beta p; Ap = A p; p.Ap  (utils_unwrap2d.py:1429-1440, :1473).
 // float4 loads along rows, vertical neighbours kept in registers while marching down the strip.
+// For an xdefer image with a pending update, first the previous step's x += alpha p_old
+// (utils_unwrap2d.py:1444-1449), with the count of non-finite x: this pass reads p_old anyway, so
+// the update costs 8 B/px here instead of 12 in the row DCT-II pass.
 __global__ __launch_bounds__(TILE_THREADS) void k_cg_pap(Ctx c, const float *__restrict__ p_old,
                                                           float *__restrict__ p_new) {
   const TilePos t = tile_pos(c);
@@ -408,6 +411,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_cg_pap(Ctx c, const float *__r
   if (!st->cg_active) return;
   const bool first = st->n_cg == 0;
   const float beta = (float)st->beta;
+  const bool upd = d.xdefer && st->xpend;
+  const float alpha = (float)st->alpha;
+  float *__restrict__ x = c.f.u;
   const float *__restrict__ z = c.f.z;
   float *__restrict__ Ap = c.f.Ap;
   const int c0 = t.j0 + STRIP * (threadIdx.x >> 6) + 4 * (threadIdx.x & 63);
@@ -425,13 +431,36 @@ __global__ __launch_bounds__(TILE_THREADS) void k_cg_pap(Ctx c, const float *__r
     const long long g = d.off + (long long)i * d.P + j;
     return first ? z[g] : pn(z[g], p_old[g]);
   };
-  double acc[1] = {0.0};
-  march_rows(c, d, t, X, XE, [&](int i, long long g, float4 x, float4 ax) {
-    st4(p_new, g, x);
+  double acc[2] = {0.0, 0.0};  // p.Ap, non-finite x
+  march_rows(c, d, t, X, XE, [&](int i, long long g, float4 pv, float4 ax) {
+    st4(p_new, g, pv);
     st4(Ap, g, ax);
-    acc[0] += (double)x.x * ax.x + (double)x.y * ax.y + (double)x.z * ax.z + (double)x.w * ax.w;
+    acc[0] += (double)pv.x * ax.x + (double)pv.y * ax.y + (double)pv.z * ax.z + (double)pv.w * ax.w;
+    if (upd) {
+      const float4 po = ld4(p_old, g), xo = ld4(x, g);
+      const float4 xn = make_float4(xo.x + alpha * po.x, xo.y + alpha * po.y, xo.z + alpha * po.z, xo.w + alpha * po.w);
+      st4(x, g, xn);
+      const float xs[4] = {xn.x, xn.y, xn.z, xn.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (c0 + k < d.W) acc[1] += isfinite(xs[k]) ? 0.0 : 1.0;
+    }
   });
-  block_partial<1>(acc, c.partials, t.blk);
+  block_partial<2>(acc, c.partials, t.blk);
+}
+
+// The update x += alpha p still pending after the last PCG step of an xdefer image
+// (utils_unwrap2d.py:1444-1449), from the p buffer that step wrote.
+__global__ __launch_bounds__(TILE_THREADS) void k_cg_xcommit(Ctx c) {
+  const TilePos t = tile_pos(c);
+  const ImgDesc d = c.img[t.img];
+  const ImgState *st = &c.st[t.img];
+  if (!(d.xdefer && st->xpend && st->irls_active)) return;
+  const float alpha = (float)st->alpha;
+  // step i (0-based) writes p1 when i is even, p0 when odd; n_cg = i + 1
+  const float *__restrict__ p = ((st->n_cg - 1) & 1) ? c.f.p0 : c.f.p1;
+  float *__restrict__ x = c.f.u;
+  for_tile(d, t, [&](int i, int j, long long idx) { x[idx] = x[idx] + alpha * p[idx]; });
 }
 
 #endif  // IRLS2D_TRANSFORMS_ONLY
@@ -656,6 +685,7 @@ __global__ __launch_bounds__(fft_threads_max<E>()) void k_row_dct2(Ctx c, float 
     float *x = c.f.u;
     const float *Ap = c.f.Ap;
     const float alpha = (float)st->alpha;
+    const bool upx = !d.xdefer;  // xdefer: x is updated by the next k_cg_pap instead
     const int T = blockDim.x;
     // (BLUE: the chirps of the batch's Makhoul slots are loaded with it)
     auto update = [&](auto blue_tag) {
@@ -674,8 +704,10 @@ __global__ __launch_bounds__(fft_threads_max<E>()) void k_row_dct2(Ctx c, float 
             xv[u][h] = pv[u][h] = rv[u][h] = av[u][h] = 0.f;
             if (m < W && (h == 0 || hasb)) {
               const long long k = oa + (long long)h * d.P + m;
-              xv[u][h] = x[k];
-              pv[u][h] = p[k];
+              if (upx) {
+                xv[u][h] = x[k];
+                pv[u][h] = p[k];
+              }
               rv[u][h] = in_[k];
               av[u][h] = Ap[k];
             }
@@ -693,10 +725,12 @@ __global__ __launch_bounds__(fft_threads_max<E>()) void k_row_dct2(Ctx c, float 
             rn2[h] = rn;
             if (h == 0 || hasb) {
               const long long k = oa + (long long)h * d.P + m;
-              x[k] = xn;
+              if (upx) {
+                x[k] = xn;
+                acc[1] += isfinite(xn) ? 0.0 : 1.0;
+              }
               in_[k] = rn;
               acc[0] += (double)rn * rn;
-              acc[1] += isfinite(xn) ? 0.0 : 1.0;
             }
           }
           float2 z = make_float2(rn2[0], hasb ? rn2[1] : 0.f);
@@ -1570,7 +1604,7 @@ __global__ __launch_bounds__(256) void k_finalize(Ctx c, int kind, int it) {
     b1 = geo[img + 1];
   }
   int nv = 1;
-  if (kind == FIN_IRLS_UPDATE || kind == FIN_UPDATE) nv = 2;
+  if (kind == FIN_IRLS_UPDATE || kind == FIN_UPDATE || kind == FIN_PAP) nv = 2;
   if (kind == FIN_IRLS_CHECK) nv = 3;
   double acc[MAX_NV] = {0.0, 0.0, 0.0};
   // partials b0 + tid, + 256, + 512, ... summed in that order; loaded FIN_U at a time so that
@@ -1613,15 +1647,27 @@ __global__ __launch_bounds__(256) void k_finalize(Ctx c, int kind, int it) {
     case FIN_CG_INIT:
       st->cg_active = acc[0] == 0.0 ? 1 : 0;
       st->n_cg = 0;
+      st->xpend = 0;
       break;
     case FIN_PAP: {
+      if (st->xpend) {
+        // k_cg_pap applied the previous step's x update: a non-finite x ends PCG after that step,
+        // as the reference's test right after its update does (utils_unwrap2d.py:1451-1453)
+        st->xpend = 0;
+        if (acc[1] != 0.0) {
+          st->cg_active = 0;
+          break;
+        }
+      }
       const double pAp = acc[0];
       st->n_cg += 1;
       st->n_cg_total += 1;
-      if (!(fabs(pAp) >= 1e-15) || !isfinite(pAp))
+      if (!(fabs(pAp) >= 1e-15) || !isfinite(pAp)) {
         st->cg_active = 0;
-      else
+      } else {
         st->alpha = clampd_nan(st->rz / pAp, -1e6, 1e6);
+        st->xpend = dsc.xdefer;
+      }
       break;
     }
     case FIN_UPDATE:
@@ -1645,6 +1691,7 @@ __global__ __launch_bounds__(256) void k_finalize(Ctx c, int kind, int it) {
       break;
     case FIN_IRLS_CHECK:
       st->cg_active = 0;
+      st->xpend = 0;  // k_cg_xcommit of this step has run
       st->n_irls = it + 1;
       if (acc[0] != 0.0) {
         st->restore_it = it;

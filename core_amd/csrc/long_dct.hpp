@@ -241,6 +241,7 @@ __global__ __launch_bounds__(LONG_THREADS) void k_long_p1(Ctx c, int dir, float 
   const int n1_end = col >= 0 ? N1 / 2 + 1 : 0;
   if (dct2) {
     const float alpha = (MODE == LONG_DCT2_UPDATE) ? (float)c.st[ln.img].alpha : 0.f;
+    const bool upx = !c.img[ln.img].xdefer;  // xdefer: x is updated by the next k_cg_pap instead
     float *x = c.f.u;
     const float *Ap = c.f.Ap;
     for (int n0 = threadIdx.x >> lb; n0 < n1_end; n0 += U * rstep) {
@@ -257,8 +258,10 @@ __global__ __launch_bounds__(LONG_THREADS) void k_long_p1(Ctx c, int dir, float 
             const long long e = ln.la + (long long)h * ln.pitch + m;
             v[u][h] = ld2(src, e);
             if (MODE == LONG_DCT2_UPDATE) {
-              xv[u][h] = ld2(x, e);
-              pv[u][h] = ld2(p, e);
+              if (upx) {
+                xv[u][h] = ld2(x, e);
+                pv[u][h] = ld2(p, e);
+              }
               av[u][h] = ld2(Ap, e);
             }
           }
@@ -276,12 +279,14 @@ __global__ __launch_bounds__(LONG_THREADS) void k_long_p1(Ctx c, int dir, float 
           for (int h = 0; h < 2; ++h) {
             if (h == 1 && !ln.hasb) break;
             const long long e = ln.la + (long long)h * ln.pitch + m;
-            const float2 xn = make_float2(xv[u][h].x + alpha * pv[u][h].x, xv[u][h].y + alpha * pv[u][h].y);
             const float2 rv = make_float2(v[u][h].x - alpha * av[u][h].x, v[u][h].y - alpha * av[u][h].y);
-            st2(x, e, xn, two);
+            if (upx) {
+              const float2 xn = make_float2(xv[u][h].x + alpha * pv[u][h].x, xv[u][h].y + alpha * pv[u][h].y);
+              st2(x, e, xn, two);
+              acc[1] += (isfinite(xn.x) ? 0.0 : 1.0) + ((two && !isfinite(xn.y)) ? 1.0 : 0.0);
+            }
             st2(src, e, rv, two);
             acc[0] += (double)rv.x * rv.x + (two ? (double)rv.y * rv.y : 0.0);
-            acc[1] += (isfinite(xn.x) ? 0.0 : 1.0) + ((two && !isfinite(xn.y)) ? 1.0 : 0.0);
             v[u][h] = rv;
           }
         }
@@ -517,6 +522,7 @@ __global__ __launch_bounds__(LONG_THREADS) void k_lb_pack(Ctx c, int dir, float 
   if (dct2) {
     // element m of both lines -> Makhoul slot makhoul_pos(m); the update first
     const float alpha = (MODE == LONG_DCT2_UPDATE) ? (float)c.st[ln.img].alpha : 0.f;
+    const bool upx = !c.img[ln.img].xdefer;  // xdefer: x is updated by the next k_cg_pap instead
     float *x = c.f.u;
     const float *Ap = c.f.Ap;
     const int e0 = ln.blk * LB_CHUNK, e1 = min(e0 + LB_CHUNK, N);
