@@ -1861,6 +1861,17 @@ int gaussian(int64_t ny, int64_t nx, int channels, const float *data, const floa
   g.lim = weight ? 1e-10f : (float)threshold;
   hipStream_t s = (hipStream_t)stream;
   LAUNCH("k_gauss_taps", k_gauss_taps, dim3(1), dim3(GAUSS_THREADS), 0, s, g);
+  if (ry <= 8 && rx <= 64) {
+    // one pass (no scratch): strips of GAUSS_THREADS - 2 rx output columns, bands of GAUSS_TY rows
+    const long long nstrip = (nx + GAUSS_THREADS - 2 * rx - 1) / (GAUSS_THREADS - 2 * rx);
+    const long long nband = (ny + GAUSS_TY - 1) / GAUSS_TY;
+    if (nstrip * nband >= ((long long)1 << 31)) return fail(IRLS2D_EINVAL, "gaussian: image too large");
+    const dim3 gf((unsigned)(nstrip * nband));
+    if (channels == 1) LAUNCH("k_gauss_fused", (k_gauss_fused<1, 8>), gf, dim3(GAUSS_THREADS), 0, s, g);
+    else LAUNCH("k_gauss_fused", (k_gauss_fused<2, 8>), gf, dim3(GAUSS_THREADS), 0, s, g);
+    HIP_TRY(hipGetLastError());
+    return IRLS2D_OK;
+  }
   const long long n = ny * nx;
   const dim3 grid((unsigned)std::min<long long>((n + GAUSS_THREADS - 1) / GAUSS_THREADS, 1 << 20));
   if (channels == 1) {

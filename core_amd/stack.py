@@ -157,7 +157,7 @@ def unwrap2d_irls(phase, weight=None, device='auto', max_iter=50, tol=1e-2, cg_m
 
 def unwrap2d(phase, weight=None, conncomp=False, conncomp_size=1_000, conncomp_gap=None, conncomp_linksize=5,
              conncomp_linkcount=30, device='auto', debug=False, xr=None, **kwargs):
-    """Stack_unwrap2d.unwrap2d (:645-759) over a burst mapping {key: xr.Dataset}.
+    """Stack_unwrap2d.unwrap2d (:525-643) over a burst mapping {key: xr.Dataset}.
 
     conncomp=True returns (unwrapped, conncomp).  conncomp=False (the default) links each grid's
     components as the reference's `_link_components` does (Stack_unwrap2d.py:293-454): components
@@ -197,7 +197,7 @@ def unwrap2d(phase, weight=None, conncomp=False, conncomp_size=1_000, conncomp_g
 
 def unwrap2d_dataset(phase, weight=None, conncomp=False, conncomp_size=1000, conncomp_gap=None,
                      conncomp_linksize=5, conncomp_linkcount=30, device='auto', debug=False, xr=None, **kwargs):
-    """Stack_unwrap2d.unwrap2d_dataset (:525-643): one merged xr.Dataset in, Dataset(s) out."""
+    """Stack_unwrap2d.unwrap2d_dataset (:645-751): one merged xr.Dataset in, Dataset(s) out."""
     xr = _xr(xr)
     if not isinstance(phase, xr.Dataset):
         raise TypeError(f"phase must be xr.Dataset, got {type(phase).__name__}")

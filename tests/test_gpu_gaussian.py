@@ -36,6 +36,7 @@ def _data(ny, nx, cplx, seed):
     (150, 170, False, False, 2.0, 0.5), (150, 170, True, False, 2.0, 0.5),
     (131, 97, False, True, (1.5, 3.0), 0.5), (131, 97, True, True, 0.7, 0.5),
     (200, 90, False, False, 10.0, 0.3), (64, 300, True, False, (4.0, 0.5), 0.8),
+    (600, 530, True, True, 2.0, 0.5), (517, 261, False, False, (2.0, 16.0), 0.5),
 ])
 def test_nanconv_matches_fp64_oracle(hip, ny, nx, cplx, weighted, sigma, thr):
     from core_amd.gaussian import nanconvolve2d_gaussian
