@@ -285,8 +285,9 @@ __device__ __forceinline__ float4 mask4(float4 v, int c0, int W) {
 }
 
 // Row-marching driver of the float4 stencils: wave w walks the strip j0 + 256 w of its tile, one
-// row at a time; lane l owns columns c0 = strip + 4l .. c0 + 3.  X(i) loads the operand at
-// (i, c0..c0+3) (zero outside the image), XE(i, j) one operand value (zero outside), E(i, ...)
+// row at a time; lane l owns columns c0 = strip + 4l .. c0 + 3.  X(i, aux) loads the operand at
+// (i, c0..c0+3) (zero outside the image) and may leave four more values of that row in aux, which
+// E receives with the row; XE(i, j) loads one operand value (zero outside), E(i, g, x, Ax, aux)
 // consumes row i.  Lanes with c0 >= W own no pixel: they touch no memory (c0 < W keeps c0 + 3
 // inside the row pitch, a multiple of 64) and only feed zeros to the shuffles.
 // Three rows of the operand and the y-weights between the current row and the one behind stay in
@@ -296,7 +297,7 @@ __device__ __forceinline__ float4 mask4(float4 v, int c0, int W) {
 // same time on the same XCD) read in step with this one, so they are L2 hits.  For the same
 // reason even tile rows march down and odd ones up: the halo rows two vertically adjacent tiles
 // share are then read by both near their starts, or both near their ends.
-template <bool UP, class LX, class LXE, class EMIT>
+template <bool UP, class A, class LX, class LXE, class EMIT>
 __device__ __forceinline__ void march_dir(const Ctx &c, const ImgDesc &d, const TilePos &t, LX &&X, LXE &&XE,
                                           EMIT &&E) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -324,7 +325,8 @@ __device__ __forceinline__ void march_dir(const Ctx &c, const ImgDesc &d, const 
   };
   // xb: the row behind the march (above when marching down), xc: the current row; wb: the
   // y-weights between them (row i - 1 down, row i up)
-  float4 xb = X(ifirst - step), xc = X(ifirst);
+  A aux_b, aux_c, aux_a;  // aux of the rows behind (unused), current, ahead
+  float4 xb = X(ifirst - step, aux_b), xc = X(ifirst, aux_c);
   const int iwb = UP ? ifirst : ifirst - 1;
   float4 wb = (own && iwb >= 0) ? ld4(wyi, d.off + (long long)iwb * d.P + c0) : z4;
   float ex, ew;
@@ -332,7 +334,7 @@ __device__ __forceinline__ void march_dir(const Ctx &c, const ImgDesc &d, const 
   for (int i = ifirst;; i += step) {
     const long long g = d.off + (long long)i * d.P + c0;
     const int ia = i + step;
-    const float4 xa = X(ia);
+    const float4 xa = X(ia, aux_a);
     const int iwf = UP ? i - 1 : i;
     const float4 wx = own ? ld4(wxi, g) : z4;
     const float4 wf = (own && iwf >= 0) ? ld4(wyi, d.off + (long long)iwf * d.P + c0) : z4;
@@ -348,22 +350,23 @@ __device__ __forceinline__ void march_dir(const Ctx &c, const ImgDesc &d, const 
     if (lane == 63) xr = ex;
     const float4 ax = UP ? apply_A4(i, c0, H, W, xc, xl, xr, xa, xb, wx, wxl, wf, wb)
                          : apply_A4(i, c0, H, W, xc, xl, xr, xb, xa, wx, wxl, wb, wf);
-    if (own) E(i, g, xc, ax);
+    if (own) E(i, g, xc, ax, aux_c);
     if (i == ilast) break;
     xb = xc;
     xc = xa;
+    aux_c = aux_a;
     wb = wf;
     ex = exa;
     ew = ewa;
   }
 }
 
-template <class LX, class LXE, class EMIT>
+template <class A = float4, class LX, class LXE, class EMIT>
 __device__ __forceinline__ void march_rows(const Ctx &c, const ImgDesc &d, const TilePos &t, LX &&X, LXE &&XE,
                                            EMIT &&E) {
   if (t.j0 + STRIP * (int)(threadIdx.x >> 6) >= d.W) return;
-  if ((t.i0 / TILE_H) & 1) march_dir<true>(c, d, t, X, XE, E);
-  else march_dir<false>(c, d, t, X, XE, E);
+  if ((t.i0 / TILE_H) & 1) march_dir<true, A>(c, d, t, X, XE, E);
+  else march_dir<false, A>(c, d, t, X, XE, E);
 }
 
 // PCG start: r = b - A x (x = u); a non-finite r returns x unchanged (utils_unwrap2d.py:1414-1419).
@@ -376,14 +379,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_cg_init(Ctx c) {
     const float *__restrict__ b = c.f.b;
     float *__restrict__ r = c.f.r;
     const int c0 = t.j0 + STRIP * (threadIdx.x >> 6) + 4 * (threadIdx.x & 63);
-    auto X = [&](int i) -> float4 {
+    auto X = [&](int i, float4 &) -> float4 {
       if (i < 0 || i >= d.H || c0 >= d.W) return make_float4(0.f, 0.f, 0.f, 0.f);
       return mask4(ld4(u, d.off + (long long)i * d.P + c0), c0, d.W);
     };
     auto XE = [&](int i, int j) -> float {
       return (j >= 0 && j < d.W) ? u[d.off + (long long)i * d.P + j] : 0.f;
     };
-    march_rows(c, d, t, X, XE, [&](int i, long long g, float4 x, float4 ax) {
+    march_rows(c, d, t, X, XE, [&](int i, long long g, float4 x, float4 ax, float4) {
       const float4 bb = ld4(b, g);
       const float4 rr = make_float4(bb.x - ax.x, bb.y - ax.y, bb.z - ax.z, bb.w - ax.w);
       st4(r, g, rr);
@@ -418,12 +421,22 @@ __global__ __launch_bounds__(TILE_THREADS) void k_cg_pap(Ctx c, const float *__r
   float *__restrict__ Ap = c.f.Ap;
   const int c0 = t.j0 + STRIP * (threadIdx.x >> 6) + 4 * (threadIdx.x & 63);
   auto pn = [&](float zv, float pv) { return first ? zv : __fadd_rn(__fmul_rn(pv, beta), zv); };
-  auto X = [&](int i) -> float4 {
-    if (i < 0 || i >= d.H || c0 >= d.W) return make_float4(0.f, 0.f, 0.f, 0.f);
+  // the row's p_old and, with a pending update, its x travel with the row to E (one load each)
+  struct Aux {
+    float4 p, x;
+  };
+  auto X = [&](int i, Aux &aux) -> float4 {
+    const float4 z0 = make_float4(0.f, 0.f, 0.f, 0.f);
+    aux.p = aux.x = z0;
+    if (i < 0 || i >= d.H || c0 >= d.W) return z0;
     const long long g = d.off + (long long)i * d.P + c0;
     const float4 z4 = ld4(z, g);
     if (first) return mask4(z4, c0, d.W);
     const float4 p4 = ld4(p_old, g);
+    if (upd && i >= t.i0 && i < t.i0 + TILE_H) {  // rows of the tile (not its halo rows)
+      aux.p = p4;
+      aux.x = ld4(x, g);
+    }
     return mask4(make_float4(pn(z4.x, p4.x), pn(z4.y, p4.y), pn(z4.z, p4.z), pn(z4.w, p4.w)), c0, d.W);
   };
   auto XE = [&](int i, int j) -> float {
@@ -432,12 +445,12 @@ __global__ __launch_bounds__(TILE_THREADS) void k_cg_pap(Ctx c, const float *__r
     return first ? z[g] : pn(z[g], p_old[g]);
   };
   double acc[2] = {0.0, 0.0};  // p.Ap, non-finite x
-  march_rows(c, d, t, X, XE, [&](int i, long long g, float4 pv, float4 ax) {
+  march_rows<Aux>(c, d, t, X, XE, [&](int i, long long g, float4 pv, float4 ax, const Aux &aux) {
     st4(p_new, g, pv);
     st4(Ap, g, ax);
     acc[0] += (double)pv.x * ax.x + (double)pv.y * ax.y + (double)pv.z * ax.z + (double)pv.w * ax.w;
     if (upd) {
-      const float4 po = ld4(p_old, g), xo = ld4(x, g);
+      const float4 po = aux.p, xo = aux.x;
       const float4 xn = make_float4(xo.x + alpha * po.x, xo.y + alpha * po.y, xo.z + alpha * po.z, xo.w + alpha * po.w);
       st4(x, g, xn);
       const float xs[4] = {xn.x, xn.y, xn.z, xn.w};
