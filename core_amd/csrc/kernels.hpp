@@ -701,10 +701,13 @@ __global__ __launch_bounds__(fft_threads_max<E>()) void k_row_dct2(Ctx c, float 
     const bool upx = !d.xdefer;  // xdefer: x is updated by the next k_cg_pap instead
     const int T = blockDim.x;
     // (BLUE: the chirps of the batch's Makhoul slots are loaded with it)
-    auto update = [&](auto blue_tag) {
+    // (DEFER: x is updated by the next k_cg_pap; two loads per element instead of four, so twice
+    // as many elements are in flight per batch)
+    auto update = [&](auto blue_tag, auto defer_tag) {
       constexpr bool BLUE = decltype(blue_tag)::value;
+      constexpr bool DEFER = decltype(defer_tag)::value;
       const float2 *__restrict__ ch = D.tab + D.P->chirp_off;
-      constexpr int U = 4;
+      constexpr int U = DEFER ? 8 : 4;
       for (int m0 = threadIdx.x; m0 < W; m0 += U * T) {
         float xv[U][2], pv[U][2], rv[U][2], av[U][2];
         float2 cm[U];
@@ -717,7 +720,7 @@ __global__ __launch_bounds__(fft_threads_max<E>()) void k_row_dct2(Ctx c, float 
             xv[u][h] = pv[u][h] = rv[u][h] = av[u][h] = 0.f;
             if (m < W && (h == 0 || hasb)) {
               const long long k = oa + (long long)h * d.P + m;
-              if (upx) {
+              if (!DEFER) {
                 xv[u][h] = x[k];
                 pv[u][h] = p[k];
               }
@@ -738,7 +741,7 @@ __global__ __launch_bounds__(fft_threads_max<E>()) void k_row_dct2(Ctx c, float 
             rn2[h] = rn;
             if (h == 0 || hasb) {
               const long long k = oa + (long long)h * d.P + m;
-              if (upx) {
+              if (!DEFER) {
                 x[k] = xn;
                 acc[1] += isfinite(xn) ? 0.0 : 1.0;
               }
@@ -752,8 +755,13 @@ __global__ __launch_bounds__(fft_threads_max<E>()) void k_row_dct2(Ctx c, float 
         }
       }
     };
-    if (D.P->blue) update(BoolC<true>{});
-    else update(BoolC<false>{});
+    if (D.P->blue) {
+      if (upx) update(BoolC<true>{}, BoolC<false>{});
+      else update(BoolC<true>{}, BoolC<true>{});
+    } else {
+      if (upx) update(BoolC<false>{}, BoolC<false>{});
+      else update(BoolC<false>{}, BoolC<true>{});
+    }
   } else {
     if (D.P->blue) lines_load<COL_U, true, false>(D, in_ + oa, d.P, hasb, W);
     else lines_load<COL_U, false, false>(D, in_ + oa, d.P, hasb, W);
