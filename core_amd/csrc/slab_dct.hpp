@@ -392,10 +392,9 @@ __global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
   if (!slab_block(c, c.geo.slab2, guard, b)) return;
   const SlabPlan &L = *b.L;
   const int N = L.N, N1 = L.N1, W = b.d->W;
-  float2 *tws = lds, *buf = lds + TW_LDS;
+  float2 *buf = lds + TW_LDS;  // (the first TW_LDS slots: the twiddle table of K1 / K3, unused here)
   float2 *mkT = buf + N2 * SB, *rt = mkT + 2 * N2, *ft = rt + N2;
   float *sT = (float *)(ft + 2 * N2);
-  load_tws_len(tws, c.tab + L.tw_off, N);
   const int s = threadIdx.x & (SB - 1), t = threadIdx.x / SB;
   const int cp = s & (SLAB_CP - 1), sig = s >> 5;
   const int k1a = b.grp, k1b = b.grp ? N1 - b.grp : 0;
@@ -418,6 +417,13 @@ __global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
   // folds the DCT-III pre-step's factor 1/2 into the reciprocal.
   const int tstep = N / N2;
   {
+    // exp(-2 pi i m / N) from sincospif: N is a power of two here, so 2 m / N is exact in float32
+    // (no table read before the first barrier)
+    auto expi = [&](int m) {
+      float sn, cs;
+      sincospif(2.f * (float)m / (float)N, &sn, &cs);
+      return make_float2(cs, -sn);
+    };
     const float2 *mk = c.tab + L.mk_off;
     const double *cosH = c.dtab + b.d->cosH_off;
     for (int e = threadIdx.x; e < 2 * N2; e += blockDim.x) {
@@ -425,8 +431,8 @@ __global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
       const int k = k1e + N1 * kk;
       mkT[e] = mk[k];
       sT[e] = (float)(4.0 - 4.0 * cosH[k]);
-      ft[e] = twiddle(tws, k1e * kk);
-      if (e < N2) rt[e] = twiddle(tws, e * tstep);
+      ft[e] = expi(k1e * kk);
+      if (e < N2) rt[e] = expi(e * tstep);
     }
   }
   __syncthreads();
