@@ -1056,13 +1056,13 @@ __global__ __launch_bounds__(fft_threads_max<E>()) __attribute__((amdgpu_waves_p
 //
 // Both recurrences are first-order linear, so a run of rows [s, e) maps its incoming carry
 // linearly: w_{e-1} = w0_{e-1} + (prod_{s<=j<e} a_j) w_{s-1} and x_s = x0_s + (prod a_j) x_e, with
-// w0 / x0 the results from a zero carry.  k_col_trc gives each thread four adjacent columns of a
-// chunk of L consecutive rows in registers (float4 loads and stores: a wave instruction moves eight
-// 128-byte row segments); a block covers TRC_COLS columns and a band of TRC_CH chunks.  A sweep
-// runs once for the chunk's transfer (end value and pivot product), the TRC_CH transfers of each
-// column are chained in LDS (4 chunks per thread, then an affine-map scan over 32 lanes), and the
-// sweep runs again from the true carry.  Forward and backward each take one such round; the field
-// is read and written once (8 B/px).  Columns taller than one band of TRC_CH x TRC_L rows are cut
+// w0 / x0 the results from a zero carry.  k_col_trc gives each thread TRC_CPT adjacent columns of
+// a chunk of L consecutive rows in registers (vector row loads and stores: a wave instruction moves
+// eight row segments of TRC_COLS floats); a block covers TRC_COLS columns and a band of TRC_CH
+// chunks.  Sweep F runs from a zero carry (the chunk's end value and pivot product), the TRC_CH
+// transfers of each column are chained in LDS (TRC_PER chunks per thread, then an affine-map scan
+// over TRC_SEG lanes), sweep B runs backward on the true w, a second chain gives each chunk its
+// incoming x, and sweep C corrects x and writes it; the field is read and written once (8 B/px).  Columns taller than one band of TRC_CH x TRC_L rows are cut
 // into bands: k_col_trc<L, true> stores each band's transfers with zero outer carries (end w,
 // pivot product, x at the band's first row and that x's response to a unit incoming w),
 // k_col_trc_carry chains the bands of each column, and k_col_trc<L, false> solves every band from
@@ -1070,10 +1070,22 @@ __global__ __launch_bounds__(fft_threads_max<E>()) __attribute__((amdgpu_waves_p
 // pass keeps its input, and k_col_trc_dc writes the singular solve over it.
 
 constexpr int TRI_U = 16;  // rows per load batch of the singular column
-constexpr int TRC_COLS = 32;     // columns per block: 8 lanes of four columns
+constexpr int TRC_CPT = 2;       // columns per thread (float2 row accesses)
+constexpr int TRC_COLS = 16;     // columns per block: 8 lanes of TRC_CPT columns
 constexpr int TRC_CH = 64;       // chunks per column and band
-constexpr int TRC_THREADS = TRC_COLS / 4 * TRC_CH;  // 512: two waves per SIMD, up to 256 VGPRs
-constexpr int TRC_SEG = TRC_CH / 4;                 // chaining: 4 chunks per thread, a TRC_SEG-lane scan
+constexpr int TRC_THREADS = TRC_COLS / TRC_CPT * TRC_CH;  // 512 = two waves per SIMD per block
+// chaining: TRC_PER chunks composed per thread, then a TRC_SEG-lane scan
+constexpr int TRC_PER = TRC_COLS * TRC_CH / TRC_THREADS;
+constexpr int TRC_SEG = TRC_CH / TRC_PER;
+static_assert(TRC_SEG <= 64 && (TRC_SEG & (TRC_SEG - 1)) == 0, "scan segments within a wave");
+// the thread's TRC_CPT columns of one row as one vector access
+typedef float TrcVec __attribute__((ext_vector_type(TRC_CPT)));
+__device__ __forceinline__ TrcVec trc_vec(const float (&v)[TRC_CPT]) {
+  TrcVec r;
+#pragma unroll
+  for (int i = 0; i < TRC_CPT; ++i) r[i] = v[i];
+  return r;
+}
 constexpr int TRC_L = 24;        // rows per chunk of a band: bands of 1536 rows
 // one-band columns: chunks of TRC_L_MIN, TRC_L_MIN + TRC_L_STEP, ... TRC_L rows, the shortest that
 // covers the column in TRC_CH chunks (a kernel instantiation and a launch per length)
@@ -1201,27 +1213,29 @@ __device__ __forceinline__ void affine_scan_seg(double &a, double &b, int l) {
   }
 }
 
-// Block of TRC_COLS columns 32 g .. 32 g + 31 and one band of TRC_CH chunks of L rows (the image's
-// blocks: g fastest, then the band).  Thread t holds columns 32 g + 4 (t mod 8) + i (i < 4) of
-// chunk t / 8.  Band carries, [6][nb][W] doubles from tri_off: the band's end w, pivot product,
-// first-row x and unit response (written by TRANSFER), its incoming w and x (written by
-// k_col_trc_carry).  Columns from W to the next multiple of 32 lie in the row padding (the pitch
-// is a multiple of 64): they are solved with meaningless values that no reader uses.
+// Block of TRC_COLS columns C g .. C g + C - 1 (C = TRC_COLS) and one band of TRC_CH chunks of L
+// rows (the image's logical blocks, numbered XCD-major: g fastest, then the band).  Thread t holds
+// columns C g + TRC_CPT (t mod 8) + i (i < TRC_CPT) of chunk t / 8.  Band carries, [6][nb][W]
+// doubles from tri_off: the band's end w, pivot product, first-row x and unit response (written by
+// TRANSFER), its incoming w and x (written by k_col_trc_carry).  Columns from W to the next
+// multiple of C lie in the row padding (the pitch is a multiple of 64): they are solved with
+// meaningless values that no reader uses.
 template <int L, bool TRANSFER>
-__global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__restrict__ T, int guard) {
+__global__ __launch_bounds__(TRC_THREADS, 4) void k_col_trc(Ctx c, float *__restrict__ T, int guard) {
   const int *start = TRANSFER ? c.geo.tritr : c.geo.coltri[(L - TRC_L_MIN) / TRC_L_STEP];
-  const int img = find_img(start, c.n, blockIdx.x);
+  const int gb = xcd_block(blockIdx.x, gridDim.x);  // neighbouring column groups share row lines
+  const int img = find_img(start, c.n, gb);
   const ImgDesc d = c.img[img];
   if (guard_off(&c.st[img], guard)) return;
   const int H = d.H, W = d.W;
   const unsigned P = d.P;
   float *__restrict__ base = T + d.off;
-  const int bi = (int)(blockIdx.x - start[img]);
+  const int bi = gb - start[img];
   const int ngrp = (W + TRC_COLS - 1) / TRC_COLS;
   const int grp = bi % ngrp, band = bi / ngrp;
   const int nb = d.tri_nb;
   const int tid = (int)threadIdx.x, qd = tid & 7, ch = tid >> 3;
-  const int k0 = grp * TRC_COLS + 4 * qd;  // the thread's first column
+  const int k0 = grp * TRC_COLS + TRC_CPT * qd;  // the thread's first column
   const int r0 = band * TRC_CH * L, rend = min(H, r0 + TRC_CH * L);
   const int s = r0 + ch * L;
   const int len = max(0, min(L, rend - s));
@@ -1253,17 +1267,15 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
   // the chunk's rows (float4 of the thread's four columns; the row pitch is a multiple of 64 and
   // k0 of 4, so every access is 16-byte aligned and inside the row).  A short chunk re-reads its
   // last row, an empty one row H - 1.
-  float4 *__restrict__ rowp = reinterpret_cast<float4 *>(base + (size_t)sc * P + k0);
-  const size_t P4 = P / 4;
+  TrcVec *__restrict__ rowp = reinterpret_cast<TrcVec *>(base + (size_t)sc * P + k0);
+  const size_t P4 = P / TRC_CPT;
   const bool full = !__ballot(!(len == L && s + L < H));
-  float v[L][4];
+  float v[L][TRC_CPT];
 #pragma unroll
   for (int u = 0; u < L; ++u) {
-    const float4 q = rowp[(size_t)min(u, lastu) * P4];
-    v[u][0] = q.x;
-    v[u][1] = q.y;
-    v[u][2] = q.z;
-    v[u][3] = q.w;
+    const TrcVec q = rowp[(size_t)min(u, lastu) * P4];
+#pragma unroll
+    for (int i = 0; i < TRC_CPT; ++i) v[u][i] = q[i];
   }
   // column 0's input for k_col_trc_dc (the sweeps below overwrite it)
   if (!TRANSFER && k0 == 0) {
@@ -1274,24 +1286,24 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
   }
   __syncthreads();
   // the pivots of the thread's four columns, copied to registers once (the sweeps read no LDS)
-  double em[4], pq[4], peq2[4], plast[4], pth[4], ep[4];
+  double em[TRC_CPT], pq[TRC_CPT], peq2[TRC_CPT], plast[TRC_CPT], pth[TRC_CPT], ep[TRC_CPT];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    em[i] = sPiv[0][4 * qd + i];
-    pq[i] = sPiv[1][4 * qd + i];
-    peq2[i] = sPiv[2][4 * qd + i];
-    plast[i] = sPiv[3][4 * qd + i];
-    pth[i] = sPiv[4][4 * qd + i];
-    ep[i] = sPiv[5][4 * qd + i];
+  for (int i = 0; i < TRC_CPT; ++i) {
+    em[i] = sPiv[0][TRC_CPT * qd + i];
+    pq[i] = sPiv[1][TRC_CPT * qd + i];
+    peq2[i] = sPiv[2][TRC_CPT * qd + i];
+    plast[i] = sPiv[3][TRC_CPT * qd + i];
+    pth[i] = sPiv[4][TRC_CPT * qd + i];
+    ep[i] = sPiv[5][TRC_CPT * qd + i];
   }
   // every chunk of the wave complete, without row H - 1, and t_s <= 2^-60 in every column of
   // every lane: a_i = e^-theta on the whole chunk, branch-free sweeps; otherwise every row is tested
   bool cl = true;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) cl = cl && pth[i] * (2.0 * sc + 1.0) > 60.0 * M_LN2;
+  for (int i = 0; i < TRC_CPT; ++i) cl = cl && pth[i] * (2.0 * sc + 1.0) > 60.0 * M_LN2;
   const bool conv = full && !__ballot(!cl);
 
-  // chaining layout: thread t takes column t / TRC_SEG and chunks 4 (t mod TRC_SEG) .. + 3
+  // chaining layout: thread t takes column t / TRC_SEG and chunks TRC_PER (t mod TRC_SEG) .. + TRC_PER - 1
   const int cc = tid / TRC_SEG, sg = tid % TRC_SEG;
   const int kc = min(grp * TRC_COLS + cc, W - 1);
   const bool actc = grp * TRC_COLS + cc < W && grp * TRC_COLS + cc > 0;
@@ -1299,7 +1311,12 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
   // The sweeps branch per wave and row (converged or general); every barrier and chaining step
   // below is reached by all threads of the block.
   {
-    double Pc[4] = {1.0, 1.0, 1.0, 1.0}, Rc[4] = {0.0, 0.0, 0.0, 0.0};
+    double Pc[TRC_CPT], Rc[TRC_CPT];
+#pragma unroll
+    for (int i = 0; i < TRC_CPT; ++i) {
+      Pc[i] = 1.0;
+      Rc[i] = 0.0;
+    }
     // Three sweeps, each reading what the previous one stored in v:
     //   F   w0 from a zero carry (stored), with the chunk's pivot product and, for TRANSFER, R;
     //   B   x0 from a zero carry (stored) on the true w = w0 + P_u w_in, x0 at the chunk's first
@@ -1310,10 +1327,10 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
     // Rows outer, the four columns interleaved, a scheduling barrier per row.  The pivot of a row
     // is e^-theta in a converged wave, else the closed form with t advanced per row; rows past a
     // short chunk's end are skipped.
-    double tf[4], tb[4];
+    double tf[TRC_CPT], tb[TRC_CPT];
     auto t_reset = [&]() {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < TRC_CPT; ++i) {
         tf[i] = 0.0;
         tb[i] = 0.0;
         if (!conv) {
@@ -1339,13 +1356,13 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
       t *= fw ? pq[i] : peq2[i];
       return a;
     };
-    auto sweep_f = [&](double (&w)[4]) {
+    auto sweep_f = [&](double (&w)[TRC_CPT]) {
       t_reset();
 #pragma unroll
       for (int u = 0; u < L; ++u) {
         if (conv) {
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
+          for (int i = 0; i < TRC_CPT; ++i) {
             w[i] = em[i] * ((double)v[u][i] + w[i]);
             v[u][i] = (float)w[i];
             if (TRANSFER) Rc[i] = fma(em[i] * Pc[i], Pc[i], Rc[i]);
@@ -1353,7 +1370,7 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
           }
         } else if (u < len) {
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
+          for (int i = 0; i < TRC_CPT; ++i) {
             double r;
             const double a = gpiv(u, i, true, r);
             w[i] = a * ((double)v[u][i] + w[i]);
@@ -1365,25 +1382,25 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
         __builtin_amdgcn_sched_barrier(0);
       }
     };
-    auto sweep_b = [&](const double (&win)[4], double (&x)[4]) {
+    auto sweep_b = [&](const double (&win)[TRC_CPT], double (&x)[TRC_CPT]) {
       t_reset();
       // ge = w_in (1 + t_s) e^-(u+1)theta at the current row u (converged: (1 + t_s) = 1)
-      double ge[4];
+      double ge[TRC_CPT];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < TRC_CPT; ++i)
         ge[i] = conv ? Pc[i] * win[i] : win[i] * (1.0 + tf[i]) * exp(-(double)len * pth[i]);
 #pragma unroll
       for (int u = L - 1; u >= 0; --u) {
         if (conv) {
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
+          for (int i = 0; i < TRC_CPT; ++i) {
             x[i] = fma(em[i], x[i], (double)v[u][i] + ge[i]);
             v[u][i] = (float)x[i];
             ge[i] *= ep[i];
           }
         } else if (u < len) {
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
+          for (int i = 0; i < TRC_CPT; ++i) {
             double r;
             const double a = gpiv(u, i, false, r);
             const double g = s + u == H - 1 ? Pc[i] * win[i] : ge[i] * r;
@@ -1395,52 +1412,54 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
         __builtin_amdgcn_sched_barrier(0);
       }
     };
-    auto sweep_c = [&](double (&qr)[4]) {
+    auto sweep_c = [&](double (&qr)[TRC_CPT]) {
       t_reset();
 #pragma unroll
       for (int u = L - 1; u >= 0; --u) {
         if (conv) {
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
+          for (int i = 0; i < TRC_CPT; ++i) {
             qr[i] *= em[i];
             v[u][i] += (float)qr[i];
           }
-          rowp[(size_t)u * P4] = make_float4(v[u][0], v[u][1], v[u][2], v[u][3]);
+          rowp[(size_t)u * P4] = trc_vec(v[u]);
         } else if (u < len) {
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
+          for (int i = 0; i < TRC_CPT; ++i) {
             double r;
             qr[i] *= gpiv(u, i, false, r);
             v[u][i] = (float)((double)v[u][i] + qr[i]);
           }
-          rowp[(size_t)u * P4] = make_float4(v[u][0], v[u][1], v[u][2], v[u][3]);
+          rowp[(size_t)u * P4] = trc_vec(v[u]);
         }
         __builtin_amdgcn_sched_barrier(0);
       }
     };
 
     // forward transfers, chained per column from the band's incoming w
-    double w[4] = {0.0, 0.0, 0.0, 0.0};
+    double w[TRC_CPT];
+#pragma unroll
+    for (int i = 0; i < TRC_CPT; ++i) w[i] = 0.0;
     sweep_f(w);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      sW[4 * qd + i][ch] = w[i];
-      sP[4 * qd + i][ch] = Pc[i];
-      if (TRANSFER) sR[4 * qd + i][ch] = Rc[i];
+    for (int i = 0; i < TRC_CPT; ++i) {
+      sW[TRC_CPT * qd + i][ch] = w[i];
+      sP[TRC_CPT * qd + i][ch] = Pc[i];
+      if (TRANSFER) sR[TRC_CPT * qd + i][ch] = Rc[i];
     }
     __syncthreads();
     {
-      // the thread's four chunks composed in order, the 32 segments scanned, then the incoming w
-      // of each of the four chunks
-      double a4[4], b4[4];
+      // the thread's TRC_PER chunks composed in order, the segments scanned, then the incoming w
+      // of each of its chunks
+      double a4[TRC_PER], b4[TRC_PER];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        a4[j] = sP[cc][4 * sg + j];
-        b4[j] = sW[cc][4 * sg + j];
+      for (int j = 0; j < TRC_PER; ++j) {
+        a4[j] = sP[cc][TRC_PER * sg + j];
+        b4[j] = sW[cc][TRC_PER * sg + j];
       }
       double A = a4[0], B = b4[0];
 #pragma unroll
-      for (int j = 1; j < 4; ++j) {
+      for (int j = 1; j < TRC_PER; ++j) {
         B = fma(a4[j], B, b4[j]);
         A *= a4[j];
       }
@@ -1450,9 +1469,9 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
       double wv = sg == 0 ? wext : fma(Ap, wext, Bp);
       double wu = sg == 0 ? 1.0 : Ap;  // response to a unit incoming w of the band (TRANSFER)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        sW[cc][4 * sg + j] = wv;
-        if (TRANSFER) sR[cc][4 * sg + j] *= wu;
+      for (int j = 0; j < TRC_PER; ++j) {
+        sW[cc][TRC_PER * sg + j] = wv;
+        if (TRANSFER) sR[cc][TRC_PER * sg + j] *= wu;
         wv = fma(a4[j], wv, b4[j]);
         wu *= a4[j];
       }
@@ -1464,27 +1483,29 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
     __syncthreads();
     // the chunk's incoming w, then its backward transfer on the true w
 #pragma unroll
-    for (int i = 0; i < 4; ++i) w[i] = sW[4 * qd + i][ch];
-    double x[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < TRC_CPT; ++i) w[i] = sW[TRC_CPT * qd + i][ch];
+    double x[TRC_CPT];
+#pragma unroll
+    for (int i = 0; i < TRC_CPT; ++i) x[i] = 0.0;
     sweep_b(w, x);
     __syncthreads();  // every thread has read its incoming w: sW now takes the x transfers
 #pragma unroll
-    for (int i = 0; i < 4; ++i) sW[4 * qd + i][ch] = x[i];
+    for (int i = 0; i < TRC_CPT; ++i) sW[TRC_CPT * qd + i][ch] = x[i];
     __syncthreads();
     {
       // backward chain: thread (column, segment 31 - sg) composes its four chunks from the last,
       // the scan runs from the band's last segment upwards; the unit response shares the products
-      const int g4 = 4 * (TRC_SEG - 1 - sg);
-      double a4[4], b4[4], u4[4];
+      const int g4 = TRC_PER * (TRC_SEG - 1 - sg);
+      double a4[TRC_PER], b4[TRC_PER], u4[TRC_PER];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < TRC_PER; ++j) {
         a4[j] = sP[cc][g4 + j];
         b4[j] = sW[cc][g4 + j];
         u4[j] = TRANSFER ? sR[cc][g4 + j] : 0.0;
       }
-      double A = a4[3], B = b4[3], U = u4[3];
+      double A = a4[TRC_PER - 1], B = b4[TRC_PER - 1], U = u4[TRC_PER - 1];
 #pragma unroll
-      for (int j = 2; j >= 0; --j) {
+      for (int j = TRC_PER - 2; j >= 0; --j) {
         B = fma(a4[j], B, b4[j]);
         if (TRANSFER) U = fma(a4[j], U, u4[j]);
         A *= a4[j];
@@ -1509,7 +1530,7 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
         xu = sg == 0 ? 0.0 : Up;
       }
 #pragma unroll
-      for (int j = 3; j >= 0; --j) {
+      for (int j = TRC_PER - 1; j >= 0; --j) {
         sW[cc][g4 + j] = xv;
         xv = fma(a4[j], xv, b4[j]);
         if (TRANSFER) xu = fma(a4[j], xu, u4[j]);
@@ -1522,7 +1543,7 @@ __global__ __launch_bounds__(TRC_THREADS, 2) void k_col_trc(Ctx c, float *__rest
     if (TRANSFER) return;
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 4; ++i) x[i] = sW[4 * qd + i][ch];
+    for (int i = 0; i < TRC_CPT; ++i) x[i] = sW[TRC_CPT * qd + i][ch];
     sweep_c(x);
   }
 }
