@@ -697,14 +697,14 @@ static int build_layout(int n, const int64_t *H, const int64_t *W, Layout &Lo) {
     d.colslab = want_slab ? add_slab(h) : -1;
     d.nslab = (w + 2 * SLAB_CP - 1) / (2 * SLAB_CP);
     const bool slab = d.colslab >= 0;
-    // tridiagonal column solves (16 B/px of memory traffic) when this image's own column transform
-    // would be Bluestein (four length-M >= 2H - 1 FFTs per column pair) or a long two-pass
-    // transform with its transposes, or for every image in mode 4 (measured on MI355X: 48-burst
-    // stack +24%, 40000^2 +32% per solve); column DCTs otherwise: LDS-resident direct on the
-    // row-major field in mode 2, on the transposed copy in the others (direct is +16% per PCG
-    // iteration at 8192^2; one thread per column has too few threads in flight for square images)
+    // tridiagonal column solves (8 B/px of memory traffic, 12 with bands) for every image off the
+    // slab path in automatic mode and in mode 4: cheaper than a column DCT pair in every measured
+    // case on MI355X (Bluestein columns of the 48-burst stack, long columns of 40000^2, and the
+    // transposes + LDS column DCTs of 8192^2: 397 vs 490 us per PCG iteration); in mode 3 for the
+    // heights without a slab split that would need Bluestein or a long two-pass transform.  Column
+    // DCTs in modes 1 and 2: on the transposed copy, or direct on the row-major field.
     const bool hblue = !lh && Lo.plans[d.colplan].blue;
-    d.coltri = !slab && (g_col_mode == 4 || ((g_col_mode == 0 || g_col_mode == 3) && (hblue || lh)));
+    d.coltri = !slab && (g_col_mode == 4 || g_col_mode == 0 || (g_col_mode == 3 && (hblue || lh)));
     d.collong = (lh && !slab && !d.coltri) ? add_long(h) : -1;
     const bool colk = !slab && !lh && !d.coltri;
     d.coldirect = colk && g_col_mode == 2;

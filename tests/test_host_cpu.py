@@ -45,11 +45,11 @@ def test_static_scratch_size_matches_header(lib):
 
 def test_column_strategy_is_per_image(lib):
     """Each image's column strategy follows from its own shape, the same alone and in any batch:
-    power-of-two-split heights take the register slab passes (one launch per split), Bluestein
-    heights the tridiagonal column solves, the rest the transposes."""
+    power-of-two-split heights take the register slab passes (one launch per split), the others
+    the tridiagonal column solves."""
     shapes = [(1024, 4000), (1030, 5000), (1000, 4500), (2048, 300), (96, 130), (103, 64)]
     alone = [lib.describe_columns([sh])[0] for sh in shapes]
-    assert alone == ["slab_register", "tridiag", "transpose", "slab_register", "transpose", "tridiag"]
+    assert alone == ["slab_register", "tridiag", "tridiag", "slab_register", "tridiag", "tridiag"]
     assert lib.describe_columns(shapes) == alone
     assert lib.describe_columns(shapes[::-1]) == alone[::-1]
 
@@ -238,8 +238,8 @@ def test_autotune_candidates_are_valid(lib):
 def test_column_strategies(lib):
     """Which column pass each image gets, from its own shape.  Automatic mode: the register-stage
     slab kernels for heights from 256 on whose split is a power-of-two pair (one launch per split);
-    the transposed pass for other heights (the batched-LDS slab kernels are slower), tridiagonal
-    solves for Bluestein heights and above 12288.
+    tridiagonal solves for every other height (the batched-LDS slab kernels and the transposed
+    column DCTs are slower).
     Slab mode: every 5-smooth height within the block limits (register kernels where the split
     allows them)."""
     d = lib.describe_columns
@@ -248,14 +248,14 @@ def test_column_strategies(lib):
         assert d([(8192, 8192)]) == ["slab_register"]
         assert d([(8192, 70), (8192, 3)]) == ["slab_register"] * 2
         assert d([(256, 256)]) == ["slab_register"]
-        assert d([(8192, 70), (1000, 7)]) == ["slab_register", "transpose"]  # 1000: no power-of-two split
+        assert d([(8192, 70), (1000, 7)]) == ["slab_register", "tridiag"]  # 1000: no power-of-two split
         assert d([(8192, 70), (2048, 9), (8192, 5)]) == ["slab_register"] * 3  # two splits, two launches
         assert d([(2048, 9), (8192, 70)]) == ["slab_register"] * 2
-        assert d([(6000, 6000)]) == ["transpose"]
-        assert d([(96, 130)]) == ["transpose"]                     # below the slab threshold
+        assert d([(6000, 6000)]) == ["tridiag"]
+        assert d([(96, 130)]) == ["tridiag"]                       # below the slab threshold
         assert d([(127, 211)]) == ["tridiag"]                      # Bluestein columns
         assert d([(12800, 8)]) == ["tridiag"]                      # long columns: tridiagonal solves
-        assert d([(1000, 12500)]) == ["transpose"]
+        assert d([(1000, 12500)]) == ["tridiag"]
         assert d([(8192, 70), (127, 211), (12800, 8)]) == ["slab_register", "tridiag", "tridiag"]
         lib.set_column_mode(lib.COLUMN_SLAB)
         assert d([(64, 48), (96, 130)]) == ["slab", "slab"]
@@ -272,8 +272,8 @@ def test_column_strategies(lib):
         lib.set_column_mode(lib.COLUMN_TRIDIAG)
         assert d([(127, 211), (8192, 70), (96, 130), (12800, 8)]) == ["tridiag"] * 4
         lib.set_column_mode(lib.COLUMN_AUTO)
-        # columns with a prime factor above 5 or above 12288: tridiagonal solves
-        assert d([(13001, 9), (7001, 9), (12800, 8), (9, 7001)]) == ["tridiag", "tridiag", "tridiag", "transpose"]
+        # columns with a prime factor above 5, above 12288, or short: tridiagonal solves
+        assert d([(13001, 9), (7001, 9), (12800, 8), (9, 7001)]) == ["tridiag"] * 4
     finally:
         lib.set_column_mode(lib.COLUMN_AUTO)
 
