@@ -210,7 +210,7 @@ struct Geo {
   const int *browP, *brow1, *brow2, *bcolP, *bcol1, *bcol2;
   // tridiagonal column solves, TRC_COLS columns x one band per block: the final solves per chunk
   // length (4, 6, 8, 10, 12 rows), the band transfers, the band carry chains, column 0
-  const int *coltri[5], *tritr, *tricarry, *tridc;
+  const int *coltri[9], *tritr, *tricarry, *tridc;  // coltri: one table per chunk length class
 };
 
 struct Ctx {

@@ -14,6 +14,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <utility>
 #include <vector>
 
 #include "irls2d.h"
@@ -1104,6 +1105,18 @@ static void row_dct3(const Layout &Lo, const Ctx &c, const float *src, float *ds
   }
 }
 
+// The final tridiagonal launches, chunk length class Q (L = TRC_L_MIN + Q TRC_L_STEP).
+template <int Q>
+static void tri_final_launch(const Layout &Lo, const Ctx &c, int guard, hipStream_t s) {
+  if (Lo.geo[G_TRI + Q].back() > 0)
+    LAUNCH("k_col_trc", (k_col_trc<TRC_L_MIN + Q * TRC_L_STEP, false>), dim3(Lo.geo[G_TRI + Q].back()),
+           dim3(TRC_THREADS), trc_lds_bytes(false), s, c, c.f.T, guard);
+}
+template <int... Q>
+static void tri_final_launches(const Layout &Lo, const Ctx &c, int guard, hipStream_t s, std::integer_sequence<int, Q...>) {
+  (tri_final_launch<Q>(Lo, c, guard, s), ...);
+}
+
 static void col_pass(const Layout &Lo, const Ctx &c, int guard, hipStream_t s) {
   for (const LaunchClass &C : Lo.slabcls) {
     const int nb1 = Lo.geo[C.geo].back(), nb2 = Lo.geo[C.geo2].back();
@@ -1152,17 +1165,8 @@ static void col_pass(const Layout &Lo, const Ctx &c, int guard, hipStream_t s) {
            c.f.T, guard);
     LAUNCH("k_col_trc_carry", k_col_trc_carry, dim3(Lo.geo[G_TRIC].back()), dim3(256), 0, s, c, guard);
   }
-  auto tri = [&](int q, auto kern) {
-    if (Lo.geo[G_TRI + q].back() > 0)
-      LAUNCH("k_col_trc", kern, dim3(Lo.geo[G_TRI + q].back()), dim3(TRC_THREADS), trc_lds_bytes(false), s, c,
-             c.f.T, guard);
-  };
-  static_assert(TRC_NL == 5, "one launch per chunk length");
-  tri(0, k_col_trc<TRC_L_MIN, false>);
-  tri(1, k_col_trc<TRC_L_MIN + TRC_L_STEP, false>);
-  tri(2, k_col_trc<TRC_L_MIN + 2 * TRC_L_STEP, false>);
-  tri(3, k_col_trc<TRC_L_MIN + 3 * TRC_L_STEP, false>);
-  tri(4, k_col_trc<TRC_L, false>);
+  // one launch per chunk length present in the batch
+  tri_final_launches(Lo, c, guard, s, std::make_integer_sequence<int, TRC_NL>{});
   if (Lo.geo[G_TRIDC].back() > 0)
     LAUNCH("k_col_trc_dc", k_col_trc_dc, dim3(Lo.geo[G_TRIDC].back()), dim3(64), 0, s, c, c.f.T, guard);
   for (const LaunchClass &C : Lo.colcls) {
@@ -1331,6 +1335,12 @@ static int get_layout(int n, const int64_t *H, const int64_t *W, std::shared_ptr
 
 static bool g_attr_done[64];
 
+// every k_col_trc instantiation: the final launches of each chunk length, then the band pass
+template <int... Q>
+static std::vector<const void *> tri_kernels(std::integer_sequence<int, Q...>) {
+  return {(const void *)k_col_trc<TRC_L_MIN + Q * TRC_L_STEP, false>..., (const void *)k_col_trc<TRC_L, true>};
+}
+
 static int set_attrs() {
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
@@ -1355,11 +1365,7 @@ static int set_attrs() {
   const int slds = (TW_LDS + SLAB_E * SLAB_MAX_THREADS + 2 * SLAB_MAX_THREADS) * (int)sizeof(float2);
   const void *sfns[] = {(const void *)k_slab1<SLAB_E>, (const void *)k_slab2<SLAB_E>, (const void *)k_slab3<SLAB_E>};
   for (const void *f : sfns) HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, slds));
-  const void *tfns[] = {(const void *)k_col_trc<TRC_L_MIN, false>, (const void *)k_col_trc<TRC_L_MIN + TRC_L_STEP, false>,
-                        (const void *)k_col_trc<TRC_L_MIN + 2 * TRC_L_STEP, false>,
-                        (const void *)k_col_trc<TRC_L_MIN + 3 * TRC_L_STEP, false>, (const void *)k_col_trc<TRC_L, false>,
-                        (const void *)k_col_trc<TRC_L, true>};
-  for (const void *f : tfns)
+  for (const void *f : tri_kernels(std::make_integer_sequence<int, TRC_NL>{}))
     HIP_TRY(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)trc_lds_bytes(true)));
   if (dev >= 0 && dev < 64) g_attr_done[dev] = true;
   return IRLS2D_OK;

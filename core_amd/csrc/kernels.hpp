@@ -1092,6 +1092,7 @@ constexpr int TRC_L = 24;        // rows per chunk of a band: bands of 1536 rows
 constexpr int TRC_L_MIN = 8;
 constexpr int TRC_L_STEP = 4;
 constexpr int TRC_NL = (TRC_L - TRC_L_MIN) / TRC_L_STEP + 1;
+static_assert(TRC_NL <= 9, "Geo::coltri holds one table per chunk length class");
 constexpr double TRI_TSMALL = 0x1p-60;
 // dynamic LDS bytes of k_col_trc: pivot constants (TRC_NPIV per column), then 2 (3 with the band
 // transfers) arrays of TRC_COLS x (TRC_CH + 1) doubles
