@@ -417,22 +417,27 @@ __global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
   // folds the DCT-III pre-step's factor 1/2 into the reciprocal.
   const int tstep = N / N2;
   {
-    // exp(-2 pi i m / N) from sincospif: N is a power of two here, so 2 m / N is exact in float32
-    // (no table read before the first barrier)
+    // exp(-2 pi i m / N) from sincospif: N is a power of two here, so 2 m / N = 2 m invN is exact
+    // in float32 (no division; no table read before the first barrier)
+    const float invN = L.invN;
     auto expi = [&](int m) {
       float sn, cs;
-      sincospif(2.f * (float)m / (float)N, &sn, &cs);
+      sincospif((float)(2 * m) * invN, &sn, &cs);
       return make_float2(cs, -sn);
     };
     const float2 *mk = c.tab + L.mk_off;
     const double *cosH = c.dtab + b.d->cosH_off;
-    for (int e = threadIdx.x; e < 2 * N2; e += blockDim.x) {
+    // 4 N2 threads: the first 2 N2 build the per-set tables, the next N2 the twiddles rt, so that
+    // no thread evaluates two sincospif
+    const int e = threadIdx.x;
+    if (e < 2 * N2) {
       const int kk = e & (N2 - 1), k1e = e < N2 ? k1a : k1b;
       const int k = k1e + N1 * kk;
       mkT[e] = mk[k];
       sT[e] = (float)(4.0 - 4.0 * cosH[k]);
       ft[e] = expi(k1e * kk);
-      if (e < N2) rt[e] = expi(e * tstep);
+    } else if (e < 3 * N2) {
+      rt[e - 2 * N2] = expi((e - 2 * N2) * tstep);
     }
   }
   __syncthreads();
