@@ -465,7 +465,8 @@ __global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
   // k1 = N1 / 2 at N2 - 1 - kk
   const int ja = 2 * (b.slab * SLAB_CP + cp);
   const bool hasa = live && ja < W, hasb = hasa && ja + 1 < W;
-  const float swa = hasa ? (float)(4.0 - 4.0 * c.dtab[b.d->cosW_off + ja]) : 0.f;
+  // a column pair past the image's edge (no line a) computes finite values that no one reads
+  const float swa = hasa ? (float)(4.0 - 4.0 * c.dtab[b.d->cosW_off + ja]) : 1.f;
   const float swb = hasb ? (float)(4.0 - 4.0 * c.dtab[b.d->cosW_off + ja + 1]) : 0.f;
   const int sp = two ? (s ^ SLAB_CP) : s;
   const int ts = sig ? N2 : 0;
@@ -482,11 +483,12 @@ __global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
       const float sk = sT[ts + kk];
       const float xa = wk.x * (Zk.x + Zn.x) - wk.y * (Zk.y - Zn.y);
       const float xb = wk.x * (Zk.y + Zn.y) + wk.y * (Zk.x - Zn.x);
-      const bool dc = ja == 0 && k1 == 0 && kk == 0;
-      o[i * R + k2] = make_float2((hasa && !dc) ? xa * __builtin_amdgcn_rcpf(sk + swa) : 0.f,
-                                  hasb ? xb * __builtin_amdgcn_rcpf(sk + swb) : 0.f);
+      // line b past the edge must be zero: it shares the complex sequence with line a
+      o[i * R + k2] = make_float2(xa * __builtin_amdgcn_rcpf(sk + swa), hasb ? xb * __builtin_amdgcn_rcpf(sk + swb) : 0.f);
     }
   }
+  // the image's DC coefficient (k = 0 of column 0, eigenvalue 0): element kk = 0 of thread t = 0
+  if (ja == 0 && k1 == 0 && t == 0) o[0].x = 0.f;
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < Q; ++i)
@@ -528,11 +530,10 @@ __global__ __launch_bounds__(256) void k_slab2_f(Ctx c, int guard) {
   Dft<16>::run(v);
   if (!live) return;
   float2 *So = const_cast<float2 *>(Sl);
+  // ft = exp(0) = (1, -0) where k1 = 0 or m = 0: the product keeps the value (up to the sign of a
+  // zero), so no select
 #pragma unroll
-  for (int mm = 0; mm < 16; ++mm) {
-    const int m = t + R * mm;
-    So[mm * rs] = (k1 && m) ? cmul(v[mm], ft[ts + m]) : v[mm];
-  }
+  for (int mm = 0; mm < 16; ++mm) So[mm * rs] = cmul(v[mm], ft[ts + t + R * mm]);
 }
 
 }  // namespace irls2d
