@@ -1176,6 +1176,16 @@ __device__ void tri_singular(const float *__restrict__ src, float *__restrict__ 
   }
 }
 
+// x^N for a compile-time N >= 1 by repeated squaring.
+template <int N>
+__device__ __forceinline__ double ipow(double x) {
+  if constexpr (N == 1) return x;
+  else {
+    const double h = ipow<N / 2>(x);
+    return (N & 1) ? h * h * x : h * h;
+  }
+}
+
 // A pointer the compiler can keep in scalar registers (every lane of the wave holds the same one).
 template <typename F>
 __device__ __forceinline__ F *uniform_ptr(F *p) {
@@ -1366,8 +1376,10 @@ __global__ __launch_bounds__(TRC_THREADS, 4) void k_col_trc(Ctx c, float *__rest
           for (int i = 0; i < TRC_CPT; ++i) {
             w[i] = em[i] * ((double)v[u][i] + w[i]);
             v[u][i] = (float)w[i];
-            if (TRANSFER) Rc[i] = fma(em[i] * Pc[i], Pc[i], Rc[i]);
-            Pc[i] *= em[i];
+            if (TRANSFER) {  // R needs the running product; otherwise e^-L theta follows the loop
+              Rc[i] = fma(em[i] * Pc[i], Pc[i], Rc[i]);
+              Pc[i] *= em[i];
+            }
           }
         } else if (u < len) {
 #pragma unroll
@@ -1381,6 +1393,10 @@ __global__ __launch_bounds__(TRC_THREADS, 4) void k_col_trc(Ctx c, float *__rest
           }
         }
         __builtin_amdgcn_sched_barrier(0);
+      }
+      if (!TRANSFER && conv) {  // a converged chunk is full: the product of its L pivots e^-theta
+#pragma unroll
+        for (int i = 0; i < TRC_CPT; ++i) Pc[i] = ipow<L>(em[i]);
       }
     };
     auto sweep_b = [&](const double (&win)[TRC_CPT], double (&x)[TRC_CPT]) {
