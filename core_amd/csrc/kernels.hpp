@@ -1312,7 +1312,8 @@ __global__ __launch_bounds__(TRC_THREADS, 4) void k_col_trc(Ctx c, float *__rest
   bool cl = true;
 #pragma unroll
   for (int i = 0; i < TRC_CPT; ++i) cl = cl && pth[i] * (2.0 * sc + 1.0) > 60.0 * M_LN2;
-  const bool conv = full && !__ballot(!cl);
+  // wave-uniform, kept in a scalar register: the per-row branches below test it without VALU work
+  const bool conv = __builtin_amdgcn_readfirstlane((int)(full && !__ballot(!cl))) != 0;
 
   // chaining layout: thread t takes column t / TRC_SEG and chunks TRC_PER (t mod TRC_SEG) .. + TRC_PER - 1
   const int cc = tid / TRC_SEG, sg = tid % TRC_SEG;
