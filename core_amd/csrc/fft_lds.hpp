@@ -419,16 +419,18 @@ template <int E>
 __device__ __forceinline__ void dct_fft(const DctCtx &d) {
   fft_lds<E>(d.buf, d.tws, &d.P->fft);
   if (d.P->blue) {
-    // pointwise product with the transformed chirp; a thread loads BLUE_U of its values before it
-    // uses any (one memory latency per batch; index clamped so that every load is unconditional)
+    // pointwise product with the transformed chirp; a thread loads BU of its values before it uses
+    // any (one memory latency per batch; index clamped so that every load is unconditional).  The
+    // 12-element kernels take all of theirs in one batch (M <= 12 T).
+    constexpr int BU = E == 12 ? 12 : BLUE_U;
     const float2 *__restrict__ bh = d.tab + d.P->bhat_off;
     const unsigned M = d.P->M, T = blockDim.x;
-    for (unsigned k0 = threadIdx.x; k0 < M; k0 += BLUE_U * T) {
-      float2 b[BLUE_U];
+    for (unsigned k0 = threadIdx.x; k0 < M; k0 += BU * T) {
+      float2 b[BU];
 #pragma unroll
-      for (int u = 0; u < BLUE_U; ++u) b[u] = bh[min(k0 + u * T, M - 1)];
+      for (int u = 0; u < BU; ++u) b[u] = bh[min(k0 + u * T, M - 1)];
 #pragma unroll
-      for (int u = 0; u < BLUE_U; ++u) {
+      for (int u = 0; u < BU; ++u) {
         const unsigned k = k0 + u * T;
         if (k < M) {
           const int pk = pidx(k);
