@@ -707,7 +707,8 @@ __global__ __launch_bounds__(fft_threads_max<E>()) void k_row_dct2(Ctx c, float 
       constexpr bool BLUE = decltype(blue_tag)::value;
       constexpr bool DEFER = decltype(defer_tag)::value;
       const float2 *__restrict__ ch = D.tab + D.P->chirp_off;
-      constexpr int U = DEFER ? 8 : 4;
+      // Bluestein rows of 4000-6000 points on 896-1024 threads: 6 per batch is all of them
+      constexpr int U = DEFER ? 8 : (BLUE ? 6 : 4);
       for (int m0 = threadIdx.x; m0 < W; m0 += U * T) {
         float xv[U][2], pv[U][2], rv[U][2], av[U][2];
         float2 cm[U];
